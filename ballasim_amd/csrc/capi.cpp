@@ -1,0 +1,259 @@
+// The C ABI of include/bsx/bsx_sensor.h over bsx::SensorModel: exceptions become status codes with
+// a per-thread message (std::invalid_argument -> BSX_E_INVALID, std::logic_error -> BSX_E_LOGIC,
+// anything else -> BSX_E_RUNTIME), as the reference's Python layer maps them to ValueError,
+// RuntimeError and so on (src/sensor/sensor_module.cpp).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/bsx/bsx_sensor.h"
+#include "sensor/model.h"
+#include "sensor/params.h"
+
+struct bsx_sensor {
+  bsx::SensorModel* model;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return BSX_OK;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return BSX_E_INVALID;
+  } catch (const std::logic_error& e) {
+    g_last_error = e.what();
+    return BSX_E_LOGIC;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return BSX_E_RUNTIME;
+  } catch (...) {
+    g_last_error = "bsx: unknown error";
+    return BSX_E_RUNTIME;
+  }
+}
+
+bsx::SensorModel& model_of(bsx_sensor* s) {
+  if (s == nullptr || s->model == nullptr) throw std::invalid_argument("bsx: null sensor handle");
+  return *s->model;
+}
+
+void need_ptr(const void* p, const char* what) {
+  if (p == nullptr) throw std::invalid_argument(std::string("bsx: null ") + what);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsx_last_error(char* buf, size_t n) {
+  if (buf != nullptr && n > 0) {
+    const size_t k = std::min(n - 1, g_last_error.size());
+    std::memcpy(buf, g_last_error.data(), k);
+    buf[k] = '\0';
+  }
+  return BSX_OK;
+}
+
+int bsx_device_count(int32_t* out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *out = n;
+  });
+}
+
+int bsx_params_preset(const char* name, bsx_params* out) {
+  return guarded([&] {
+    need_ptr(name, "name");
+    need_ptr(out, "out");
+    *out = bsx::preset(name);
+  });
+}
+
+int bsx_preset_names(char* buf, size_t n) {
+  return guarded([&] {
+    std::string all;
+    for (const std::string& s : bsx::preset_names()) all += (all.empty() ? "" : ",") + s;
+    if (buf != nullptr && n > 0) {
+      const size_t k = std::min(n - 1, all.size());
+      std::memcpy(buf, all.data(), k);
+      buf[k] = '\0';
+    }
+  });
+}
+
+int bsx_params_validate(const bsx_params* p) {
+  return guarded([&] {
+    need_ptr(p, "params");
+    bsx::validate(*p);
+  });
+}
+
+int bsx_sensor_create(const bsx_params* params, uint32_t width, uint32_t height, uint64_t seed,
+                      const bsx_waveform* light_groups, int32_t n_groups, uint32_t camera, bsx_sensor** out) {
+  return guarded([&] {
+    need_ptr(params, "params");
+    need_ptr(out, "out");
+    *out = nullptr;
+    if (n_groups < 0 || (n_groups > 0 && light_groups == nullptr)) {
+      throw std::invalid_argument("bsx_sensor_create: light_groups must hold n_groups waveforms");
+    }
+    std::vector<bsx_waveform> groups(light_groups, light_groups + n_groups);
+    auto* m = new bsx::SensorModel(*params, width, height, seed, std::move(groups), camera);
+    *out = new bsx_sensor{m};
+  });
+}
+
+int bsx_sensor_destroy(bsx_sensor* s) {
+  return guarded([&] {
+    if (s == nullptr) return;
+    delete s->model;
+    delete s;
+  });
+}
+
+int bsx_sensor_set_photometry(bsx_sensor* s, double illuminance_scale, const float* gain, size_t n_gain) {
+  return guarded([&] { model_of(s).set_photometry(illuminance_scale, gain, n_gain); });
+}
+
+int bsx_sensor_init(bsx_sensor* s, int64_t t0_ns, const float* irradiance0) {
+  return guarded([&] {
+    need_ptr(irradiance0, "irradiance0");
+    model_of(s).init(t0_ns, irradiance0);
+  });
+}
+
+int bsx_sensor_advance(bsx_sensor* s, const uint32_t* pixel_ids, size_t n, const int64_t* t, const float* irr,
+                       size_t m, const int32_t* counts, int64_t t_end_ns) {
+  return guarded([&] {
+    if (n > 0) need_ptr(pixel_ids, "pixel_ids");
+    if (n * m > 0) {
+      need_ptr(t, "t");
+      need_ptr(irr, "irr");
+    }
+    model_of(s).advance(pixel_ids, n, t, irr, m, counts, t_end_ns);
+  });
+}
+
+int bsx_sensor_advance_runs(bsx_sensor* s, const uint32_t* pixel_ids, size_t n, const int64_t* t, const float* irr,
+                            size_t n_samples, const int64_t* row_start, const int32_t* row_count, int64_t t_end_ns) {
+  return guarded([&] {
+    if (n > 0) {
+      need_ptr(pixel_ids, "pixel_ids");
+      need_ptr(row_start, "row_start");
+      need_ptr(row_count, "row_count");
+    }
+    if (n_samples > 0) {
+      need_ptr(t, "t");
+      need_ptr(irr, "irr");
+    }
+    model_of(s).advance_runs(pixel_ids, n, t, irr, n_samples, row_start, row_count, t_end_ns);
+  });
+}
+
+int bsx_sensor_advance_device(bsx_sensor* s, const bsx_device_runs* runs, int64_t t_end_ns, void* stream,
+                              bsx_device_events* out) {
+  return guarded([&] {
+    need_ptr(runs, "runs");
+    need_ptr(out, "out");
+    *out = model_of(s).advance_device(*runs, t_end_ns, static_cast<hipStream_t>(stream));
+  });
+}
+
+int bsx_sensor_download_events(bsx_sensor* s, const bsx_device_events* events) {
+  return guarded([&] {
+    need_ptr(events, "events");
+    model_of(s).download_events(*events);
+  });
+}
+
+int bsx_sensor_events(bsx_sensor* s, bsx_events* out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    const bsx::HostEvents& e = model_of(s).events();
+    out->count = e.size();
+    out->x = e.x.data();
+    out->y = e.y.data();
+    out->p = e.p.data();
+    out->t_cross_ns = e.t_cross.data();
+    out->t_reported_ns = e.t_reported.data();
+    out->label = e.label.data();
+    out->flags = e.flags.data();
+    out->ds = e.ds.data();
+    out->dn = e.dn.data();
+    out->dl = e.dl.data();
+    out->l_first = e.l_first.data();
+  });
+}
+
+int bsx_sensor_state_bytes(bsx_sensor* s, size_t* out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    *out = model_of(s).state_bytes();
+  });
+}
+
+int bsx_sensor_save_state(bsx_sensor* s, uint8_t* buf, size_t n) {
+  return guarded([&] {
+    need_ptr(buf, "buf");
+    model_of(s).save_state(buf, n);
+  });
+}
+
+int bsx_sensor_load_state(bsx_sensor* s, const uint8_t* buf, size_t n) {
+  return guarded([&] {
+    need_ptr(buf, "buf");
+    model_of(s).load_state(buf, n);
+  });
+}
+
+int bsx_sensor_stats(bsx_sensor* s, bsx_stats* out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    *out = model_of(s).stats();
+  });
+}
+
+int bsx_sensor_reset_stats(bsx_sensor* s) {
+  return guarded([&] { model_of(s).reset_stats(); });
+}
+
+int bsx_sensor_set_launch_policy(bsx_sensor* s, const bsx_launch_policy* policy) {
+  return guarded([&] {
+    need_ptr(policy, "policy");
+    model_of(s).set_policy(*policy);
+  });
+}
+
+int bsx_sensor_info(bsx_sensor* s, bsx_sensor_info_t* out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    *out = model_of(s).info();
+  });
+}
+
+int bsx_sensor_maps(bsx_sensor* s, double* theta_on, double* theta_off, double* leak, double* dark_fA, double* phi0,
+                    uint8_t* hot) {
+  return guarded([&] {
+    const bsx::SensorMaps& m = model_of(s).maps();
+    const size_t n = m.size();
+    if (theta_on != nullptr) std::memcpy(theta_on, m.theta_on.data(), n * 8);
+    if (theta_off != nullptr) std::memcpy(theta_off, m.theta_off.data(), n * 8);
+    if (leak != nullptr) std::memcpy(leak, m.leak.data(), n * 8);
+    if (dark_fA != nullptr) std::memcpy(dark_fA, m.dark_fA.data(), n * 8);
+    if (phi0 != nullptr) std::memcpy(phi0, m.phi0.data(), n * 8);
+    if (hot != nullptr) std::memcpy(hot, m.hot.data(), n);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Device-side data of one sensor model: the call constants, the per-pixel maps and the per-pixel
+// working state, all structure-of-arrays so that a wave of 64 lanes (64 consecutive pixels) reads
+// and writes each field with one coalesced 256-byte (fp32) or 512-byte (int64) access.
+#pragma once
+
+#include "../core/hd.h"
+
+namespace bsx {
+
+inline constexpr int kMaxGroups = 8;
+inline constexpr int64_t kNotBlind = INT64_MIN;
+inline constexpr int kPlainSegmentLog2 = 23;  // pieces never cross a multiple of 2^23 ns
+inline constexpr int64_t kPlainSegmentNs = int64_t{1} << kPlainSegmentLog2;
+inline constexpr int64_t kNoBreakpoint = INT64_MAX;
+
+// Event labels and flag bits (reference: src/sensor/pixel.h:70-78).
+inline constexpr uint8_t kLabelSignal = 0, kLabelNoise = 1, kLabelLeak = 2, kLabelHot = 3;
+inline constexpr uint16_t kFlagQuarantined = 1u << 9;
+inline constexpr uint16_t kFlagSolverFallback = 1u << 10;
+inline constexpr uint16_t kFlagFirstAfterInit = 1u << 12;
+// This library's flag for events of the rate-specified shot-noise process.
+inline constexpr uint16_t kFlagShot = 1u << 13;
+
+// Constants of one call of the fp32 kernel (kernel units: fA, s, log-e).
+struct KConst {
+  int32_t groups;
+  int32_t instant;      // instantaneous log response
+  int32_t ideal_reset;
+  int32_t n_pixels;
+  int64_t refractory_ns;
+  float irr_max;
+  float eps_ci;
+  float label_cut;
+  // Cascade (reference: CascadeConstants, src/sensor/substep.h:26-34).
+  float k, a2, a3, tau_o, tau_sf, eps_accept, eps_drop;
+  float eps_pl_base;
+  // Shot noise (this library's extension): rate per pixel, Hz; 0 = off.
+  double shot_rate_hz;
+  uint64_t seed;
+  uint32_t camera;
+  uint32_t width;
+};
+
+// Per-pixel constants in the kernel's precision (the reference casts its fp64 maps to T once per
+// call: src/sensor/pixel.h:771-775).
+struct MapsDev {
+  const float* theta_on;
+  const float* theta_off;
+  const float* leak;
+  const float* dark;
+  const float* scale;  // illuminance_scale x gain, formed in fp32 as the reference's scale_
+  const float* phi0;
+};
+
+// Working state (the fp32 kernel's view of the reference's PixelState, src/sensor/pixel.h:105-121;
+// the fp32 values round-trip exactly through the reference's fp64 storage, so the saved-state format
+// is produced on the host by widening).
+struct StateDev {
+  float* z;
+  float* x1;
+  float* s2;
+  float* s3;
+  float* i_scale;
+  float* tv;
+  float* s3_ref;
+  float* n3_ref;
+  float* dr_ref;
+  int64_t* t_ref;
+  int64_t* blind_until;
+  int64_t* t_search;
+  int64_t* shot_next;  // next shot-noise arrival (ns), this library's extension
+  uint32_t* shot_index;
+  uint16_t* flags;
+  float* held;  // [g * n_pixels + p]
+};
+
+// Sample runs of a call as the kernels read them: row_of[p] is pixel p's run or -1.
+struct RunsDev {
+  const int32_t* row_of;
+  const int64_t* row_start;
+  const int32_t* row_count;
+  const int64_t* t;
+  const float* irr;
+};
+
+struct RawEventDev {
+  uint32_t pixel;
+  uint8_t polarity;
+  uint8_t label;
+  uint16_t flags;
+  int64_t t_cross;
+  float ds, dn, dl;
+  float l_first;
+};
+static_assert(sizeof(RawEventDev) == 32, "RawEventDev must keep the reference's 32-byte RawEvent layout");
+
+// Per-call work counters, summed over a wave before one atomic add each.
+enum StatIdx : int {
+  kStSubsteps = 0,
+  kStAttempts,
+  kStSettled,
+  kStCells,
+  kStRefinements,
+  kStLeaves,
+  kStFallbacks,
+  kStClamps,
+  kStQuarantines,
+  kStPieces,
+  kStCount
+};
+
+}  // namespace bsx

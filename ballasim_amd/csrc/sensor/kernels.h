@@ -1,0 +1,23 @@
+// Host-side launch wrappers of the sensor kernels (kernels.hip). counters[0] is the event counter,
+// counters[1 + StatIdx] the work counters.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kconst.h"
+
+namespace bsx {
+
+hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, const float* irr0, int64_t t0,
+                       unsigned long long* counters, hipStream_t stream);
+hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
+                          const RunsDev& runs, int64_t t_state, int64_t t_end, RawEventDev* ev,
+                          unsigned long long cap, unsigned long long* counters, int block, hipStream_t stream);
+hipError_t launch_scatter_rows(const uint32_t* pixel, const int64_t* row_start, const int32_t* row_count, size_t n,
+                               size_t n_samples, int32_t* row_of, uint32_t n_pixels, unsigned int* flags,
+                               hipStream_t stream);
+hipError_t launch_fill_u64(unsigned long long* dst, size_t n, unsigned long long v, hipStream_t stream);
+hipError_t launch_check_samples(const float* irr, size_t count, float irr_max, unsigned long long* out,
+                                hipStream_t stream);
+
+}  // namespace bsx

@@ -1,0 +1,260 @@
+// gfx950 kernels of the sensor path: init, the instantaneous-response advance, the input checks and
+// the run scatter. One lane per pixel; a 64-lane wave covers 64 consecutive pixels, so every state,
+// map and row-index access is one coalesced vector access; events leave through a wave-aggregated
+// sink (one atomic per wave and push site, consecutive slots for the active lanes).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "instant.h"
+
+namespace bsx {
+
+namespace {
+
+// Wave-aggregated event sink: the lanes that reach a push together reserve consecutive slots with
+// one atomic add (ballot over the active lanes, rank by a popcount of the lower lanes), then each
+// writes its 32-byte event; slots past the capacity are counted, not written (the host re-runs the
+// call from the untouched input state with a larger buffer).
+struct WaveSink {
+  RawEventDev* ev;
+  unsigned long long cap;
+  unsigned long long* count;
+  __device__ __forceinline__ void push(const RawEventDev& e) {
+    const unsigned long long mask = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    const unsigned rank = __popcll(mask & ((1ull << lane) - 1ull));
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(count, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, leader);
+    const unsigned long long slot = base + rank;
+    if (slot < cap) ev[slot] = e;
+  }
+};
+
+__device__ __forceinline__ float clamp_irr(float v, float vmax_) { return v >= 0.0f ? (v <= vmax_ ? v : vmax_) : 0.0f; }
+
+// Sum of a per-lane counter over the wave, added once to the global counter.
+__device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
+  unsigned long long s = v;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (__lane_id() == 0 && s != 0) atomicAdd(dst, s);
+}
+
+template <int kMaxG>
+__global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateDev st, const float* irr0, int64_t t0,
+                                                   unsigned long long* counters) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t clamps = 0;
+  if (p < kc.n_pixels) {
+    InstantLane<kMaxG> L;
+    L.G = kc.groups;
+    L.dark = mp.dark[p];
+    L.scale = mp.scale[p];
+    L.irr_max = kc.irr_max;
+    L.n = 0;
+    L.skip = 0;
+    L.t_state = t0;
+    float mean_I = L.dark;
+    for (int g = 0; g < kMaxG; ++g) {
+      if (g >= kc.groups) continue;
+      const float raw = irr0[static_cast<int64_t>(p) * kc.groups + g];
+      const float v = clamp_irr(raw, kc.irr_max);
+      if (!(v == raw)) ++clamps;
+      L.held[g] = v;
+      mean_I = mean_I + L.scale * v;
+      st.held[static_cast<int64_t>(g) * kc.n_pixels + p] = v;
+    }
+    L.scan_input();
+    L.i_scale = mean_I;
+    st.i_scale[p] = mean_I;
+    st.z[p] = 1.0f / mean_I;
+    st.x1[p] = 0.0f;
+    st.s2[p] = 0.0f;
+    st.s3[p] = 0.0f;
+    st.tv[p] = 0.0f;
+    const float s3_ref = kc.instant ? L.log_ratio(L.photocurrent(0, t0)) : 0.0f;
+    st.t_ref[p] = t0;
+    st.s3_ref[p] = s3_ref;
+    st.n3_ref[p] = 0.0f;
+    st.dr_ref[p] = mp.phi0[p];
+    st.blind_until[p] = kNotBlind;
+    st.t_search[p] = t0 + 1;
+    st.flags[p] = kFlagFirstAfterInit;
+    st.shot_next[p] = INT64_MAX;
+    st.shot_index[p] = 0;
+  }
+  wave_add(counters + 1 + kStClamps, clamps);
+}
+
+template <int kMaxG>
+__global__ __launch_bounds__(256) void instant_kernel(KConst kc, MapsDev mp, StateDev in, StateDev out, RunsDev runs,
+                                                      int64_t t_state, int64_t t_end, RawEventDev* ev,
+                                                      unsigned long long cap, unsigned long long* counters) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  InstantLane<kMaxG> L;
+  L.n_substeps = L.n_pieces = L.n_leaves = 0;
+  WaveSink sink{ev, cap, counters};
+  if (p < kc.n_pixels) {
+    L.pixel = static_cast<uint32_t>(p);
+    L.theta_on = mp.theta_on[p];
+    L.theta_off = mp.theta_off[p];
+    L.leak = mp.leak[p];
+    L.dark = mp.dark[p];
+    L.scale = mp.scale[p];
+    L.irr_max = kc.irr_max;
+    L.eps_ci = kc.eps_ci;
+    L.label_cut = kc.label_cut;
+    L.refractory_ns = kc.refractory_ns;
+    L.G = kc.groups;
+    L.ideal_reset = kc.ideal_reset != 0;
+    const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
+    if (row >= 0) {
+      const int64_t start = runs.row_start[row];
+      L.ts = runs.t + start;
+      L.irr = runs.irr + start * kc.groups;
+      L.n = runs.row_count[row];
+    } else {
+      L.ts = nullptr;
+      L.irr = nullptr;
+      L.n = 0;
+    }
+    L.t_state = t_state;
+    L.skip = L.n > 0 && L.ts[0] == t_state ? 1 : 0;
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) L.held[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
+    L.i_scale = in.i_scale[p];
+    L.s3_ref = in.s3_ref[p];
+    L.n3_ref = in.n3_ref[p];
+    L.dr_ref = in.dr_ref[p];
+    L.t_ref = in.t_ref[p];
+    L.blind_until = in.blind_until[p];
+    L.t_search = in.t_search[p];
+    L.flags = in.flags[p];
+    L.scan_input();
+    L.advance(t_end, sink);
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) out.held[static_cast<int64_t>(g) * kc.n_pixels + p] = L.held[g];
+    out.z[p] = in.z[p];
+    out.x1[p] = in.x1[p];
+    out.s2[p] = in.s2[p];
+    out.s3[p] = in.s3[p];
+    out.tv[p] = in.tv[p];
+    out.i_scale[p] = L.i_scale;
+    out.s3_ref[p] = L.s3_ref;
+    out.n3_ref[p] = L.n3_ref;
+    out.dr_ref[p] = L.dr_ref;
+    out.t_ref[p] = L.t_ref;
+    out.blind_until[p] = L.blind_until;
+    out.t_search[p] = L.t_search;
+    out.flags[p] = static_cast<uint16_t>(L.flags);
+    out.shot_next[p] = in.shot_next[p];
+    out.shot_index[p] = in.shot_index[p];
+  }
+  wave_add(counters + 1 + kStSubsteps, L.n_substeps);
+  wave_add(counters + 1 + kStPieces, L.n_pieces);
+  wave_add(counters + 1 + kStLeaves, L.n_leaves);
+}
+
+// row_of[pixel[i]] = i, flagging out-of-range ids (flags[0]), duplicates (flags[1]) and runs outside
+// the sample array (flags[2]), so that no kernel reads past the runs it was given.
+__global__ void scatter_rows_kernel(const uint32_t* pixel, const int64_t* row_start, const int32_t* row_count,
+                                    size_t n, size_t n_samples, int32_t* row_of, uint32_t n_pixels,
+                                    unsigned int* flags) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = row_start[i];
+  const int32_t c = row_count[i];
+  if (s < 0 || c < 0 || static_cast<uint64_t>(s) + static_cast<uint64_t>(c) > n_samples) {
+    atomicOr(flags + 2, 1u);
+    return;
+  }
+  const uint32_t p = pixel[i];
+  if (p >= n_pixels) {
+    atomicOr(flags + 0, 1u);
+    return;
+  }
+  const int32_t prev = atomicExch(row_of + p, static_cast<int32_t>(i));
+  if (prev >= 0) atomicOr(flags + 1, 1u);
+}
+
+__global__ void fill_i32_kernel(int32_t* dst, size_t n, int32_t v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+
+__global__ void fill_u64_kernel(unsigned long long* dst, size_t n, unsigned long long v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+
+// Input check of device runs: non-finite samples (out[0]) and clamped samples (out[1]); one lane
+// per sample, coalesced, wave-reduced.
+__global__ void check_samples_kernel(const float* irr, size_t count, float vmax_, unsigned long long* out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t bad = 0, clamped = 0;
+  if (i < count) {
+    const float v = irr[i];
+    if (!(v - v == 0.0f)) bad = 1;
+    else if (clamp_irr(v, vmax_) != v) clamped = 1;
+  }
+  wave_add(out + 0, bad);
+  wave_add(out + 1, clamped);
+}
+
+inline dim3 grid_for(size_t n, int block) { return dim3(static_cast<unsigned>((n + block - 1) / block)); }
+
+}  // namespace
+
+hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, const float* irr0, int64_t t0,
+                       unsigned long long* counters, hipStream_t stream) {
+  const int block = 256;
+  const dim3 grid = grid_for(kc.n_pixels, block);
+  if (kc.groups <= 1) init_kernel<1><<<grid, block, 0, stream>>>(kc, mp, st, irr0, t0, counters);
+  else if (kc.groups <= 2) init_kernel<2><<<grid, block, 0, stream>>>(kc, mp, st, irr0, t0, counters);
+  else if (kc.groups <= 4) init_kernel<4><<<grid, block, 0, stream>>>(kc, mp, st, irr0, t0, counters);
+  else init_kernel<8><<<grid, block, 0, stream>>>(kc, mp, st, irr0, t0, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
+                          const RunsDev& runs, int64_t t_state, int64_t t_end, RawEventDev* ev,
+                          unsigned long long cap, unsigned long long* counters, int block, hipStream_t stream) {
+  const dim3 grid = grid_for(kc.n_pixels, block);
+  if (kc.groups <= 1)
+    instant_kernel<1><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters);
+  else if (kc.groups <= 2)
+    instant_kernel<2><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters);
+  else if (kc.groups <= 4)
+    instant_kernel<4><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters);
+  else
+    instant_kernel<8><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_rows(const uint32_t* pixel, const int64_t* row_start, const int32_t* row_count, size_t n,
+                               size_t n_samples, int32_t* row_of, uint32_t n_pixels, unsigned int* flags,
+                               hipStream_t stream) {
+  const int block = 256;
+  fill_i32_kernel<<<grid_for(n_pixels, block), block, 0, stream>>>(row_of, n_pixels, -1);
+  fill_i32_kernel<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(flags), 4, 0);
+  if (n > 0)
+    scatter_rows_kernel<<<grid_for(n, block), block, 0, stream>>>(pixel, row_start, row_count, n, n_samples, row_of,
+                                                                  n_pixels, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* dst, size_t n, unsigned long long v, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  fill_u64_kernel<<<grid_for(n, 256), 256, 0, stream>>>(dst, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_check_samples(const float* irr, size_t count, float irr_max, unsigned long long* out,
+                                hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  check_samples_kernel<<<grid_for(count, 256), 256, 0, stream>>>(irr, count, irr_max, out);
+  return hipGetLastError();
+}
+
+}  // namespace bsx

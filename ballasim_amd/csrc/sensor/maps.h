@@ -1,0 +1,36 @@
+// Frozen per-pixel sensor maps and the camera's readout, drawn on the host in fp64.
+//
+// Every map is a fixed function of (seed, camera, pixel) drawn from Philox purpose kSensorMap: block 0
+// gives four normals by Box-Muller (fp64 log, sqrt and sincos), block 1 four open uniforms
+// (reference: draw_maps, src/sensor/maps.cpp:78-156). The values are the reference's, bit for bit,
+// which the unit tests check against the reference's own maps.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "params.h"
+
+namespace bsx {
+
+// Structure of arrays, one entry per pixel (y * width + x).
+struct SensorMaps {
+  std::vector<double> theta_on, theta_off, leak, dark_fA, phi0;
+  std::vector<uint8_t> hot;   // 0: none (hot pixels are refused by validate)
+  std::vector<float> gain;    // static photometric gain, 1 unless set_photometry sets a map
+  size_t size() const { return theta_on.size(); }
+};
+
+SensorMaps draw_sensor_maps(const SensorParams& p, uint64_t seed, uint32_t camera, uint32_t n_pixels,
+                            int num_threads = 0);
+
+// The readout of one crossing (reference: Readout / reported_time, src/sensor/readout.h:13-26).
+struct Readout {
+  int64_t quantum_ns, delay_ns, phase_ns, clock_offset_ns;
+  double clock_drift_ppm;
+};
+Readout readout_of(const SensorParams& p, uint64_t seed, uint32_t camera);
+int64_t reported_time(const Readout& r, int64_t t_cross_ns);
+
+}  // namespace bsx

@@ -1,0 +1,596 @@
+#include "model.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "kernels.h"
+
+namespace bsx {
+
+namespace {
+
+constexpr double kMaxSamplePhotocurrent_fA = 1e9;  // the reference's input clamp (src/sensor/pixel.h:97)
+constexpr uint32_t kStateMagic = 0x534e5342;      // "BSNS"
+constexpr uint32_t kStateVersion = 6;
+
+struct StateHeader {
+  uint32_t magic, version;
+  uint32_t width, height;
+  uint32_t groups, camera;
+  uint64_t seed;
+  uint64_t fingerprint;
+  int64_t t_state;
+  uint32_t initialized;
+  uint32_t state_size;
+};
+static_assert(sizeof(StateHeader) == 56, "state header layout");
+
+size_t align256(size_t b) { return (b + 255) & ~size_t{255}; }
+
+// FNV-1a 64 over the parameters, the waveforms and a gain map (when not all ones).
+uint64_t fingerprint(const SensorParams& p, const std::vector<bsx_waveform>& groups, const std::vector<float>& gain) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  const auto mix = [&](const void* d, size_t n) {
+    const auto* b = static_cast<const unsigned char*>(d);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+  };
+  mix(&p, sizeof(p));
+  for (const bsx_waveform& w : groups) mix(&w, sizeof(w));
+  if (!std::all_of(gain.begin(), gain.end(), [](float g) { return g == 1.0f; })) mix(gain.data(), gain.size() * 4);
+  return h;
+}
+
+}  // namespace
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("bsx: ") + what + ": " + hipGetErrorString(e));
+}
+
+DeviceBuffer::~DeviceBuffer() {
+  if (p_ != nullptr) (void)hipFree(p_);
+}
+
+void* DeviceBuffer::ensure(size_t bytes) {
+  if (bytes <= bytes_ && p_ != nullptr) return p_;
+  if (p_ != nullptr) hip_check(hipFree(p_), "hipFree");
+  p_ = nullptr;
+  bytes_ = 0;
+  const size_t want = std::max<size_t>(bytes, 256);
+  hip_check(hipMalloc(&p_, want), "hipMalloc");
+  bytes_ = want;
+  return p_;
+}
+
+void HostEvents::resize(size_t n) {
+  x.resize(n);
+  y.resize(n);
+  p.resize(n);
+  label.resize(n);
+  t_cross.resize(n);
+  t_reported.resize(n);
+  flags.resize(n);
+  ds.resize(n);
+  dn.resize(n);
+  dl.resize(n);
+  l_first.resize(n);
+}
+
+SensorModel::SensorModel(const SensorParams& params, uint32_t width, uint32_t height, uint64_t seed,
+                         std::vector<bsx_waveform> groups, uint32_t camera)
+    : params_(params), width_(width), height_(height), seed_(seed), camera_(camera), groups_(std::move(groups)) {
+  validate(params_);
+  if (width == 0 || height == 0 || width > 65535 || height > 65535) {
+    throw std::invalid_argument("SensorModel: width and height must lie in [1, 65535]");
+  }
+  if (camera > 255) throw std::invalid_argument("SensorModel: camera index must be < 256 (purpose.h layout)");
+  if (groups_.empty() || groups_.size() > static_cast<size_t>(kMaxGroups)) {
+    throw std::invalid_argument("SensorModel: need 1 to 8 light groups");
+  }
+  for (const bsx_waveform& w : groups_) {
+    if (w.type != BSX_WAVE_CONSTANT) {
+      throw std::invalid_argument("SensorModel: modulated light groups (fourier, pulse) are not supported by the bsx "
+                                  "backend");
+    }
+  }
+  int n_dev = 0;
+  if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= 0) {
+    throw std::runtime_error("SensorModel: no HIP device available");
+  }
+  G_ = static_cast<int>(groups_.size());
+  P_ = static_cast<size_t>(width) * height;
+  maps_ = draw_sensor_maps(params_, seed_, camera_, static_cast<uint32_t>(P_));
+  // The kernel needs k * 1 ns * I^2 normal in fp32 (src/sensor/sensor_model.cpp:235-259).
+  const double floor_fA = std::sqrt(static_cast<double>(FLT_MIN) / (params_.k * 1e-15 * 1e-9));
+  double lowest = INFINITY;
+  for (const double d : maps_.dark_fA) lowest = std::min(lowest, d);
+  if (!(lowest >= floor_fA)) {
+    throw std::invalid_argument("SensorModel: the pixel dark current " + std::to_string(lowest) +
+                                " fA is below the fp32 kernel's floor of " + std::to_string(floor_fA) + " fA");
+  }
+  readout_ = readout_of(params_, seed_, camera_);
+  irr_max_ = static_cast<float>(kMaxSamplePhotocurrent_fA / (params_.illuminance_scale * gain_max_));
+  size_t state_bytes = 0;
+  state_bytes += 9 * align256(P_ * 4) + 4 * align256(P_ * 8) + align256(P_ * 4) + align256(P_ * 2);
+  state_bytes += align256(P_ * 4 * G_);
+  d_state_[0].ensure(state_bytes);
+  d_state_[1].ensure(state_bytes);
+  upload_maps();
+  d_counters_.ensure((1 + kStCount) * sizeof(unsigned long long));
+  d_flags_.ensure(64);
+  hip_check(hipEventCreate(&ev0_), "hipEventCreate");
+  hip_check(hipEventCreate(&ev1_), "hipEventCreate");
+}
+
+SensorModel::~SensorModel() {
+  if (ev0_ != nullptr) (void)hipEventDestroy(ev0_);
+  if (ev1_ != nullptr) (void)hipEventDestroy(ev1_);
+}
+
+KConst SensorModel::kconst() const {
+  KConst kc{};
+  kc.groups = G_;
+  kc.instant = params_.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS ? 1 : 0;
+  kc.ideal_reset = params_.reset_mode == BSX_RESET_IDEAL ? 1 : 0;
+  kc.n_pixels = static_cast<int32_t>(P_);
+  kc.refractory_ns = std::llround(params_.refractory_s * 1e9);
+  kc.irr_max = irr_max_;
+  kc.eps_ci = static_cast<float>(params_.eps_ci);
+  kc.label_cut = static_cast<float>(params_.label_share_cut);
+  // Cascade constants (src/sensor/sensor_model.cpp:375-390): eps_h = eps_L / 8, accepted at eps_h / 1.5.
+  const double eps_h = params_.eps_L / 8.0;
+  kc.k = static_cast<float>(params_.k * 1e-15);
+  kc.a2 = static_cast<float>(1.0 / params_.tau_o);
+  kc.a3 = static_cast<float>(1.0 / params_.tau_sf);
+  kc.tau_o = static_cast<float>(params_.tau_o);
+  kc.tau_sf = static_cast<float>(params_.tau_sf);
+  kc.eps_accept = static_cast<float>(eps_h / 1.5);
+  kc.eps_drop = static_cast<float>(std::max(eps_h / 1024.0, 32.0 * static_cast<double>(FLT_EPSILON)));
+  kc.eps_pl_base = static_cast<float>(params_.eps_L / 8.0);
+  kc.shot_rate_hz = params_.shot_rate_hz;
+  kc.seed = seed_;
+  kc.camera = camera_;
+  kc.width = width_;
+  return kc;
+}
+
+void SensorModel::upload_maps() {
+  const size_t a = align256(P_ * 4);
+  d_maps_.ensure(6 * a);
+  std::vector<float> buf(P_);
+  const float scale32 = static_cast<float>(params_.illuminance_scale);
+  const auto put = [&](int slot, const auto& fn) {
+    for (size_t i = 0; i < P_; ++i) buf[i] = fn(i);
+    hip_check(hipMemcpy(static_cast<char*>(d_maps_.get()) + slot * a, buf.data(), P_ * 4, hipMemcpyHostToDevice),
+              "upload maps");
+  };
+  put(0, [&](size_t i) { return static_cast<float>(maps_.theta_on[i]); });
+  put(1, [&](size_t i) { return static_cast<float>(maps_.theta_off[i]); });
+  put(2, [&](size_t i) { return static_cast<float>(maps_.leak[i]); });
+  put(3, [&](size_t i) { return static_cast<float>(maps_.dark_fA[i]); });
+  put(4, [&](size_t i) { return scale32 * maps_.gain[i]; });  // kc.scale x gain in fp32
+  put(5, [&](size_t i) { return static_cast<float>(maps_.phi0[i]); });
+}
+
+StateDev SensorModel::state_view(int which) const {
+  char* b = static_cast<char*>(d_state_[which].get());
+  const size_t f4 = align256(P_ * 4), f8 = align256(P_ * 8);
+  StateDev s{};
+  float** fl[9] = {&s.z, &s.x1, &s.s2, &s.s3, &s.i_scale, &s.tv, &s.s3_ref, &s.n3_ref, &s.dr_ref};
+  for (float** f : fl) {
+    *f = reinterpret_cast<float*>(b);
+    b += f4;
+  }
+  s.t_ref = reinterpret_cast<int64_t*>(b);
+  b += f8;
+  s.blind_until = reinterpret_cast<int64_t*>(b);
+  b += f8;
+  s.t_search = reinterpret_cast<int64_t*>(b);
+  b += f8;
+  s.shot_next = reinterpret_cast<int64_t*>(b);
+  b += f8;
+  s.shot_index = reinterpret_cast<uint32_t*>(b);
+  b += f4;
+  s.flags = reinterpret_cast<uint16_t*>(b);
+  b += align256(P_ * 2);
+  s.held = reinterpret_cast<float*>(b);
+  return s;
+}
+
+static MapsDev maps_view(const DeviceBuffer& d, size_t P) {
+  const size_t a = align256(P * 4);
+  const char* b = static_cast<const char*>(d.get());
+  MapsDev m{};
+  m.theta_on = reinterpret_cast<const float*>(b + 0 * a);
+  m.theta_off = reinterpret_cast<const float*>(b + 1 * a);
+  m.leak = reinterpret_cast<const float*>(b + 2 * a);
+  m.dark = reinterpret_cast<const float*>(b + 3 * a);
+  m.scale = reinterpret_cast<const float*>(b + 4 * a);
+  m.phi0 = reinterpret_cast<const float*>(b + 5 * a);
+  return m;
+}
+
+void SensorModel::set_photometry(double illuminance_scale, const float* gain, size_t n_gain) {
+  if (gain != nullptr && n_gain != P_) {
+    throw std::invalid_argument("SensorModel::set_photometry: one gain per pixel of the frame (or none)");
+  }
+  SensorParams p = params_;
+  p.illuminance_scale = illuminance_scale;
+  validate(p);
+  double g_max = gain == nullptr ? 1.0 : 0.0;
+  for (size_t i = 0; gain != nullptr && i < n_gain; ++i) {
+    if (!(gain[i] > 0.0f) || !std::isfinite(gain[i])) {
+      throw std::invalid_argument("SensorModel::set_photometry: gains must be > 0");
+    }
+    g_max = std::max(g_max, static_cast<double>(gain[i]));
+  }
+  params_ = p;
+  gain_max_ = g_max;
+  for (size_t i = 0; i < P_; ++i) maps_.gain[i] = gain == nullptr ? 1.0f : gain[i];
+  irr_max_ = static_cast<float>(kMaxSamplePhotocurrent_fA / (params_.illuminance_scale * gain_max_));
+  upload_maps();
+}
+
+void SensorModel::init(int64_t t0_ns, const float* irradiance0) {
+  const size_t count = P_ * G_;
+  for (size_t i = 0; i < count; ++i) {
+    if (!std::isfinite(irradiance0[i])) throw std::invalid_argument("init: non-finite irradiance");
+  }
+  initialized_ = false;
+  d_frame_.ensure(count * sizeof(float));
+  hip_check(hipMemcpy(d_frame_.get(), irradiance0, count * sizeof(float), hipMemcpyHostToDevice), "upload frame");
+  hip_check(launch_fill_u64(d_counters_.as<unsigned long long>(), 1 + kStCount, 0, nullptr), "fill");
+  hip_check(launch_init(kconst(), maps_view(d_maps_, P_), state_view(cur_), d_frame_.as<float>(), t0_ns,
+                        d_counters_.as<unsigned long long>(), nullptr),
+            "init kernel");
+  unsigned long long c[1 + kStCount];
+  hip_check(hipMemcpy(c, d_counters_.get(), sizeof(c), hipMemcpyDeviceToHost), "read counters");
+  stats_.input_clamps += c[1 + kStClamps];
+  t_state_ = t0_ns;
+  initialized_ = true;
+}
+
+void SensorModel::advance(const uint32_t* ids, size_t n, const int64_t* t, const float* irr, size_t m,
+                          const int32_t* counts, int64_t t_end_ns) {
+  if (m > static_cast<size_t>(INT32_MAX)) throw std::invalid_argument("SensorModel.advance: too many samples per pixel");
+  std::vector<int64_t> start(n);
+  std::vector<int32_t> count(n);
+  for (size_t i = 0; i < n; ++i) {
+    const int32_t c = counts != nullptr ? counts[i] : static_cast<int32_t>(m);
+    if (c < 0 || static_cast<size_t>(c) > m) throw std::invalid_argument("SensorModel.advance: count out of range");
+    start[i] = static_cast<int64_t>(i * m);
+    count[i] = c;
+  }
+  advance_runs(ids, n, t, irr, n * m, start.data(), count.data(), t_end_ns);
+}
+
+void SensorModel::advance_runs(const uint32_t* ids, size_t n, const int64_t* t, const float* irr, size_t n_samples,
+                               const int64_t* row_start, const int32_t* row_count, int64_t t_end_ns) {
+  if (!initialized_) {
+    throw std::logic_error(
+        "SensorModel.advance: call init() or load_state() first (a call that failed part-way leaves the model "
+        "uninitialized)");
+  }
+  if (t_end_ns < t_state_) throw std::invalid_argument("SensorModel.advance: t_end_ns is before the model time");
+  // The reference's input rules (src/sensor/sensor_model.cpp:1075-1103), checked before any state change.
+  std::vector<int32_t> row_of(P_, -1);
+  uint64_t clamps = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t p = ids[i];
+    if (p >= P_) throw std::invalid_argument("SensorModel.advance: pixel id out of range");
+    if (row_of[p] >= 0) throw std::invalid_argument("SensorModel.advance: duplicate pixel id");
+    row_of[p] = static_cast<int32_t>(i);
+    const int64_t s = row_start[i];
+    const int32_t c = row_count[i];
+    if (c < 0 || s < 0 || static_cast<size_t>(s) + static_cast<size_t>(c) > n_samples) {
+      throw std::invalid_argument("SensorModel.advance: sample run out of range");
+    }
+    const int64_t* ti = t + s;
+    for (int32_t j = 0; j < c; ++j) {
+      if (j == 0 ? ti[0] < t_state_ : ti[j] <= ti[j - 1]) {
+        throw std::invalid_argument(
+            "SensorModel.advance: sample times must be strictly increasing and start at or after the model time");
+      }
+    }
+    if (c > 0 && ti[c - 1] > t_end_ns) throw std::invalid_argument("SensorModel.advance: sample after t_end_ns");
+    const float* vi = irr + static_cast<size_t>(s) * G_;
+    for (size_t j = 0; j < static_cast<size_t>(c) * G_; ++j) {
+      if (!std::isfinite(vi[j])) {
+        throw std::invalid_argument("SensorModel.advance: non-finite irradiance (pixel " + std::to_string(p) +
+                                    ", sample " + std::to_string(j / G_) + ")");
+      }
+      const float v = vi[j];
+      if (!(v >= 0.0f && v <= irr_max_)) ++clamps;
+    }
+  }
+  stats_.input_clamps += clamps;
+  initialized_ = false;
+  d_t_.ensure(std::max<size_t>(n_samples, 1) * 8);
+  d_irr_.ensure(std::max<size_t>(n_samples * G_, 1) * 4);
+  d_row_start_.ensure(std::max<size_t>(n, 1) * 8);
+  d_row_count_.ensure(std::max<size_t>(n, 1) * 4);
+  d_row_of_.ensure(P_ * 4);
+  if (n_samples > 0) {
+    hip_check(hipMemcpy(d_t_.get(), t, n_samples * 8, hipMemcpyHostToDevice), "upload t");
+    hip_check(hipMemcpy(d_irr_.get(), irr, n_samples * G_ * 4, hipMemcpyHostToDevice), "upload irr");
+  }
+  if (n > 0) {
+    hip_check(hipMemcpy(d_row_start_.get(), row_start, n * 8, hipMemcpyHostToDevice), "upload row_start");
+    hip_check(hipMemcpy(d_row_count_.get(), row_count, n * 4, hipMemcpyHostToDevice), "upload row_count");
+  }
+  hip_check(hipMemcpy(d_row_of_.get(), row_of.data(), P_ * 4, hipMemcpyHostToDevice), "upload row_of");
+  run_advance(runs_view(), t_end_ns, nullptr);
+  t_state_ = t_end_ns;
+  initialized_ = true;
+  unsigned long long count = 0;
+  hip_check(hipMemcpy(&count, d_counters_.get(), 8, hipMemcpyDeviceToHost), "read count");
+  raw_host_.resize(count);
+  if (count > 0) {
+    hip_check(hipMemcpy(raw_host_.data(), d_events_.get(), count * sizeof(RawEventDev), hipMemcpyDeviceToHost),
+              "download events");
+  }
+  finish_host_events(raw_host_.data(), count);
+}
+
+RunsDev SensorModel::runs_view() const {
+  RunsDev r{};
+  r.row_of = d_row_of_.as<const int32_t>();
+  r.row_start = d_row_start_.as<const int64_t>();
+  r.row_count = d_row_count_.as<const int32_t>();
+  r.t = d_t_.as<const int64_t>();
+  r.irr = d_irr_.as<const float>();
+  return r;
+}
+
+void SensorModel::run_advance(const RunsDev& runs, int64_t t_end, hipStream_t stream) {
+  if (params_.photoreceptor != BSX_PHOTORECEPTOR_INSTANTANEOUS) {
+    throw std::logic_error("SensorModel: the cascade photoreceptor is not built into this backend yet");
+  }
+  if (cap_ == 0) cap_ = std::max<unsigned long long>(1ull << 20, 2 * P_);
+  d_events_.ensure(cap_ * sizeof(RawEventDev));
+  const KConst kc = kconst();
+  const MapsDev mp = maps_view(d_maps_, P_);
+  unsigned long long c[1 + kStCount];
+  while (true) {
+    hip_check(launch_fill_u64(d_counters_.as<unsigned long long>(), 1 + kStCount, 0, stream), "fill counters");
+    hip_check(hipEventRecord(ev0_, stream), "event record");
+    hip_check(launch_instant(kc, mp, state_view(cur_), state_view(1 - cur_), runs, t_state_, t_end,
+                             d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(), block_, stream),
+              "instant kernel");
+    hip_check(hipEventRecord(ev1_, stream), "event record");
+    hip_check(hipMemcpyAsync(c, d_counters_.get(), sizeof(c), hipMemcpyDeviceToHost, stream), "read counters");
+    hip_check(hipStreamSynchronize(stream), "advance");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event time");
+    stats_.kernel_ms += ms;
+    ++stats_.launches;
+    if (c[0] <= cap_) break;
+    cap_ = std::max<unsigned long long>(cap_ * 2, c[0] + c[0] / 4);
+    d_events_.ensure(cap_ * sizeof(RawEventDev));
+    ++stats_.overflow_reruns;
+  }
+  cur_ = 1 - cur_;
+  stats_.events += c[0];
+  stats_.substeps += c[1 + kStSubsteps];
+  stats_.substep_attempts += c[1 + kStAttempts];
+  stats_.settled_substeps += c[1 + kStSettled];
+  stats_.cells += c[1 + kStCells];
+  stats_.refinements += c[1 + kStRefinements];
+  stats_.leaf_intervals += c[1 + kStLeaves];
+  stats_.fallbacks += c[1 + kStFallbacks];
+  stats_.quarantines += c[1 + kStQuarantines];
+  stats_.pieces += c[1 + kStPieces];
+}
+
+bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream) {
+  if (!initialized_) {
+    throw std::logic_error(
+        "SensorModel.advance: call init() or load_state() first (a call that failed part-way leaves the model "
+        "uninitialized)");
+  }
+  if (t_end_ns < t_state_) throw std::invalid_argument("SensorModel.advance: t_end_ns is before the model time");
+  d_row_of_.ensure(P_ * 4);
+  unsigned int* flags = d_flags_.as<unsigned int>();
+  hip_check(launch_scatter_rows(runs.pixel, runs.row_start, runs.row_count, runs.n, runs.n_samples,
+                                d_row_of_.as<int32_t>(), static_cast<uint32_t>(P_), flags, stream),
+            "scatter rows");
+  unsigned long long* chk = reinterpret_cast<unsigned long long*>(d_flags_.as<char>() + 16);
+  hip_check(launch_fill_u64(chk, 2, 0, stream), "fill");
+  hip_check(launch_check_samples(runs.irr, runs.n_samples * G_, irr_max_, chk, stream), "check samples");
+  unsigned int f[4];
+  hip_check(hipMemcpyAsync(f, d_flags_.get(), 16, hipMemcpyDeviceToHost, stream), "read flags");
+  unsigned long long cc[2];
+  hip_check(hipMemcpyAsync(cc, chk, 16, hipMemcpyDeviceToHost, stream), "read checks");
+  hip_check(hipStreamSynchronize(stream), "checks");
+  if (f[0] != 0) throw std::invalid_argument("SensorModel.advance: pixel id out of range");
+  if (f[1] != 0) throw std::invalid_argument("SensorModel.advance: duplicate pixel id");
+  if (f[2] != 0) throw std::invalid_argument("SensorModel.advance: sample run out of range");
+  if (cc[0] != 0) throw std::invalid_argument("SensorModel.advance_device: non-finite irradiance in the runs");
+  stats_.input_clamps += cc[1];
+  RunsDev r{};
+  r.row_of = d_row_of_.as<const int32_t>();
+  r.row_start = runs.row_start;
+  r.row_count = runs.row_count;
+  r.t = runs.t;
+  r.irr = runs.irr;
+  initialized_ = false;
+  run_advance(r, t_end_ns, stream);
+  t_state_ = t_end_ns;
+  initialized_ = true;
+  unsigned long long count = 0;
+  hip_check(hipMemcpy(&count, d_counters_.get(), 8, hipMemcpyDeviceToHost), "read count");
+  return bsx_device_events{reinterpret_cast<const bsx_raw_event*>(d_events_.get()), static_cast<size_t>(count)};
+}
+
+void SensorModel::download_events(const bsx_device_events& ev) {
+  raw_host_.resize(ev.count);
+  if (ev.count > 0) {
+    hip_check(hipMemcpy(raw_host_.data(), ev.data, ev.count * sizeof(RawEventDev), hipMemcpyDeviceToHost),
+              "download events");
+  }
+  finish_host_events(raw_host_.data(), ev.count);
+}
+
+void SensorModel::finish_host_events(const RawEventDev* raw, size_t n) {
+  // (t_cross, pixel) is unique: a pixel never fires twice in one ns.
+  std::vector<RawEventDev> all(raw, raw + n);
+  std::sort(all.begin(), all.end(), [](const RawEventDev& a, const RawEventDev& b) {
+    return a.t_cross != b.t_cross ? a.t_cross < b.t_cross : a.pixel < b.pixel;
+  });
+  events_.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const RawEventDev& e = all[i];
+    events_.x[i] = static_cast<uint16_t>(e.pixel % width_);
+    events_.y[i] = static_cast<uint16_t>(e.pixel / width_);
+    events_.p[i] = e.polarity;
+    events_.t_cross[i] = e.t_cross;
+    events_.t_reported[i] = reported_time(readout_, e.t_cross);
+    events_.label[i] = e.label;
+    events_.flags[i] = e.flags;
+    events_.ds[i] = e.ds;
+    events_.dn[i] = e.dn;
+    events_.dl[i] = e.dl;
+    events_.l_first[i] = e.l_first;
+  }
+}
+
+size_t SensorModel::state_bytes() const { return sizeof(StateHeader) + P_ * sizeof(SavedPixelState); }
+
+void SensorModel::save_state(uint8_t* out, size_t n) const {
+  if (n < state_bytes()) throw std::invalid_argument("SensorModel.save_state: buffer too small");
+  StateHeader h{};
+  h.magic = kStateMagic;
+  h.version = kStateVersion;
+  h.width = width_;
+  h.height = height_;
+  h.groups = static_cast<uint32_t>(G_);
+  h.camera = camera_;
+  h.seed = seed_;
+  h.fingerprint = fingerprint(params_, groups_, maps_.gain);
+  h.t_state = t_state_;
+  h.initialized = initialized_ ? 1 : 0;
+  h.state_size = sizeof(SavedPixelState);
+  std::memcpy(out, &h, sizeof(h));
+  const StateDev s = state_view(cur_);
+  std::vector<float> f(P_);
+  std::vector<int64_t> q(P_);
+  std::vector<uint32_t> u(P_);
+  std::vector<uint16_t> fl(P_);
+  std::vector<SavedPixelState> ps(P_);
+  std::memset(ps.data(), 0, P_ * sizeof(SavedPixelState));
+  const auto getf = [&](const float* src, double SavedPixelState::*dst) {
+    hip_check(hipMemcpy(f.data(), src, P_ * 4, hipMemcpyDeviceToHost), "download state");
+    for (size_t i = 0; i < P_; ++i) ps[i].*dst = static_cast<double>(f[i]);
+  };
+  const auto geti = [&](const int64_t* src, int64_t SavedPixelState::*dst) {
+    hip_check(hipMemcpy(q.data(), src, P_ * 8, hipMemcpyDeviceToHost), "download state");
+    for (size_t i = 0; i < P_; ++i) ps[i].*dst = q[i];
+  };
+  getf(s.z, &SavedPixelState::z);
+  getf(s.x1, &SavedPixelState::x1);
+  getf(s.s2, &SavedPixelState::s2);
+  getf(s.s3, &SavedPixelState::s3);
+  getf(s.i_scale, &SavedPixelState::i_scale);
+  getf(s.tv, &SavedPixelState::tv);
+  getf(s.s3_ref, &SavedPixelState::s3_ref);
+  getf(s.n3_ref, &SavedPixelState::n3_ref);
+  getf(s.dr_ref, &SavedPixelState::dr_ref);
+  geti(s.t_ref, &SavedPixelState::t_ref);
+  geti(s.blind_until, &SavedPixelState::blind_until);
+  geti(s.t_search, &SavedPixelState::t_search);
+  // The shot-noise process's next arrival and index ride in fields the noise-free kernel leaves
+  // unused (cell and nk[0]).
+  geti(s.shot_next, &SavedPixelState::cell);
+  hip_check(hipMemcpy(u.data(), s.shot_index, P_ * 4, hipMemcpyDeviceToHost), "download state");
+  for (size_t i = 0; i < P_; ++i) ps[i].nk[0] = static_cast<double>(u[i]);
+  hip_check(hipMemcpy(fl.data(), s.flags, P_ * 2, hipMemcpyDeviceToHost), "download state");
+  for (size_t i = 0; i < P_; ++i) ps[i].flags = fl[i];
+  for (int g = 0; g < G_; ++g) {
+    hip_check(hipMemcpy(f.data(), s.held + static_cast<size_t>(g) * P_, P_ * 4, hipMemcpyDeviceToHost),
+              "download state");
+    for (size_t i = 0; i < P_; ++i) ps[i].held[g] = f[i];
+  }
+  std::memcpy(out + sizeof(h), ps.data(), P_ * sizeof(SavedPixelState));
+}
+
+void SensorModel::load_state(const uint8_t* data, size_t n) {
+  if (n != state_bytes()) throw std::invalid_argument("SensorModel.load_state: size does not match this model");
+  StateHeader h{};
+  std::memcpy(&h, data, sizeof(h));
+  if (h.magic != kStateMagic || h.version != kStateVersion || h.state_size != sizeof(SavedPixelState)) {
+    throw std::invalid_argument("SensorModel.load_state: not a sensor state of this version");
+  }
+  if (h.width != width_ || h.height != height_ || h.groups != static_cast<uint32_t>(G_) || h.camera != camera_ ||
+      h.seed != seed_ || h.fingerprint != fingerprint(params_, groups_, maps_.gain)) {
+    throw std::invalid_argument("SensorModel.load_state: state was saved by a model with other parameters");
+  }
+  initialized_ = false;
+  std::vector<SavedPixelState> ps(P_);
+  std::memcpy(ps.data(), data + sizeof(h), P_ * sizeof(SavedPixelState));
+  const StateDev s = state_view(cur_);
+  std::vector<float> f(P_);
+  std::vector<int64_t> q(P_);
+  std::vector<uint32_t> u(P_);
+  std::vector<uint16_t> fl(P_);
+  const auto putf = [&](float* dst, double SavedPixelState::*src) {
+    for (size_t i = 0; i < P_; ++i) f[i] = static_cast<float>(ps[i].*src);
+    hip_check(hipMemcpy(dst, f.data(), P_ * 4, hipMemcpyHostToDevice), "upload state");
+  };
+  const auto puti = [&](int64_t* dst, int64_t SavedPixelState::*src) {
+    for (size_t i = 0; i < P_; ++i) q[i] = ps[i].*src;
+    hip_check(hipMemcpy(dst, q.data(), P_ * 8, hipMemcpyHostToDevice), "upload state");
+  };
+  putf(s.z, &SavedPixelState::z);
+  putf(s.x1, &SavedPixelState::x1);
+  putf(s.s2, &SavedPixelState::s2);
+  putf(s.s3, &SavedPixelState::s3);
+  putf(s.i_scale, &SavedPixelState::i_scale);
+  putf(s.tv, &SavedPixelState::tv);
+  putf(s.s3_ref, &SavedPixelState::s3_ref);
+  putf(s.n3_ref, &SavedPixelState::n3_ref);
+  putf(s.dr_ref, &SavedPixelState::dr_ref);
+  puti(s.t_ref, &SavedPixelState::t_ref);
+  puti(s.blind_until, &SavedPixelState::blind_until);
+  puti(s.t_search, &SavedPixelState::t_search);
+  puti(s.shot_next, &SavedPixelState::cell);
+  for (size_t i = 0; i < P_; ++i) u[i] = static_cast<uint32_t>(ps[i].nk[0]);
+  hip_check(hipMemcpy(s.shot_index, u.data(), P_ * 4, hipMemcpyHostToDevice), "upload state");
+  for (size_t i = 0; i < P_; ++i) fl[i] = ps[i].flags;
+  hip_check(hipMemcpy(s.flags, fl.data(), P_ * 2, hipMemcpyHostToDevice), "upload state");
+  for (int g = 0; g < G_; ++g) {
+    for (size_t i = 0; i < P_; ++i) f[i] = ps[i].held[g];
+    hip_check(hipMemcpy(s.held + static_cast<size_t>(g) * P_, f.data(), P_ * 4, hipMemcpyHostToDevice),
+              "upload state");
+  }
+  t_state_ = h.t_state;
+  initialized_ = h.initialized != 0;
+}
+
+void SensorModel::reset_stats() { stats_ = bsx_stats{}; }
+
+void SensorModel::set_policy(const bsx_launch_policy& p) {
+  if (p.block_size != 0 && (p.block_size % 64 != 0 || p.block_size < 64 || p.block_size > 1024)) {
+    throw std::invalid_argument("set_launch_policy: block_size must be a multiple of 64 in [64, 1024]");
+  }
+  if (p.event_capacity != 0) cap_ = p.event_capacity;
+  if (p.block_size != 0) block_ = p.block_size;
+}
+
+bsx_sensor_info_t SensorModel::info() const {
+  bsx_sensor_info_t i{};
+  i.width = width_;
+  i.height = height_;
+  i.groups = G_;
+  i.initialized = initialized_ ? 1 : 0;
+  i.time_ns = t_state_;
+  i.readout_phase_ns = readout_.phase_ns;
+  i.irr_max = irr_max_;
+  i.camera = camera_;
+  i.seed = seed_;
+  return i;
+}
+
+}  // namespace bsx

@@ -1,0 +1,144 @@
+// SensorModel on one MI355X: the host side of the sensor path.
+//
+// It owns the frozen maps (drawn on the host in fp64, kept on the device in the kernel's fp32), the
+// per-pixel working state (two structure-of-arrays copies in HBM: a call reads one and writes the
+// other, so an event-buffer overflow re-runs from the untouched input state without a backup copy),
+// the event buffer and the launch loop. The member functions mirror the reference's SensorModel
+// (src/sensor/sensor_model.h:98-320) for the device kCuda / precision kFp32 configuration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "kconst.h"
+#include "maps.h"
+#include "params.h"
+
+namespace bsx {
+
+// Grow-only device allocation.
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer();
+  void* ensure(size_t bytes);  // keeps the old contents only when no growth is needed
+  void* get() const { return p_; }
+  size_t bytes() const { return bytes_; }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p_);
+  }
+
+ private:
+  void* p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+void hip_check(hipError_t e, const char* what);
+
+struct HostEvents {
+  std::vector<uint16_t> x, y;
+  std::vector<uint8_t> p, label;
+  std::vector<int64_t> t_cross, t_reported;
+  std::vector<uint16_t> flags;
+  std::vector<float> ds, dn, dl, l_first;
+  void resize(size_t n);
+  size_t size() const { return t_cross.size(); }
+};
+
+// The reference's saved per-pixel state (src/sensor/pixel.h:105-121), 200 bytes: the format of
+// save_state / load_state.
+struct SavedPixelState {
+  double z, x1, s2, s3;
+  double i_scale;
+  double tv;
+  double nk[3], nk1[3];
+  double ln_a1;
+  int64_t cell;
+  int64_t t_ref;
+  double s3_ref, n3_ref, dr_ref;
+  int64_t blind_until;
+  int64_t t_search;
+  float held[8];
+  uint16_t flags;
+  uint8_t have_k1;
+  uint8_t reserved;
+  float rho_m1;
+};
+static_assert(sizeof(SavedPixelState) == 200, "the saved-state layout is the reference's");
+
+class SensorModel {
+ public:
+  SensorModel(const SensorParams& params, uint32_t width, uint32_t height, uint64_t seed,
+              std::vector<bsx_waveform> groups, uint32_t camera);
+  ~SensorModel();
+  SensorModel(const SensorModel&) = delete;
+  SensorModel& operator=(const SensorModel&) = delete;
+
+  void set_photometry(double illuminance_scale, const float* gain, size_t n_gain);
+  void init(int64_t t0_ns, const float* irradiance0);
+  void advance(const uint32_t* ids, size_t n, const int64_t* t, const float* irr, size_t m, const int32_t* counts,
+               int64_t t_end_ns);
+  void advance_runs(const uint32_t* ids, size_t n, const int64_t* t, const float* irr, size_t n_samples,
+                    const int64_t* row_start, const int32_t* row_count, int64_t t_end_ns);
+  bsx_device_events advance_device(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream);
+  void download_events(const bsx_device_events& ev);
+  const HostEvents& events() const { return events_; }
+
+  size_t state_bytes() const;
+  void save_state(uint8_t* out, size_t n) const;
+  void load_state(const uint8_t* data, size_t n);
+
+  const bsx_stats& stats() const { return stats_; }
+  void reset_stats();
+  void set_policy(const bsx_launch_policy& p);
+  bsx_sensor_info_t info() const;
+  const SensorMaps& maps() const { return maps_; }
+  const SensorParams& params() const { return params_; }
+
+ private:
+  void upload_maps();
+  StateDev state_view(int which) const;
+  RunsDev runs_view() const;
+  // The launch loop of one call on device runs (row_of built), with overflow re-runs.
+  void run_advance(const RunsDev& runs, int64_t t_end, hipStream_t stream);
+  void finish_host_events(const RawEventDev* raw, size_t n);
+  KConst kconst() const;
+
+  SensorParams params_;
+  uint32_t width_, height_;
+  uint64_t seed_;
+  uint32_t camera_;
+  std::vector<bsx_waveform> groups_;
+  int G_;
+  size_t P_;
+  SensorMaps maps_;
+  Readout readout_{};
+  double gain_max_ = 1.0;
+  float irr_max_ = 0.0f;
+
+  DeviceBuffer d_maps_;       // 6 fp32 arrays of P
+  DeviceBuffer d_state_[2];   // two state copies
+  int cur_ = 0;
+  DeviceBuffer d_events_;
+  unsigned long long cap_ = 0;
+  DeviceBuffer d_counters_;   // 1 + kStCount uint64
+  DeviceBuffer d_flags_;      // scatter flags + check counters
+  DeviceBuffer d_row_of_;
+  DeviceBuffer d_t_, d_irr_, d_row_start_, d_row_count_, d_frame_;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+
+  int block_ = 256;
+  int64_t t_state_ = 0;
+  bool initialized_ = false;
+  bsx_stats stats_{};
+  HostEvents events_;
+  std::vector<RawEventDev> raw_host_;
+};
+
+}  // namespace bsx

@@ -1,0 +1,234 @@
+/* bsx: the per-pixel continuous-time event-sensor path on MI355X (gfx950), as a C ABI.
+ *
+ * The entry points replace the reference's sensor backend seam: the C++ class
+ * balla::sensor::SensorModel (src/sensor/sensor_model.h:98-320) with its CUDA pimpl
+ * CudaSensorKernel (src/sensor/sensor_cuda.h:20-68), and the Python module _sensor built on it
+ * (src/sensor/sensor_module.cpp:446-584). Each declaration names the reference member it stands in
+ * for. Plain pointers and sizes only: no C++ or torch types cross this boundary.
+ *
+ * Conventions.
+ *  - Every function returns a status: BSX_OK (0), or BSX_E_INVALID (the reference throws
+ *    std::invalid_argument: bad input, refused before any state change), BSX_E_LOGIC
+ *    (std::logic_error: misuse such as advancing an uninitialised model), BSX_E_RUNTIME
+ *    (std::runtime_error: no device, a HIP error). bsx_last_error() returns the message of the last
+ *    failure on the calling thread.
+ *  - A call that fails after its input checks leaves the model uninitialised until init or
+ *    load_state (reference: src/sensor/sensor_model.h:177-180).
+ *  - Times are int64 ns; irradiance is fp32 in renderer units; one model is used by one thread at a
+ *    time; device work is queued on the caller's stream (a hipStream_t passed as void*, 0 = the
+ *    null stream).
+ */
+#ifndef BSX_SENSOR_H_
+#define BSX_SENSOR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSX_OK 0
+#define BSX_E_INVALID 1
+#define BSX_E_LOGIC 2
+#define BSX_E_RUNTIME 3
+
+#define BSX_MAX_LIGHT_GROUPS 8
+#define BSX_MAX_HARMONICS 8
+
+enum { BSX_PHOTORECEPTOR_CASCADE = 0, BSX_PHOTORECEPTOR_INSTANTANEOUS = 1 };
+enum { BSX_RESET_PHYSICAL = 0, BSX_RESET_IDEAL = 1 };
+enum { BSX_PHASE_UNIFORM = 0, BSX_PHASE_SYMMETRIC = 1, BSX_PHASE_ZERO = 2 };
+enum { BSX_WAVE_CONSTANT = 0, BSX_WAVE_FOURIER = 1, BSX_WAVE_PULSE = 2 };
+
+/* Sensor parameters: the fields of the reference's SensorParams (src/sensor/params.h:65-172), SI
+ * units, plus shot_rate_hz (this library's rate-specified shot-noise process; 0 = off, as every
+ * reference family). Enum-valued fields hold the BSX_* values above. */
+typedef struct bsx_params {
+  char family[32];
+  int32_t photoreceptor;
+  int32_t noise;
+  double k, tau_o, tau_sf, sigma_o;
+  double illuminance_scale, veiling_glare;
+  double dark_current_fA, dark_sigma_ln, dark_ea_eV, temperature_K, reference_temperature_K;
+  double theta_on, theta_off, sigma_theta_on, sigma_theta_off, theta_onoff_corr, theta_floor;
+  int32_t reset_mode;
+  int32_t initial_phase;
+  double refractory_s, eps_ci;
+  double leak_rate_hz, leak_sigma_log10, leak_ea_eV;
+  double tau_hp, tau_hp_sigma_ln;
+  double hot_fraction;
+  int32_t hot_rate_law;
+  int32_t limiter;
+  double hot_rate_min_hz, hot_rate_max_hz, hot_rate_median_hz, hot_rate_sigma_ln, hot_leak_fraction,
+      hot_reference_current_fA;
+  double readout_quantum_s, readout_delay_s;
+  int64_t clock_offset_ns;
+  double clock_drift_ppm;
+  double eps_L;
+  int32_t noise_leaf_level;
+  int32_t noise_diffusion;
+  double noise_margin_z;
+  int32_t initial_state;
+  int32_t reserved0;
+  double label_share_cut;
+  double shot_rate_hz;
+} bsx_params;
+
+/* A light group's waveform m_g(t) (reference: struct Waveform, src/sensor/waveform.h:38-52). */
+typedef struct bsx_waveform {
+  int32_t type;
+  int32_t n_harmonics;
+  double f0_hz;
+  double a[BSX_MAX_HARMONICS];
+  double phi[BSX_MAX_HARMONICS];
+  double period_ns, phase_ns, rise_ns, top_ns, floor_level, peak_level;
+} bsx_waveform;
+
+/* One event as the kernel emits it: the reference's RawEvent (src/sensor/pixel.h:123-135), 32 B. */
+typedef struct bsx_raw_event {
+  uint32_t pixel;
+  uint8_t polarity; /* 1 = ON */
+  uint8_t label;    /* 0 signal, 1 noise, 2 leak, 3 hot */
+  uint16_t flags;
+  int64_t t_cross;
+  float ds, dn, dl;
+  float l_first;
+} bsx_raw_event;
+
+/* Sample runs in device memory (reference: DeviceRuns, src/sensor/sensor_model.h:81-89): listed
+ * pixel pixel[i] takes the row_count[i] samples starting at sample row_start[i] of t and
+ * irr (n_samples x groups). */
+typedef struct bsx_device_runs {
+  const uint32_t* pixel;
+  const int64_t* row_start;
+  const int32_t* row_count;
+  size_t n;
+  const int64_t* t;
+  const float* irr;
+  size_t n_samples;
+} bsx_device_runs;
+
+/* Raw events left in device memory by advance_device, unordered, valid until the next advance
+ * (reference: DeviceEvents, src/sensor/sensor_model.h:93-96). */
+typedef struct bsx_device_events {
+  const bsx_raw_event* data;
+  size_t count;
+} bsx_device_events;
+
+/* Host view of the events of the last advance/download, sorted by (t_cross, pixel), as the
+ * reference's Events (src/sensor/sensor_model.h:66-75). Pointers into model-owned memory, valid
+ * until the next call on the model. */
+typedef struct bsx_events {
+  size_t count;
+  const uint16_t* x;
+  const uint16_t* y;
+  const uint8_t* p;
+  const int64_t* t_cross_ns;
+  const int64_t* t_reported_ns;
+  const uint8_t* label;
+  const uint16_t* flags;
+  const float* ds;
+  const float* dn;
+  const float* dl;
+  const float* l_first;
+} bsx_events;
+
+/* Work counters (reference: KernelStats, src/sensor/pixel.h:137-169), plus launch timing. */
+typedef struct bsx_stats {
+  uint64_t substeps, substep_attempts, settled_substeps, cells, refinements, leaf_intervals, fallbacks,
+      input_clamps, quarantines, pieces;
+  uint64_t launches, overflow_reruns, events;
+  double kernel_ms; /* summed device time of the advance kernels (HIP events) */
+} bsx_stats;
+
+/* Launch policy (reference: GpuLaunchPolicy, src/sensor/launch.h:16-27): the initial event
+ * capacity (0 = automatic), which the model grows on overflow. */
+typedef struct bsx_launch_policy {
+  uint64_t event_capacity;
+  int32_t block_size; /* threads per workgroup, a multiple of 64 in [64, 1024]; 0 = automatic */
+  int32_t reserved;
+} bsx_launch_policy;
+
+typedef struct bsx_sensor_info_t {
+  uint32_t width, height;
+  int32_t groups;
+  int32_t initialized;
+  int64_t time_ns;
+  int64_t readout_phase_ns;
+  float irr_max;
+  uint32_t camera;
+  uint64_t seed;
+} bsx_sensor_info_t;
+
+typedef struct bsx_sensor bsx_sensor;
+
+/* The message of the last failure on this thread (truncated to n - 1 bytes); BSX_OK. */
+int bsx_last_error(char* buf, size_t n);
+
+/* Visible HIP devices. Reference: sensor_cuda_available(), src/sensor/sensor_model.h:52. */
+int bsx_device_count(int32_t* out);
+
+/* Reference: preset(name), src/sensor/params.h:176; preset_names(), :177 (names joined by ','). */
+int bsx_params_preset(const char* name, bsx_params* out);
+int bsx_preset_names(char* buf, size_t n);
+/* Reference: SensorParams::validate(), src/sensor/params.cpp:44-96. */
+int bsx_params_validate(const bsx_params* p);
+
+/* Reference: SensorModel::SensorModel(params, width, height, seed, light_groups, camera, ...,
+ * Precision::kFp32, Device::kCuda), src/sensor/sensor_model.h:107-110 (num_threads, precision and
+ * roi have no meaning here: one GPU, fp32, full frame). */
+int bsx_sensor_create(const bsx_params* params, uint32_t width, uint32_t height, uint64_t seed,
+                      const bsx_waveform* light_groups, int32_t n_groups, uint32_t camera, bsx_sensor** out);
+/* Reference: SensorModel::~SensorModel. */
+int bsx_sensor_destroy(bsx_sensor* s);
+
+/* Reference: SensorModel::set_photometry(scale, gain), src/sensor/sensor_model.h:116 (gain: one
+ * per pixel, or null for 1 everywhere). */
+int bsx_sensor_set_photometry(bsx_sensor* s, double illuminance_scale, const float* gain, size_t n_gain);
+
+/* Reference: SensorModel::init(t0_ns, irradiance0), src/sensor/sensor_model.h:121. irradiance0 is
+ * a host array of (width * height) x groups. */
+int bsx_sensor_init(bsx_sensor* s, int64_t t0_ns, const float* irradiance0);
+
+/* Reference: SensorModel::advance(ids, n, t, irr, m, counts, t_end), src/sensor/sensor_model.h:150.
+ * Host arrays; the events are then in bsx_sensor_events. */
+int bsx_sensor_advance(bsx_sensor* s, const uint32_t* pixel_ids, size_t n, const int64_t* t, const float* irr,
+                       size_t m, const int32_t* counts, int64_t t_end_ns);
+/* Reference: SensorModel::advance_runs(...), src/sensor/sensor_model.h:157-158. Host arrays. */
+int bsx_sensor_advance_runs(bsx_sensor* s, const uint32_t* pixel_ids, size_t n, const int64_t* t, const float* irr,
+                            size_t n_samples, const int64_t* row_start, const int32_t* row_count, int64_t t_end_ns);
+/* Reference: SensorModel::advance_device(runs, t_end, stream), src/sensor/sensor_model.h:166. The
+ * runs are in device memory; the events stay there (out). */
+int bsx_sensor_advance_device(bsx_sensor* s, const bsx_device_runs* runs, int64_t t_end_ns, void* stream,
+                              bsx_device_events* out);
+/* Reference: SensorModel::download_events(events), src/sensor/sensor_model.h:167. Sorts by
+ * (t_cross, pixel) and fills the host view of bsx_sensor_events. */
+int bsx_sensor_download_events(bsx_sensor* s, const bsx_device_events* events);
+/* The host events of the last advance / advance_runs / download_events. */
+int bsx_sensor_events(bsx_sensor* s, bsx_events* out);
+
+/* Reference: state_bytes / save_state / load_state, src/sensor/sensor_model.h:181-183. The blob is
+ * the reference's layout: a 56-byte header ("BSNS") followed by one 200-byte PixelState per pixel. */
+int bsx_sensor_state_bytes(bsx_sensor* s, size_t* out);
+int bsx_sensor_save_state(bsx_sensor* s, uint8_t* buf, size_t n);
+int bsx_sensor_load_state(bsx_sensor* s, const uint8_t* buf, size_t n);
+
+/* Reference: stats() / reset_stats(), src/sensor/sensor_model.h:187-192. */
+int bsx_sensor_stats(bsx_sensor* s, bsx_stats* out);
+int bsx_sensor_reset_stats(bsx_sensor* s);
+/* Reference: set_gpu_launch_policy(policy), src/sensor/sensor_model.h:194. */
+int bsx_sensor_set_launch_policy(bsx_sensor* s, const bsx_launch_policy* policy);
+/* width, height, groups, initialized, time_ns, readout_phase_ns, irr_max (reference accessors,
+ * src/sensor/sensor_model.h:195-219). */
+int bsx_sensor_info(bsx_sensor* s, bsx_sensor_info_t* out);
+/* The frozen maps (reference: SensorModel::maps(), src/sensor/sensor_model.h:186), one value per
+ * pixel in pixel order; any pointer may be null. */
+int bsx_sensor_maps(bsx_sensor* s, double* theta_on, double* theta_off, double* leak, double* dark_fA, double* phi0,
+                    uint8_t* hot);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BSX_SENSOR_H_ */

@@ -1,0 +1,159 @@
+// A C entry layer over the reference's own SensorModel (src/sensor/sensor_model.h), linked with the
+// reference's sources where they lie (oracle/Makefile): the compiled reference, used by the tests
+// to pin the C restatement (bsx_oracle.c) and, where it is present, as the timed CPU baseline.
+// Test infrastructure only; nothing of the product links it.
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../include/bsx/bsx_sensor.h"
+#include "sensor/params.h"
+#include "sensor/sensor_model.h"
+
+using namespace balla::sensor;
+
+namespace {
+
+struct RefModel {
+  std::unique_ptr<SensorModel> m;
+  std::vector<RawEvent> ev;
+};
+
+void put_err(char* err, size_t n, const char* msg) {
+  if (err != nullptr && n > 0) {
+    std::strncpy(err, msg, n - 1);
+    err[n - 1] = '\0';
+  }
+}
+
+SensorParams to_ref(const bsx_params& b) {
+  SensorParams p = preset(std::string(b.family, strnlen(b.family, sizeof(b.family))));
+  p.photoreceptor = b.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS ? Photoreceptor::kInstantaneous
+                                                                       : Photoreceptor::kCascade;
+  p.noise = b.noise != 0;
+  p.k = b.k;
+  p.tau_o = b.tau_o;
+  p.tau_sf = b.tau_sf;
+  p.sigma_o = b.sigma_o;
+  p.illuminance_scale = b.illuminance_scale;
+  p.veiling_glare = b.veiling_glare;
+  p.dark_current_fA = b.dark_current_fA;
+  p.dark_sigma_ln = b.dark_sigma_ln;
+  p.dark_ea_eV = b.dark_ea_eV;
+  p.temperature_K = b.temperature_K;
+  p.reference_temperature_K = b.reference_temperature_K;
+  p.theta_on = b.theta_on;
+  p.theta_off = b.theta_off;
+  p.sigma_theta_on = b.sigma_theta_on;
+  p.sigma_theta_off = b.sigma_theta_off;
+  p.theta_onoff_corr = b.theta_onoff_corr;
+  p.theta_floor = b.theta_floor;
+  p.reset_mode = b.reset_mode == BSX_RESET_IDEAL ? ResetMode::kIdeal : ResetMode::kPhysical;
+  p.initial_phase = b.initial_phase == BSX_PHASE_SYMMETRIC ? InitialPhase::kSymmetric
+                    : b.initial_phase == BSX_PHASE_ZERO    ? InitialPhase::kZero
+                                                           : InitialPhase::kUniform;
+  p.refractory_s = b.refractory_s;
+  p.eps_ci = b.eps_ci;
+  p.leak_rate_hz = b.leak_rate_hz;
+  p.leak_sigma_log10 = b.leak_sigma_log10;
+  p.leak_ea_eV = b.leak_ea_eV;
+  p.tau_hp = b.tau_hp;
+  p.hot_fraction = b.hot_fraction;
+  p.readout_quantum_s = b.readout_quantum_s;
+  p.readout_delay_s = b.readout_delay_s;
+  p.clock_offset_ns = b.clock_offset_ns;
+  p.clock_drift_ppm = b.clock_drift_ppm;
+  p.eps_L = b.eps_L;
+  p.noise_leaf_level = b.noise_leaf_level;
+  p.noise_margin_z = b.noise_margin_z;
+  p.label_share_cut = b.label_share_cut;
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* ref_create(const bsx_params* b, uint32_t width, uint32_t height, uint64_t seed, uint32_t camera,
+                 int32_t groups, int32_t num_threads, int32_t fp32, char* err, size_t err_n) {
+  try {
+    auto r = std::make_unique<RefModel>();
+    std::vector<Waveform> g(static_cast<size_t>(groups), make_constant_waveform());
+    r->m = std::make_unique<SensorModel>(to_ref(*b), width, height, seed, std::move(g), camera, num_threads,
+                                         fp32 ? Precision::kFp32 : Precision::kFp64, Device::kCpu);
+    return r.release();
+  } catch (const std::exception& e) {
+    put_err(err, err_n, e.what());
+    return nullptr;
+  }
+}
+
+void ref_destroy(void* h) { delete static_cast<RefModel*>(h); }
+
+int ref_init(void* h, int64_t t0, const float* irr0, char* err, size_t err_n) {
+  try {
+    static_cast<RefModel*>(h)->m->init(t0, irr0);
+    return 0;
+  } catch (const std::exception& e) {
+    put_err(err, err_n, e.what());
+    return 1;
+  }
+}
+
+int ref_advance_runs(void* h, const uint32_t* ids, size_t n, const int64_t* t, const float* irr, size_t n_samples,
+                     const int64_t* row_start, const int32_t* row_count, int64_t t_end, char* err, size_t err_n) {
+  RefModel* r = static_cast<RefModel*>(h);
+  try {
+    r->ev = r->m->advance_runs_raw(ids, n, t, irr, n_samples, row_start, row_count, t_end);
+    return 0;
+  } catch (const std::invalid_argument& e) {
+    put_err(err, err_n, e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    put_err(err, err_n, e.what());
+    return 2;
+  }
+}
+
+size_t ref_event_count(void* h) { return static_cast<RefModel*>(h)->ev.size(); }
+
+// The sorted raw events: the reference's RawEvent has the layout of bsx_raw_event.
+void ref_events(void* h, bsx_raw_event* out) {
+  const RefModel* r = static_cast<RefModel*>(h);
+  static_assert(sizeof(RawEvent) == sizeof(bsx_raw_event), "RawEvent layout");
+  if (!r->ev.empty()) std::memcpy(out, r->ev.data(), r->ev.size() * sizeof(RawEvent));
+}
+
+void ref_reported(void* h, int64_t* out) {
+  const RefModel* r = static_cast<RefModel*>(h);
+  for (size_t i = 0; i < r->ev.size(); ++i) out[i] = r->m->reported_time(r->ev[i].t_cross);
+}
+
+void ref_maps(void* h, double* theta_on, double* theta_off, double* leak, double* dark, double* phi0) {
+  const SensorMaps& m = static_cast<RefModel*>(h)->m->maps();
+  for (size_t i = 0; i < m.pixels.size(); ++i) {
+    theta_on[i] = m.pixels[i].theta_on;
+    theta_off[i] = m.pixels[i].theta_off;
+    leak[i] = m.pixels[i].leak;
+    dark[i] = m.pixels[i].dark_fA;
+    phi0[i] = m.pixels[i].phi0;
+  }
+}
+
+// The states, 200 bytes per pixel in the reference's PixelState layout.
+void ref_states(void* h, uint8_t* out) {
+  const std::vector<PixelState> s = static_cast<RefModel*>(h)->m->states();
+  std::memcpy(out, s.data(), s.size() * sizeof(PixelState));
+}
+
+void ref_stats(void* h, uint64_t* out) {
+  const KernelStats& s = static_cast<RefModel*>(h)->m->stats();
+  const uint64_t v[10] = {s.substeps, s.substep_attempts, s.settled_substeps, s.cells,      s.refinements,
+                          s.leaf_intervals, s.fallbacks, s.input_clamps,     s.quarantines, s.pieces};
+  std::memcpy(out, v, sizeof(v));
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+"""Shared input generators and comparisons for the tests (seeded, small)."""
+import numpy as np
+
+from oracle import oracle as O
+
+
+def random_runs(rng, P, t0, t1, max_n=12, G=1, frac=0.8, start_at_t0=0.2, log_mu=3.0, log_sigma=1.5):
+    """Ragged runs for a random subset of pixels in random order: strictly increasing integer
+    sample times in [t0, t1] (some starting exactly at t0), log-normal irradiance."""
+    ids = np.nonzero(rng.random(P) < frac)[0].astype(np.uint32)
+    rng.shuffle(ids)
+    ts, irs, offs = [], [], [0]
+    for _ in ids:
+        n = int(rng.integers(0, max_n))
+        tt = np.unique(rng.integers(t0, t1 + 1, size=n)).astype(np.int64)
+        if len(tt) and rng.random() < start_at_t0:
+            tt[0] = t0
+            tt = np.unique(tt)
+        ts.append(tt)
+        irs.append(np.exp(rng.normal(log_mu, log_sigma, size=(len(tt), G))).astype(np.float32))
+        offs.append(offs[-1] + len(tt))
+    t = np.concatenate(ts) if ts else np.zeros(0, np.int64)
+    irr = np.concatenate(irs).reshape(-1, G) if irs else np.zeros((0, G), np.float32)
+    return ids, t, irr, np.array(offs, np.int64)
+
+
+def events_from_dict(ev, width):
+    """The product's events dict as the oracle's structured array (pixel, polarity, ...)."""
+    out = np.zeros(len(ev["t_cross_ns"]), O.EVENT_DTYPE)
+    out["pixel"] = ev["y"].astype(np.uint32) * width + ev["x"]
+    out["polarity"] = ev["p"]
+    out["label"] = ev["label"]
+    out["flags"] = ev["flags"]
+    out["t_cross"] = ev["t_cross_ns"]
+    out["ds"] = ev["ds"]
+    out["dn"] = ev["dn"]
+    out["dl"] = ev["dl"]
+    out["l_first"] = ev["l_first"]
+    return out
+
+
+def first_difference(a, b):
+    n = min(len(a), len(b))
+    for i in range(n):
+        if a[i] != b[i]:
+            return i, a[i], b[i]
+    return n, None, None

@@ -1,0 +1,190 @@
+"""GPU parity of the instantaneous log response: the HIP kernel through the C ABI against the oracle
+(the C restatement, itself pinned bit for bit to the compiled reference by test_oracle_vs_ref.py),
+on the same seeded inputs. Integer and event outputs must be bit-exact: event count, pixel,
+polarity, t_cross (int64 ns), label, flags, the fp32 increments and the reported times."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import events_from_dict, first_difference, random_runs
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "ideal": dict(),
+    "physical_leak": dict(reset_mode="physical", refractory_s=1e-4, leak_rate_hz=0.5, sigma_theta_on=0.03,
+                          sigma_theta_off=0.02, dark_sigma_ln=0.3),
+    "groups3_fast_leak": dict(leak_rate_hz=5.0, leak_sigma_log10=0.2, refractory_s=3e-5, theta_on=0.1,
+                              theta_off=0.12, eps_ci=0.001),
+    "symmetric_readout": dict(reset_mode="physical", refractory_s=0.0, initial_phase="symmetric", eps_ci=-0.002,
+                              readout_quantum_s=1e-6, readout_delay_s=5e-6, clock_drift_ppm=3.0),
+}
+
+
+def _params(case):
+    import ballasim_amd as bsx
+    p = bsx.preset("ideal")
+    for k, v in CASES[case].items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_instant_matches_oracle(case):
+    import ballasim_amd as bsx
+    G = 3 if case == "groups3_fast_leak" else 1
+    W, H, seed = 48, 32, 11
+    rng = np.random.default_rng(sum(case.encode()))
+    p = _params(case)
+    gpu = bsx.SensorModel(p, W, H, seed, ["constant"] * G, 0, device="cuda")
+    ref = O.OracleModel(p.c, W, H, seed, G, 0)
+    maps = gpu.sensor_maps()
+    om = ref.maps()
+    assert np.array_equal(maps["theta_on"].reshape(-1), om["theta_on"])
+    assert np.array_equal(maps["phi0"].reshape(-1), om["phi0"])
+    irr0 = np.exp(rng.normal(3, 1, size=(W * H, G))).astype(np.float32)
+    gpu.init(-1000, irr0)
+    ref.init(-1000, irr0)
+    t = -1000
+    total = 0
+    for call in range(4):
+        t1 = t + int(rng.integers(1, 3 * 2 ** 23))
+        ids, ts, ir, offs = random_runs(rng, W * H, t, t1, G=G)
+        ev = gpu.advance(ids, ts, ir, t1, offsets=offs)
+        want, rep = ref.advance_runs(ids, ts, ir, offs, t1)
+        got = events_from_dict(ev, W)
+        i, a, b = first_difference(got, want)
+        assert len(got) == len(want) and i == len(got), f"call {call}: first difference at {i}: gpu {a} oracle {b}"
+        assert np.array_equal(ev["t_reported_ns"], rep)
+        total += len(got)
+        t = t1
+    assert total > 1000
+    assert gpu.time_ns == t
+
+
+def test_padded_counts_and_unlisted_pixels():
+    """The padded form with counts, a call with no listed pixels (held input, leak fires), and a
+    zero-length call."""
+    import ballasim_amd as bsx
+    W, H = 40, 20
+    rng = np.random.default_rng(5)
+    p = _params("physical_leak")
+    p.leak_rate_hz = 20.0
+    gpu = bsx.SensorModel(p, W, H, 2, device="cuda")
+    ref = O.OracleModel(p.c, W, H, 2, 1, 0)
+    irr0 = np.full((W * H, 1), 50.0, np.float32)
+    gpu.init(0, irr0)
+    ref.init(0, irr0)
+    n, m = 300, 9
+    ids = rng.choice(W * H, n, replace=False).astype(np.uint32)
+    tt = np.sort(rng.choice(np.arange(0, 2 ** 22), size=(n, m), replace=False), axis=1).astype(np.int64)
+    irr = np.exp(rng.normal(4, 1, (n, m, 1))).astype(np.float32)
+    counts = rng.integers(0, m + 1, n).astype(np.int32)
+    ev = gpu.advance(ids, tt, irr, 2 ** 22, counts=counts)
+    want, _ = ref.advance_padded(ids, tt, irr, 2 ** 22, counts=counts)
+    assert np.array_equal(events_from_dict(ev, W), want)
+    empty = np.zeros(0, np.uint32)
+    for t_end in (2 ** 22, 3 * 2 ** 24):
+        ev = gpu.advance(empty, np.zeros((0, 0), np.int64), np.zeros((0, 0, 1), np.float32), t_end)
+        want, _ = ref.advance_padded(empty, np.zeros((0, 0), np.int64), np.zeros((0, 0, 1), np.float32), t_end)
+        assert np.array_equal(events_from_dict(ev, W), want)
+    assert len(want) > 0  # leak-only events of held pixels
+
+
+def test_state_roundtrip_and_slicing_invariance():
+    """save_state/load_state resume bit for bit; the saved fp32 state equals the oracle's."""
+    import ballasim_amd as bsx
+    W, H = 32, 16
+    rng = np.random.default_rng(9)
+    p = _params("physical_leak")
+    a = bsx.SensorModel(p, W, H, 4, device="cuda")
+    b = bsx.SensorModel(p, W, H, 4, device="cuda")
+    ref = O.OracleModel(p.c, W, H, 4, 1, 0)
+    irr0 = np.exp(rng.normal(3, 1, (W * H, 1))).astype(np.float32)
+    a.init(0, irr0)
+    ref.init(0, irr0)
+    ids, ts, ir, offs = random_runs(rng, W * H, 0, 2 ** 23)
+    a.advance(ids, ts, ir, 2 ** 23, offsets=offs)
+    ref.advance_runs(ids, ts, ir, offs, 2 ** 23)
+    blob = a.save_state()
+    st = np.frombuffer(blob[56:], O.REF_STATE_DTYPE)
+    os_ = ref.states()
+    for f in ("s3_ref", "dr_ref", "i_scale", "t_ref", "t_search", "blind_until", "flags"):
+        assert np.array_equal(st[f], os_[f]), f
+    b.load_state(blob)
+    ids2, ts2, ir2, offs2 = random_runs(rng, W * H, 2 ** 23, 2 ** 24)
+    ea = a.advance(ids2, ts2, ir2, 2 ** 24, offsets=offs2)
+    eb = b.advance(ids2, ts2, ir2, 2 ** 24, offsets=offs2)
+    for k in ea:
+        assert np.array_equal(ea[k], eb[k]), k
+
+
+def test_input_refusal_before_state_change():
+    import ballasim_amd as bsx
+    W, H = 8, 8
+    m = bsx.SensorModel("ideal", W, H, 1, device="cuda")
+    with pytest.raises(RuntimeError):
+        m.advance(np.zeros(1, np.uint32), np.zeros((1, 1), np.int64), np.ones((1, 1, 1), np.float32), 10)
+    m.init(0, np.ones((W * H, 1), np.float32))
+    bad = [
+        (np.array([64], np.uint32), np.array([[1]], np.int64), np.ones((1, 1, 1), np.float32), 10),
+        (np.array([1, 1], np.uint32), np.array([[1], [2]], np.int64), np.ones((2, 1, 1), np.float32), 10),
+        (np.array([1], np.uint32), np.array([[3, 3]], np.int64), np.ones((1, 2, 1), np.float32), 10),
+        (np.array([1], np.uint32), np.array([[11]], np.int64), np.ones((1, 1, 1), np.float32), 10),
+        (np.array([1], np.uint32), np.array([[5]], np.int64), np.full((1, 1, 1), np.nan, np.float32), 10),
+    ]
+    for ids, t, irr, t_end in bad:
+        with pytest.raises(ValueError):
+            m.advance(ids, t, irr, t_end)
+        assert m.initialized and m.time_ns == 0
+    with pytest.raises(ValueError):
+        m.advance(np.zeros(0, np.uint32), np.zeros((0, 0), np.int64), np.zeros((0, 0, 1), np.float32), -5)
+
+
+def test_event_buffer_overflow_reruns():
+    """A forced tiny event capacity: the call re-runs from its input state with a larger buffer and
+    gives the same events."""
+    import ballasim_amd as bsx
+    W, H = 32, 32
+    rng = np.random.default_rng(3)
+    p = _params("ideal")
+    a = bsx.SensorModel(p, W, H, 8, device="cuda")
+    b = bsx.SensorModel(p, W, H, 8, device="cuda")
+    a.set_launch_policy(event_capacity=7)
+    irr0 = np.exp(rng.normal(3, 1, (W * H, 1))).astype(np.float32)
+    a.init(0, irr0)
+    b.init(0, irr0)
+    ids, ts, ir, offs = random_runs(rng, W * H, 0, 2 ** 23)
+    ea = a.advance(ids, ts, ir, 2 ** 23, offsets=offs)
+    eb = b.advance(ids, ts, ir, 2 ** 23, offsets=offs)
+    assert len(ea["t_cross_ns"]) > 7
+    for k in ea:
+        assert np.array_equal(ea[k], eb[k]), k
+    assert a.stats()["overflow_reruns"] >= 1
+
+
+def test_advance_device_torch_runs():
+    """advance_device on runs resident in HBM (torch tensors), then download_events."""
+    import torch
+
+    import ballasim_amd as bsx
+    W, H = 64, 32
+    rng = np.random.default_rng(21)
+    p = _params("physical_leak")
+    a = bsx.SensorModel(p, W, H, 6, device="cuda")
+    ref = O.OracleModel(p.c, W, H, 6, 1, 0)
+    irr0 = np.exp(rng.normal(3, 1, (W * H, 1))).astype(np.float32)
+    a.init(0, irr0)
+    ref.init(0, irr0)
+    ids, ts, ir, offs = random_runs(rng, W * H, 0, 2 ** 23)
+    dev = torch.device("cuda:0")
+    tp = torch.from_numpy(ids.astype(np.int32)).to(dev)
+    rs = torch.from_numpy(offs[:-1].copy()).to(dev)
+    rc = torch.from_numpy(np.diff(offs).astype(np.int32)).to(dev)
+    tt = torch.from_numpy(ts).to(dev)
+    ii = torch.from_numpy(ir.reshape(-1)).to(dev)
+    torch.cuda.synchronize()
+    ptr, count = a.advance_device(tp, rs, rc, tt, ii, 2 ** 23)
+    ev = a.download_events(ptr, count)
+    want, _ = ref.advance_runs(ids, ts, ir, offs, 2 ** 23)
+    assert np.array_equal(events_from_dict(ev, W), want)
