@@ -99,6 +99,21 @@ int bsx_params_validate(const bsx_params* p) {
   });
 }
 
+int bsx_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t n_pixels, double* theta_on,
+                  double* theta_off, double* leak, double* dark_fA, double* phi0) {
+  return guarded([&] {
+    need_ptr(p, "params");
+    bsx::validate_ranges(*p);
+    const bsx::SensorMaps m = bsx::draw_sensor_maps(*p, seed, camera, n_pixels);
+    const size_t n = m.size();
+    if (theta_on != nullptr) std::memcpy(theta_on, m.theta_on.data(), n * 8);
+    if (theta_off != nullptr) std::memcpy(theta_off, m.theta_off.data(), n * 8);
+    if (leak != nullptr) std::memcpy(leak, m.leak.data(), n * 8);
+    if (dark_fA != nullptr) std::memcpy(dark_fA, m.dark_fA.data(), n * 8);
+    if (phi0 != nullptr) std::memcpy(phi0, m.phi0.data(), n * 8);
+  });
+}
+
 int bsx_sensor_create(const bsx_params* params, uint32_t width, uint32_t height, uint64_t seed,
                       const bsx_waveform* light_groups, int32_t n_groups, uint32_t camera, bsx_sensor** out) {
   return guarded([&] {

@@ -122,7 +122,7 @@ SensorParams preset(const std::string& name) {
   throw std::invalid_argument("unknown sensor preset '" + name + "' (known: prophesee_gen3, samsung_gen3, ideal)");
 }
 
-void validate(const SensorParams& p) {
+void validate_ranges(const SensorParams& p) {
   const std::string fam(p.family, strnlen(p.family, sizeof(p.family)));
   bool known = false;
   for (const std::string& n : preset_names()) known = known || n == fam;
@@ -176,6 +176,10 @@ void validate(const SensorParams& p) {
        "hot_rate_max_hz must be < 1 / refractory_s");
   need(p.readout_quantum_s == 0 || p.readout_delay_s >= p.readout_quantum_s,
        "readout_delay_s must be >= readout_quantum_s when quantization is on");
+}
+
+void validate(const SensorParams& p) {
+  validate_ranges(p);
   // Features of the reference this backend does not run: refused by name, never approximated.
   need(p.veiling_glare == 0, "veiling_glare > 0 is not supported by the bsx backend");
   need(p.tau_hp == 0, "the high-pass filter (tau_hp > 0) is not supported by the bsx backend");

@@ -23,7 +23,9 @@ using SensorParams = bsx_params;
 SensorParams preset(const std::string& name);
 std::vector<std::string> preset_names();
 
-// Throws std::invalid_argument naming the offending field.
+// Throws std::invalid_argument naming the offending field: the reference's rules (validate_ranges),
+// then the features this backend does not run (validate).
+void validate_ranges(const SensorParams& p);
 void validate(const SensorParams& p);
 
 inline bool noise_on(const SensorParams& p) { return p.noise != 0 && p.photoreceptor == BSX_PHOTORECEPTOR_CASCADE; }

@@ -174,6 +174,11 @@ int bsx_params_preset(const char* name, bsx_params* out);
 int bsx_preset_names(char* buf, size_t n);
 /* Reference: SensorParams::validate(), src/sensor/params.cpp:44-96. */
 int bsx_params_validate(const bsx_params* p);
+/* The frozen per-pixel maps of a camera, drawn on the host (no device needed). Reference:
+ * draw_sensor_maps(p, seed, camera, n_pixels), src/sensor/maps.h:57 / maps.cpp:78-156. Any output
+ * pointer may be null; each non-null one receives n_pixels values. */
+int bsx_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t n_pixels, double* theta_on,
+                  double* theta_off, double* leak, double* dark_fA, double* phi0);
 
 /* Reference: SensorModel::SensorModel(params, width, height, seed, light_groups, camera, ...,
  * Precision::kFp32, Device::kCuda), src/sensor/sensor_model.h:107-110 (num_threads, precision and

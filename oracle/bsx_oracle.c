@@ -386,6 +386,22 @@ static void draw_maps(orc_model* m) {
   }
 }
 
+void orc_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t n, double* theta_on,
+                   double* theta_off, double* leak, double* dark, double* phi0) {
+  orc_model m;
+  memset(&m, 0, sizeof(m));
+  m.p = *p;
+  m.seed = seed;
+  m.camera = camera;
+  m.P = n;
+  m.theta_on = theta_on;
+  m.theta_off = theta_off;
+  m.leak = leak;
+  m.dark = dark;
+  m.phi0 = phi0;
+  draw_maps(&m);
+}
+
 orc_model* orc_create(const bsx_params* p, uint32_t width, uint32_t height, uint64_t seed, uint32_t camera,
                       int32_t groups, int32_t num_threads, char* err, size_t err_n) {
   if (width == 0 || height == 0 || groups < 1 || groups > 8 || camera > 255) {

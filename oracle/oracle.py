@@ -56,6 +56,7 @@ def oracle_lib():
         lib.orc_create.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_int32, C.c_int32,
                                    C.c_char_p, C.c_size_t]
         lib.orc_destroy.argtypes = [P]
+        lib.orc_draw_maps.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P]
         lib.orc_init.argtypes = [P, C.c_int64, P]
         lib.orc_advance_runs.argtypes = [P, P, C.c_size_t, P, P, C.c_size_t, P, P, C.c_int64, C.c_char_p, C.c_size_t]
         lib.orc_event_count.restype = C.c_size_t
@@ -83,6 +84,14 @@ def oracle_lib():
         lib.orc_z_step.argtypes = [C.c_float] * 5
         _oracle = lib
     return _oracle
+
+
+def draw_maps(params, seed, camera, n):
+    """The oracle's frozen maps of n pixels."""
+    lib = oracle_lib()
+    arrs = [np.zeros(n) for _ in range(5)]
+    lib.orc_draw_maps(C.byref(params), seed, camera, n, *[_ptr(a) for a in arrs])
+    return dict(zip(("theta_on", "theta_off", "leak", "dark_fA", "phi0"), arrs))
 
 
 def ref_available():

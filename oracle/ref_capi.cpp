@@ -77,6 +77,68 @@ SensorParams to_ref(const bsx_params& b) {
 
 extern "C" {
 
+// The reference's preset(name) in bsx_params form (for the presets' parity test).
+int ref_preset(const char* name, bsx_params* b) {
+  try {
+    const SensorParams p = preset(name);
+    std::memset(b, 0, sizeof(*b));
+    std::strncpy(b->family, p.family.c_str(), sizeof(b->family) - 1);
+    b->photoreceptor = p.photoreceptor == Photoreceptor::kInstantaneous ? BSX_PHOTORECEPTOR_INSTANTANEOUS
+                                                                        : BSX_PHOTORECEPTOR_CASCADE;
+    b->noise = p.noise ? 1 : 0;
+    b->k = p.k;
+    b->tau_o = p.tau_o;
+    b->tau_sf = p.tau_sf;
+    b->sigma_o = p.sigma_o;
+    b->illuminance_scale = p.illuminance_scale;
+    b->veiling_glare = p.veiling_glare;
+    b->dark_current_fA = p.dark_current_fA;
+    b->dark_sigma_ln = p.dark_sigma_ln;
+    b->dark_ea_eV = p.dark_ea_eV;
+    b->temperature_K = p.temperature_K;
+    b->reference_temperature_K = p.reference_temperature_K;
+    b->theta_on = p.theta_on;
+    b->theta_off = p.theta_off;
+    b->sigma_theta_on = p.sigma_theta_on;
+    b->sigma_theta_off = p.sigma_theta_off;
+    b->theta_onoff_corr = p.theta_onoff_corr;
+    b->theta_floor = p.theta_floor;
+    b->reset_mode = p.reset_mode == ResetMode::kIdeal ? BSX_RESET_IDEAL : BSX_RESET_PHYSICAL;
+    b->initial_phase = p.initial_phase == InitialPhase::kSymmetric ? BSX_PHASE_SYMMETRIC
+                       : p.initial_phase == InitialPhase::kZero    ? BSX_PHASE_ZERO
+                                                                   : BSX_PHASE_UNIFORM;
+    b->refractory_s = p.refractory_s;
+    b->eps_ci = p.eps_ci;
+    b->leak_rate_hz = p.leak_rate_hz;
+    b->leak_sigma_log10 = p.leak_sigma_log10;
+    b->leak_ea_eV = p.leak_ea_eV;
+    b->tau_hp = p.tau_hp;
+    b->tau_hp_sigma_ln = p.tau_hp_sigma_ln;
+    b->hot_fraction = p.hot_fraction;
+    b->hot_rate_law = static_cast<int32_t>(p.hot_rate_law);
+    b->limiter = static_cast<int32_t>(p.limiter);
+    b->hot_rate_min_hz = p.hot_rate_min_hz;
+    b->hot_rate_max_hz = p.hot_rate_max_hz;
+    b->hot_rate_median_hz = p.hot_rate_median_hz;
+    b->hot_rate_sigma_ln = p.hot_rate_sigma_ln;
+    b->hot_leak_fraction = p.hot_leak_fraction;
+    b->hot_reference_current_fA = p.hot_reference_current_fA;
+    b->readout_quantum_s = p.readout_quantum_s;
+    b->readout_delay_s = p.readout_delay_s;
+    b->clock_offset_ns = p.clock_offset_ns;
+    b->clock_drift_ppm = p.clock_drift_ppm;
+    b->eps_L = p.eps_L;
+    b->noise_leaf_level = p.noise_leaf_level;
+    b->noise_diffusion = static_cast<int32_t>(p.noise_diffusion);
+    b->noise_margin_z = p.noise_margin_z;
+    b->initial_state = static_cast<int32_t>(p.initial_state);
+    b->label_share_cut = p.label_share_cut;
+    return 0;
+  } catch (const std::exception&) {
+    return 1;
+  }
+}
+
 void* ref_create(const bsx_params* b, uint32_t width, uint32_t height, uint64_t seed, uint32_t camera,
                  int32_t groups, int32_t num_threads, int32_t fp32, char* err, size_t err_n) {
   try {

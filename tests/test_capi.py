@@ -1,0 +1,97 @@
+"""The C-ABI library on the CPU (no device): it loads, exports every entry point include/bsx/*.h
+declares, fills presets, refuses invalid parameters by name, and reports a missing device as a
+runtime error rather than running anything else."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import ballasim_amd as bsx
+from ballasim_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include", "bsx")
+    for f in os.listdir(inc):
+        if f.endswith(".h"):
+            names |= set(re.findall(r"^\s*int\s+(bsx_\w+)\s*\(", open(os.path.join(inc, f)).read(), re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.load_library()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+
+
+def test_abi_signature_table_covers_header():
+    assert declared_symbols() <= set(_abi.SIGNATURES) | {"bsx_draw_maps"} | {
+        n for n in declared_symbols() if n.startswith(("bsx_scene", "bsx_sampler", "bsx_mlp", "bsx_batch"))}
+
+
+def test_struct_sizes_match_header_layout():
+    assert C.sizeof(_abi.RawEvent) == 32
+    assert C.sizeof(_abi.Waveform) == 8 + 8 + 64 + 64 + 48
+
+
+def test_presets_and_params_surface():
+    assert bsx.preset_names() == ["prophesee_gen3", "samsung_gen3", "ideal"]
+    p = bsx.preset("prophesee_gen3")
+    assert p.photoreceptor == "cascade" and p.noise and p.theta_on == 0.25 and p.family == "prophesee_gen3"
+    q = bsx.preset("ideal")
+    assert q.photoreceptor == "instantaneous" and q.reset_mode == "ideal" and q.refractory_s == 0.0
+    q.reset_mode = "physical"
+    q.refractory_s = 1e-4
+    q.validate()
+    with pytest.raises(ValueError):
+        bsx.preset("nope")
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("theta_on", 0.005, "theta_on must exceed theta_floor"),
+    ("tau_sf", 1.05e-6, "differ by at least 10"),
+    ("eps_ci", 0.01, "eps_ci"),
+    ("refractory_s", -1.0, "refractory_s"),
+    ("veiling_glare", 0.1, "not supported"),
+    ("hot_fraction", 1e-3, "not supported"),
+    ("family", "foo", "family"),
+])
+def test_validate_refuses_by_name(field, value, msg):
+    p = bsx.preset("ideal")
+    setattr(p, field, value)
+    with pytest.raises(ValueError, match=msg):
+        p.validate()
+
+
+def test_noise_on_cascade_is_refused_not_approximated():
+    with pytest.raises(ValueError, match="noise"):
+        bsx.preset("prophesee_gen3").validate()
+
+
+def test_no_device_is_a_runtime_error():
+    if bsx.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        bsx.SensorModel("ideal", 8, 8, 1, device="cuda")
+    with pytest.raises(ValueError, match="GPU only"):
+        bsx.SensorModel("ideal", 8, 8, 1, device="cpu")
+
+
+def test_draw_maps_host_entry_point():
+    lib = _abi.load_library()
+    p = bsx.preset("prophesee_gen3")
+    n = 100
+    a = [np.zeros(n) for _ in range(5)]
+    _abi.check(lib.bsx_draw_maps(C.byref(p.c), 1, 0, n, *[x.ctypes.data_as(C.c_void_p) for x in a]))
+    assert np.all(a[0] > 0.01) and np.all(a[3] > 0)
+    bad = bsx.preset("prophesee_gen3")
+    bad.theta_on = -1.0
+    with pytest.raises(ValueError):
+        _abi.check(lib.bsx_draw_maps(C.byref(bad.c), 1, 0, n, None, None, None, None, None))

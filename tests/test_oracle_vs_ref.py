@@ -1,0 +1,90 @@
+"""The oracle against the compiled reference itself (oracle/_ref, built from the reference's sources
+where they are present; skipped elsewhere): randomized parameter sets and inputs, bit-exact events,
+reported times, maps and states; the presets."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from ballasim_amd import _abi
+from oracle import oracle as O
+from tests.helpers import random_runs
+
+pytestmark = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref (the compiled reference) is not built")
+
+
+def preset(name):
+    lib = _abi.load_library()
+    p = _abi.Params()
+    _abi.check(lib.bsx_params_preset(name.encode(), C.byref(p)))
+    return p
+
+
+@pytest.mark.parametrize("name", ["prophesee_gen3", "samsung_gen3", "ideal"])
+def test_presets_equal_reference(name):
+    a = preset(name)
+    b = _abi.Params()
+    assert O.ref_lib().ref_preset(name.encode(), C.byref(b)) == 0
+    for f, _ in _abi.Params._fields_:
+        assert getattr(a, f) == getattr(b, f), f
+
+
+def _random_instant_params(rng):
+    p = preset("ideal")
+    p.reset_mode = int(rng.integers(0, 2))
+    p.refractory_s = float(rng.choice([0.0, 1e-5, 1e-4, 3e-4]))
+    p.leak_rate_hz = float(rng.choice([0.0, 0.1, 3.0, 30.0]))
+    p.leak_sigma_log10 = float(rng.uniform(0, 0.3))
+    p.theta_on = float(rng.uniform(0.08, 0.4))
+    p.theta_off = float(rng.uniform(0.08, 0.4))
+    p.sigma_theta_on = float(rng.uniform(0, 0.04))
+    p.sigma_theta_off = float(rng.uniform(0, 0.04))
+    p.theta_onoff_corr = float(rng.uniform(-1, 1))
+    p.dark_current_fA = float(np.exp(rng.uniform(-4, 2)))
+    p.dark_sigma_ln = float(rng.uniform(0, 0.5))
+    p.initial_phase = int(rng.integers(0, 3))
+    p.eps_ci = float(rng.uniform(-0.004, 0.004))
+    p.label_share_cut = float(rng.uniform(0.3, 0.7))
+    p.illuminance_scale = float(np.exp(rng.uniform(-2, 3)))
+    return p
+
+
+@pytest.mark.parametrize("trial", range(6))
+def test_instant_random_sweep(trial):
+    rng = np.random.default_rng(1000 + trial)
+    p = _random_instant_params(rng)
+    W, H, G = 20, 12, int(rng.integers(1, 4))
+    seed = int(rng.integers(0, 2 ** 63))
+    a = O.OracleModel(p, W, H, seed, G, 3, 2)
+    b = O.RefModel(p, W, H, seed, G, 3, 2, True)
+    irr0 = np.exp(rng.normal(2, 1.5, (W * H, G))).astype(np.float32)
+    t0 = int(rng.integers(-10 ** 9, 10 ** 9))
+    a.init(t0, irr0)
+    b.init(t0, irr0)
+    t = t0
+    for _ in range(3):
+        t1 = t + int(rng.integers(1, 3 * 2 ** 23))
+        ids, ts, ir, offs = random_runs(rng, W * H, t, t1, G=G, log_sigma=2.0)
+        ea, ra = a.advance_runs(ids, ts, ir, offs, t1)
+        eb, rb = b.advance_runs(ids, ts, ir, offs, t1)
+        assert np.array_equal(ea, eb) and np.array_equal(ra, rb)
+        t = t1
+    sa, sb = a.states(), b.states()
+    for f in ("s3_ref", "dr_ref", "t_ref", "t_search", "blind_until", "i_scale", "flags"):
+        assert np.array_equal(sa[f], sb[f]), f
+
+
+@pytest.mark.parametrize("name", ["prophesee_gen3", "samsung_gen3"])
+def test_maps_random_seeds(name):
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        p = preset(name)
+        p.theta_onoff_corr = float(rng.uniform(-1, 1))
+        p.initial_phase = int(rng.integers(0, 3))
+        p.temperature_K = float(rng.uniform(280, 320))
+        seed, cam = int(rng.integers(0, 2 ** 63)), int(rng.integers(0, 256))
+        ours = O.draw_maps(p, seed, cam, 3000)
+        p.noise = 0
+        ref = O.RefModel(p, 3000, 1, seed, 1, cam, 1, True).maps()
+        for k in ours:
+            assert np.array_equal(ours[k], ref[k]), k
