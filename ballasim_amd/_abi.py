@@ -98,8 +98,26 @@ class Info(C.Structure):
                 ("camera", C.c_uint32), ("seed", C.c_uint64)]
 
 
+class Shape(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("tex_seed", C.c_uint32), ("cx", C.c_double), ("cy", C.c_double),
+                ("vx", C.c_double), ("vy", C.c_double), ("angle", C.c_double), ("omega", C.c_double),
+                ("hx", C.c_double), ("hy", C.c_double), ("lux", C.c_float), ("contrast", C.c_float),
+                ("octaves", C.c_int32), ("tex_cell_px", C.c_float)]
+
+
+class View(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("origin_x", C.c_double), ("origin_y", C.c_double),
+                ("px_scale", C.c_double)]
+
+
+class SamplerRules(C.Structure):
+    _fields_ = [("d_px", C.c_double), ("refine_tol_log", C.c_float), ("dark", C.c_float), ("l_min", C.c_int32),
+                ("l_max", C.c_int32), ("max_refine_levels", C.c_int32), ("min_split_ns", C.c_uint32),
+                ("refine_signature", C.c_int32), ("reserved", C.c_int32), ("fixed_period_ns", C.c_int64)]
+
+
 P = C.c_void_p
-# name -> (restype, argtypes); every entry point of include/bsx/bsx_sensor.h.
+# name -> (restype, argtypes); every entry point of include/bsx/*.h.
 SIGNATURES = {
     "bsx_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
     "bsx_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
@@ -124,6 +142,13 @@ SIGNATURES = {
     "bsx_sensor_set_launch_policy": (C.c_int, [P, C.POINTER(LaunchPolicy)]),
     "bsx_sensor_info": (C.c_int, [P, C.POINTER(Info)]),
     "bsx_sensor_maps": (C.c_int, [P, P, P, P, P, P, P]),
+    "bsx_draw_maps": (C.c_int, [C.POINTER(Params), C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P]),
+    "bsx_scene_create": (C.c_int, [C.POINTER(Shape), C.c_int32, C.c_float, C.POINTER(P)]),
+    "bsx_scene_destroy": (C.c_int, [P]),
+    "bsx_copy_to_host": (C.c_int, [P, P, C.c_size_t]),
+    "bsx_scene_render": (C.c_int, [P, C.POINTER(View), C.c_int64, P]),
+    "bsx_scene_sample_window": (C.c_int, [P, C.POINTER(View), C.POINTER(SamplerRules), C.c_int64, C.c_int32, P,
+                                          C.POINTER(DeviceRuns), C.POINTER(C.c_uint64)]),
 }
 
 _lib = None

@@ -22,6 +22,8 @@ SOURCES = [
     "sensor/maps.cpp",
     "sensor/model.cpp",
     "sensor/kernels.hip",
+    "scene/scene_host.cpp",
+    "scene/scene.hip",
 ]
 
 COMMON = [

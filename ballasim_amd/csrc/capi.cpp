@@ -54,6 +54,12 @@ void need_ptr(const void* p, const char* what) {
 
 extern "C" {
 
+// The other C-ABI translation units (scene_host.cpp) report through the same per-thread message.
+int bsx_internal_set_error(int code, const char* msg) {
+  g_last_error = msg;
+  return code;
+}
+
 int bsx_last_error(char* buf, size_t n) {
   if (buf != nullptr && n > 0) {
     const size_t k = std::min(n - 1, g_last_error.size());

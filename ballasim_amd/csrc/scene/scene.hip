@@ -1,0 +1,237 @@
+// Kernels of the procedural scene and the sample-time controller (scene_dev.h): a frame render, the
+// two passes of a window's sampling (count, then write at the scanned offsets) and an exclusive scan.
+#include <hip/hip_runtime.h>
+
+#include "scene_dev.h"
+#include "scene_kernels.h"
+
+namespace bsx {
+
+namespace {
+
+struct RulesDev {
+  double d_px;
+  float tol, dark;
+  int32_t l_min, l_max, max_levels;
+  uint32_t min_split;
+  int32_t signature;
+  int64_t fixed_period;
+};
+
+__global__ void render_kernel(SceneDev sc, ViewDev view, double t, float* out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= view.width * view.height) return;
+  const double x = view.ox + (static_cast<double>(p % view.width) + 0.5) * view.scale;
+  const double y = view.oy + (static_cast<double>(p / view.width) + 0.5) * view.scale;
+  uint64_t mask[kMaskWords];
+  for (int w = 0; w < kMaskWords; ++w) mask[w] = 0;
+  for (int i = 0; i < sc.n; ++i) mask[i >> 6] |= 1ull << (i & 63);
+  out[p] = eval_scene(sc, mask, x, y, t).e;
+}
+
+// One pixel's samples over (t_w, t_w + W]: counted (kWrite false) or written from out_base.
+template <bool kWrite>
+__device__ uint32_t sample_pixel(const SceneDev& sc, const ViewDev& view, const RulesDev& r, int64_t t_w, uint32_t p,
+                                 int64_t* t_out, float* irr_out) {
+  const double x = view.ox + (static_cast<double>(p % view.width) + 0.5) * view.scale;
+  const double y = view.oy + (static_cast<double>(p / view.width) + 0.5) * view.scale;
+  const double tw_s = static_cast<double>(t_w) * 1e-9;
+  uint64_t mask[kMaskWords];
+  const float v_hat = plan_pixel(sc, x, y, tw_s, tw_s + kWindowNs * 1e-9, mask);
+  uint32_t count = 0;
+  const auto emit = [&](uint32_t dt, float e) {
+    if constexpr (kWrite) {
+      t_out[count] = t_w + dt;
+      irr_out[count] = e;
+    }
+    ++count;
+  };
+  const auto at = [&](uint32_t dt) { return eval_scene(sc, mask, x, y, tw_s + static_cast<double>(dt) * 1e-9); };
+  if (r.fixed_period > 0) {
+    for (int64_t d = r.fixed_period; d <= kWindowNs; d += r.fixed_period) emit(static_cast<uint32_t>(d), at(d).e);
+    return count;
+  }
+  if (v_hat <= 0.0f) return 0;  // static: no shape reaches the pixel this window
+  // Level: clamp(ceil(log2(v_hat W / d_px)), l_min, l_max), v_hat in camera px/s.
+  const double lv = ceil(log2(static_cast<double>(v_hat) / view.scale * (kWindowNs * 1e-9) / r.d_px));
+  const int level = lv != lv ? r.l_min : static_cast<int>(fmin(fmax(lv, static_cast<double>(r.l_min)),
+                                                               static_cast<double>(r.l_max)));
+  const uint32_t n = 1u << level, step = 1u << (kWindowLog2 - level);
+  const bool mid = r.tol > 0.0f;
+  // Sliding lattice window: slots j-2, j-1, j, j+1.
+  Sample sm2{0.0f, 0u}, sm1 = at(0), s0 = at(step), sp1{0.0f, 0u};
+  if (n >= 2) sp1 = at(2 * step);
+  struct Node {
+    uint32_t lo, hi;
+    float e_lo, e_hi;
+    uint32_t s_lo, s_hi;
+    int32_t depth;  // 0: emit hi's... unused; > 0: refine at this depth; -1: emit (lo, e_lo)
+  };
+  for (uint32_t j = 1; j <= n; ++j) {
+    const uint32_t lo = (j - 1) * step, hi = j * step;
+    const bool s = r.signature && sm1.sig != s0.sig;
+    bool m = false;
+    if (mid) {
+      m = (j + 1 <= n && midpoint_eta(r.dark, sm1.e, s0.e, sp1.e) > r.tol) ||
+          (j >= 2 && midpoint_eta(r.dark, sm2.e, sm1.e, s0.e) > r.tol);
+    }
+    if ((s || m) && step >= r.min_split && r.max_levels > 0) {
+      Node stack[16];
+      int sp = 0;
+      stack[sp++] = Node{lo, hi, sm1.e, s0.e, sm1.sig, s0.sig, 1};
+      while (sp > 0) {
+        const Node nd = stack[--sp];
+        if (nd.depth < 0) {
+          emit(nd.lo, nd.e_lo);
+          continue;
+        }
+        const uint32_t g = (nd.lo + nd.hi) >> 1;
+        const Sample sg = at(g);
+        const bool s1 = r.signature && nd.s_lo != sg.sig, s2 = r.signature && sg.sig != nd.s_hi;
+        const bool bad = mid && midpoint_eta(r.dark, nd.e_lo, sg.e, nd.e_hi) > r.tol;
+        const bool p1 = (s1 || bad) && g - nd.lo >= r.min_split;
+        const bool p2 = (s2 || bad) && nd.hi - g >= r.min_split;
+        const bool deeper = nd.depth < r.max_levels;
+        if (p2 && deeper) stack[sp++] = Node{g, nd.hi, sg.e, nd.e_hi, sg.sig, nd.s_hi, nd.depth + 1};
+        stack[sp++] = Node{g, 0u, sg.e, 0.0f, 0u, 0u, -1};
+        if (p1 && deeper) stack[sp++] = Node{nd.lo, g, nd.e_lo, sg.e, nd.s_lo, sg.sig, nd.depth + 1};
+      }
+    }
+    emit(hi, s0.e);
+    sm2 = sm1;
+    sm1 = s0;
+    s0 = sp1;
+    if (j + 2 <= n) sp1 = at((j + 2) * step);
+  }
+  return count;
+}
+
+__global__ __launch_bounds__(256) void count_kernel(SceneDev sc, ViewDev view, RulesDev r, int64_t t_w,
+                                                    int32_t* counts) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= view.width * view.height) return;
+  counts[p] = static_cast<int32_t>(sample_pixel<false>(sc, view, r, t_w, p, nullptr, nullptr));
+}
+
+__global__ __launch_bounds__(256) void write_kernel(SceneDev sc, ViewDev view, RulesDev r, int64_t t_w,
+                                                    const int64_t* row_start, int64_t* t, float* irr) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= view.width * view.height) return;
+  const int64_t s = row_start[p];
+  sample_pixel<true>(sc, view, r, t_w, p, t + s, irr + s);
+}
+
+// Exclusive scan of int32 counts into int64 offsets: per-block scans (wave shuffles, then the
+// block's wave totals through LDS), the block totals scanned by one block, then added back.
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
+  __shared__ int64_t wave_sums[kScanBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int64_t inc = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t o = __shfl_up(inc, off);
+    if (lane >= off) inc += o;
+  }
+  if (lane == 63) wave_sums[wave] = inc;
+  __syncthreads();
+  if (wave == 0) {
+    int64_t w = lane < kScanBlock / 64 ? wave_sums[lane] : 0;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t o = __shfl_up(w, off);
+      if (lane >= off) w += o;
+    }
+    if (lane < kScanBlock / 64) wave_sums[lane] = w;
+  }
+  __syncthreads();
+  const int64_t before = wave == 0 ? 0 : wave_sums[wave - 1];
+  *total = wave_sums[kScanBlock / 64 - 1];
+  __syncthreads();
+  return before + inc - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_blocks_kernel(const int32_t* counts, size_t n, int64_t* out,
+                                                                 int64_t* block_sums) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * kScanBlock + threadIdx.x;
+  const int64_t v = i < n ? counts[i] : 0;
+  int64_t total;
+  const int64_t ex = block_exclusive_scan(v, &total);
+  if (i < n) out[i] = ex;
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_sums_kernel(int64_t* block_sums, size_t n_blocks, int64_t* total) {
+  // One block scans the block totals in chunks of kScanBlock, carrying the running sum.
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (size_t base = 0; base < n_blocks; base += kScanBlock) {
+    const size_t i = base + threadIdx.x;
+    const int64_t v = i < n_blocks ? block_sums[i] : 0;
+    int64_t chunk;
+    const int64_t ex = block_exclusive_scan(v, &chunk);
+    const int64_t c = carry;
+    if (i < n_blocks) block_sums[i] = c + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry = c + chunk;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void scan_add_kernel(int64_t* out, size_t n, const int64_t* block_sums) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * kScanBlock + threadIdx.x;
+  if (i < n) out[i] += block_sums[blockIdx.x];
+}
+
+RulesDev rules_dev(const bsx_sampler_rules& r) {
+  RulesDev d{};
+  d.d_px = r.d_px;
+  d.tol = r.refine_tol_log;
+  d.dark = r.dark;
+  d.l_min = r.l_min;
+  d.l_max = r.l_max;
+  d.max_levels = r.max_refine_levels;
+  d.min_split = r.min_split_ns;
+  d.signature = r.refine_signature;
+  d.fixed_period = r.fixed_period_ns;
+  return d;
+}
+
+}  // namespace
+
+hipError_t launch_render(const SceneDev& sc, const ViewDev& view, int64_t t_ns, float* out, hipStream_t stream) {
+  const size_t P = static_cast<size_t>(view.width) * view.height;
+  render_kernel<<<static_cast<unsigned>((P + 255) / 256), 256, 0, stream>>>(sc, view, static_cast<double>(t_ns) * 1e-9,
+                                                                           out);
+  return hipGetLastError();
+}
+
+hipError_t launch_count(const SceneDev& sc, const ViewDev& view, const bsx_sampler_rules& r, int64_t t_w,
+                        int32_t* counts, hipStream_t stream) {
+  const size_t P = static_cast<size_t>(view.width) * view.height;
+  count_kernel<<<static_cast<unsigned>((P + 255) / 256), 256, 0, stream>>>(sc, view, rules_dev(r), t_w, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_write(const SceneDev& sc, const ViewDev& view, const bsx_sampler_rules& r, int64_t t_w,
+                        const int64_t* row_start, int64_t* t, float* irr, hipStream_t stream) {
+  const size_t P = static_cast<size_t>(view.width) * view.height;
+  write_kernel<<<static_cast<unsigned>((P + 255) / 256), 256, 0, stream>>>(sc, view, rules_dev(r), t_w, row_start, t,
+                                                                          irr);
+  return hipGetLastError();
+}
+
+size_t scan_scratch_count(size_t n) { return (n + kScanBlock - 1) / kScanBlock + 1; }
+
+hipError_t launch_exclusive_scan(const int32_t* counts, size_t n, int64_t* out, int64_t* scratch, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = (n + kScanBlock - 1) / kScanBlock;
+  int64_t* total = scratch + blocks;
+  scan_blocks_kernel<<<static_cast<unsigned>(blocks), kScanBlock, 0, stream>>>(counts, n, out, scratch);
+  scan_sums_kernel<<<1, kScanBlock, 0, stream>>>(scratch, blocks, total);
+  scan_add_kernel<<<static_cast<unsigned>(blocks), kScanBlock, 0, stream>>>(out, n, scratch);
+  return hipGetLastError();
+}
+
+}  // namespace bsx

@@ -1,0 +1,201 @@
+// The scene object and its C ABI (include/bsx/bsx_scene.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/bsx/bsx_scene.h"
+#include "../sensor/model.h"
+#include "scene_kernels.h"
+
+namespace bsx {
+
+namespace {
+
+struct WindowSlot {
+  DeviceBuffer counts, row_start, t, irr;
+};
+
+}  // namespace
+
+class Scene {
+ public:
+  Scene(const bsx_shape* shapes, int32_t n, float background) : n_(n), background_(background) {
+    if (n < 0 || n > kMaxShapes) throw std::invalid_argument("bsx_scene_create: 0 to 256 shapes");
+    if (!(background > 0.0f) || !std::isfinite(background)) {
+      throw std::invalid_argument("bsx_scene_create: background irradiance must be finite and > 0");
+    }
+    std::vector<DevShape> d(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+      const bsx_shape& s = shapes[i];
+      if (!(s.hx > 0) || (s.kind == BSX_SHAPE_RECT && !(s.hy > 0)) || !(s.lux > 0.0f) || !(s.contrast >= 0.0f) ||
+          s.contrast >= 1.0f || s.octaves < 0 || s.octaves > 16 || !(s.tex_cell_px > 0.0f) ||
+          (s.kind != BSX_SHAPE_RECT && s.kind != BSX_SHAPE_DISK)) {
+        throw std::invalid_argument("bsx_scene_create: shape " + std::to_string(i) + " is invalid");
+      }
+      DevShape& o = d[i];
+      o.cx = s.cx;
+      o.cy = s.cy;
+      o.vx = s.vx;
+      o.vy = s.vy;
+      o.angle = s.angle;
+      o.omega = s.omega;
+      o.hx = static_cast<float>(s.hx);
+      o.hy = static_cast<float>(s.kind == BSX_SHAPE_DISK ? s.hx : s.hy);
+      o.radius = static_cast<float>(s.kind == BSX_SHAPE_DISK ? s.hx : std::sqrt(s.hx * s.hx + s.hy * s.hy));
+      o.lux = s.lux;
+      o.contrast = s.contrast;
+      o.inv_cell = 1.0f / s.tex_cell_px;
+      o.speed = static_cast<float>(std::sqrt(s.vx * s.vx + s.vy * s.vy) + std::fabs(s.omega) * o.radius);
+      o.seed = s.tex_seed;
+      o.kind = s.kind;
+      o.octaves = s.octaves;
+    }
+    shapes_.ensure(std::max<size_t>(1, d.size()) * sizeof(DevShape));
+    if (n > 0) hip_check(hipMemcpy(shapes_.get(), d.data(), d.size() * sizeof(DevShape), hipMemcpyHostToDevice), "shapes");
+  }
+
+  SceneDev dev() const { return SceneDev{shapes_.as<const DevShape>(), n_, background_}; }
+
+  static ViewDev view_of(const bsx_view& v) {
+    if (v.width == 0 || v.height == 0 || !(v.px_scale > 0)) throw std::invalid_argument("bsx_view: bad size or scale");
+    return ViewDev{v.width, v.height, v.origin_x, v.origin_y, v.px_scale};
+  }
+
+  void render(const bsx_view& view, int64_t t, float* out) {
+    const ViewDev v = view_of(view);
+    const size_t P = static_cast<size_t>(v.width) * v.height;
+    frame_.ensure(P * 4);
+    hip_check(launch_render(dev(), v, t, frame_.as<float>(), nullptr), "render");
+    hip_check(hipMemcpy(out, frame_.get(), P * 4, hipMemcpyDeviceToHost), "download frame");
+  }
+
+  bsx_device_runs sample(const bsx_view& view, const bsx_sampler_rules& r, int64_t t_w, int32_t slot,
+                         hipStream_t stream, uint64_t* total_out) {
+    const ViewDev v = view_of(view);
+    if ((t_w & (kWindowNs - 1)) != 0) throw std::invalid_argument("sample_window: t_w must be a multiple of 2^23 ns");
+    if (r.fixed_period_ns < 0 || (r.fixed_period_ns == 0 && !(r.d_px > 0)) || r.l_min < 0 || r.l_max > 13 ||
+        r.l_min > r.l_max || r.max_refine_levels < 0 || r.max_refine_levels > 7) {
+      throw std::invalid_argument("sample_window: rules need d_px > 0, 0 <= l_min <= l_max <= 13, "
+                                  "0 <= max_refine_levels <= 7");
+    }
+    if (r.fixed_period_ns > 0 && r.fixed_period_ns < 64) {
+      throw std::invalid_argument("sample_window: fixed_period_ns must be >= 64");
+    }
+    if (slot < 0 || slot > 100000) throw std::invalid_argument("sample_window: slot out of range");
+    if (static_cast<size_t>(slot) >= slots_.size()) slots_.resize(static_cast<size_t>(slot) + 1);
+    if (!slots_[slot]) slots_[slot] = std::make_unique<WindowSlot>();
+    WindowSlot& w = *slots_[slot];
+    const size_t P = static_cast<size_t>(v.width) * v.height;
+    w.counts.ensure(P * 4);
+    w.row_start.ensure(P * 8);
+    scan_.ensure(scan_scratch_count(P) * 8);
+    if (ids_.bytes() < P * 4) {
+      std::vector<uint32_t> iota(P);
+      for (size_t i = 0; i < P; ++i) iota[i] = static_cast<uint32_t>(i);
+      ids_.ensure(P * 4);
+      hip_check(hipMemcpy(ids_.get(), iota.data(), P * 4, hipMemcpyHostToDevice), "ids");
+    }
+    hip_check(launch_count(dev(), v, r, t_w, w.counts.as<int32_t>(), stream), "count samples");
+    hip_check(launch_exclusive_scan(w.counts.as<int32_t>(), P, w.row_start.as<int64_t>(), scan_.as<int64_t>(), stream),
+              "scan");
+    int64_t total = 0;
+    const size_t blocks = scan_scratch_count(P) - 1;
+    hip_check(hipMemcpyAsync(&total, scan_.as<int64_t>() + blocks, 8, hipMemcpyDeviceToHost, stream), "total");
+    hip_check(hipStreamSynchronize(stream), "sample window");
+    w.t.ensure(std::max<int64_t>(total, 1) * 8);
+    w.irr.ensure(std::max<int64_t>(total, 1) * 4);
+    hip_check(launch_write(dev(), v, r, t_w, w.row_start.as<int64_t>(), w.t.as<int64_t>(), w.irr.as<float>(), stream),
+              "write samples");
+    hip_check(hipStreamSynchronize(stream), "sample window");
+    if (total_out != nullptr) *total_out = static_cast<uint64_t>(total);
+    bsx_device_runs out{};
+    out.pixel = ids_.as<const uint32_t>();
+    out.row_start = w.row_start.as<const int64_t>();
+    out.row_count = w.counts.as<const int32_t>();
+    out.n = P;
+    out.t = w.t.as<const int64_t>();
+    out.irr = w.irr.as<const float>();
+    out.n_samples = static_cast<size_t>(total);
+    return out;
+  }
+
+ private:
+  int32_t n_;
+  float background_;
+  DeviceBuffer shapes_, frame_, ids_, scan_;
+  std::vector<std::unique_ptr<WindowSlot>> slots_;
+};
+
+}  // namespace bsx
+
+struct bsx_scene {
+  bsx::Scene* s;
+};
+
+// Status mapping shared with capi.cpp's conventions.
+extern "C" int bsx_internal_set_error(int code, const char* msg);
+
+template <class F>
+static int scene_guarded(F&& f) {
+  try {
+    f();
+    return BSX_OK;
+  } catch (const std::invalid_argument& e) {
+    return bsx_internal_set_error(BSX_E_INVALID, e.what());
+  } catch (const std::logic_error& e) {
+    return bsx_internal_set_error(BSX_E_LOGIC, e.what());
+  } catch (const std::exception& e) {
+    return bsx_internal_set_error(BSX_E_RUNTIME, e.what());
+  }
+}
+
+extern "C" {
+
+int bsx_scene_create(const bsx_shape* shapes, int32_t n_shapes, float background, bsx_scene** out) {
+  return scene_guarded([&] {
+    if (out == nullptr || (n_shapes > 0 && shapes == nullptr)) throw std::invalid_argument("bsx_scene_create: null");
+    *out = nullptr;
+    auto* s = new bsx::Scene(shapes, n_shapes, background);
+    *out = new bsx_scene{s};
+  });
+}
+
+int bsx_scene_destroy(bsx_scene* s) {
+  return scene_guarded([&] {
+    if (s == nullptr) return;
+    delete s->s;
+    delete s;
+  });
+}
+
+int bsx_copy_to_host(void* dst, const void* src, size_t bytes) {
+  return scene_guarded([&] {
+    if (bytes == 0) return;
+    if (dst == nullptr || src == nullptr) throw std::invalid_argument("bsx_copy_to_host: null");
+    bsx::hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "copy to host");
+  });
+}
+
+int bsx_scene_render(bsx_scene* s, const bsx_view* view, int64_t t_ns, float* out) {
+  return scene_guarded([&] {
+    if (s == nullptr || view == nullptr || out == nullptr) throw std::invalid_argument("bsx_scene_render: null");
+    s->s->render(*view, t_ns, out);
+  });
+}
+
+int bsx_scene_sample_window(bsx_scene* s, const bsx_view* view, const bsx_sampler_rules* rules, int64_t t_w,
+                            int32_t slot, void* stream, bsx_device_runs* out, uint64_t* total_samples) {
+  return scene_guarded([&] {
+    if (s == nullptr || view == nullptr || rules == nullptr || out == nullptr) {
+      throw std::invalid_argument("bsx_scene_sample_window: null");
+    }
+    *out = s->s->sample(*view, *rules, t_w, slot, static_cast<hipStream_t>(stream), total_samples);
+  });
+}
+
+}  // extern "C"

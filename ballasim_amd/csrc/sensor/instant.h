@@ -10,7 +10,13 @@
 // reset is physical (blind on (t_c, t_c + T_r], re-reference at release) or ideal (r += p theta).
 //
 // Every arithmetic step is the reference's, in its order, because the fp32 results must equal the
-// reference's fp32 CPU path bit for bit (events, crossing times, increments and state).
+// reference's fp32 CPU path bit for bit (events, crossing times, increments and state). What differs
+// is the data movement, which leaves results unchanged:
+//   - segments are walked once, in order, with both end samples in registers (the reference
+//     re-reads them per evaluation), and the next sample is loaded one segment ahead so its memory
+//     latency hides behind the current segment's work;
+//   - the live-group scan is dropped: a dead group adds scale x (+-0) to a sum that starts at the
+//     dark current > 0, which leaves it bit for bit unchanged.
 #pragma once
 
 #include "../core/pmath.h"
@@ -27,9 +33,11 @@ struct InstantLane {
   int64_t refractory_ns;
   int G;
   bool ideal_reset;
-  // The call's samples, augmented by the held input at the model time (sample 0).
+  // The call's input run, and its segments' prepared values (prep[j]: the segment ending at input
+  // sample j; null: every search is exact).
   const int64_t* ts;
   const float* irr;
+  const InstantPrep* prep;
   int n, skip;
   int64_t t_state;
   // State.
@@ -37,64 +45,26 @@ struct InstantLane {
   float i_scale, s3_ref, n3_ref, dr_ref;
   int64_t t_ref, blind_until, t_search;
   uint32_t flags;
-  uint32_t live;
+  // The current segment [ta, tb) with its end values (tb = kNoBreakpoint: the held segment).
+  int64_t ta, tb;
+  float va[kMaxG], vb[kMaxG];
   // Counters.
   uint32_t n_substeps, n_pieces, n_leaves;
 
-  BSX_HD BSX_INLINE int n_aug() const { return 1 + n - skip; }
-  BSX_HD BSX_INLINE int64_t sample_time(int i) const { return i == 0 ? t_state : ts[i - 1 + skip]; }
   BSX_HD BSX_INLINE float clampv(float v) const { return v >= 0.0f ? (v <= irr_max ? v : irr_max) : 0.0f; }
-  BSX_HD BSX_INLINE float sample_value(int i, int g) const {
-    if (i == 0 && skip == 0) return held[g];
-    return clampv(irr[static_cast<int64_t>(i - 1 + skip) * G + g]);
-  }
 
-  // Live groups: a non-zero value somewhere in the augmented samples (a dead group adds exactly 0).
-  BSX_HD BSX_INLINE void scan_input() {
-    const uint32_t all = (1u << G) - 1u;
-    uint32_t m = 0;
-    if (skip == 0) {
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < G && held[g] != 0.0f) m |= 1u << g;
-    }
-    for (int i = 0; i < n && m != all; ++i) {
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < G && clampv(irr[static_cast<int64_t>(i) * G + g]) != 0.0f) m |= 1u << g;
-    }
-    live = m;
-  }
-
-  BSX_HD BSX_INLINE int segment_of(int seg, int64_t t) const {
-    const int last = n_aug() - 1;
-    while (seg < last && sample_time(seg + 1) <= t) ++seg;
-    return seg;
-  }
-  BSX_HD BSX_INLINE int64_t segment_end(int seg) const {
-    return seg + 1 < n_aug() ? sample_time(seg + 1) : kNoBreakpoint;
-  }
-
-  // I(t) in segment seg: dark + sum over live groups of s times the linear interpolation.
-  BSX_HD BSX_INLINE float photocurrent(int seg, int64_t t) const {
+  // I(t) in the current segment: dark + sum over groups of s times the linear interpolation (or the
+  // held value in the last segment), as the reference's photocurrent/group_value.
+  BSX_HD BSX_INLINE float photocurrent(int64_t t) const {
     float I = dark;
-    const bool held_seg = seg + 1 >= n_aug();
-    int64_t t0 = 0, t1 = 0;
-    float f = 0.0f;
-    if (!held_seg) {
-      t0 = sample_time(seg);
-      t1 = sample_time(seg + 1);
-      f = static_cast<float>(t - t0) / static_cast<float>(t1 - t0);
+    if (tb == kNoBreakpoint) {
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < G) I = I + scale * va[g];
+      return I;
     }
-    for (int g = 0; g < kMaxG; ++g) {
-      if (g >= G || !((live >> g) & 1u)) continue;
-      float v;
-      if (held_seg) {
-        v = sample_value(seg, g);
-      } else {
-        const float v0 = sample_value(seg, g), v1 = sample_value(seg + 1, g);
-        v = v0 + (v1 - v0) * f;
-      }
-      I = I + scale * v;
-    }
+    const float f = static_cast<float>(t - ta) / static_cast<float>(tb - ta);
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < G) I = I + scale * (va[g] + (vb[g] - va[g]) * f);
     return I;
   }
 
@@ -147,38 +117,36 @@ struct InstantLane {
     return h.found;
   }
 
-  // First integer t in [lo, hi) of segment seg where Delta fires: time-ordered depth-first bisection
-  // of (lo, hi - 1]. The pending right siblings are kept as their right ends only: in time order a
-  // popped node always starts where the last finished one ended.
-  BSX_HD BSX_INLINE Hit search(int seg, int64_t lo, int64_t hi) {
-    const float I_lo = photocurrent(seg, lo);
+  // First integer t in [lo, hi) of the current segment where Delta fires: time-ordered depth-first
+  // bisection of (lo, hi - 1]. The pending right siblings are kept as their right ends only: in time
+  // order a popped node always starts where the last finished one ended.
+  BSX_HD BSX_INLINE Hit search(int64_t lo, int64_t hi) {
+    const float I_lo = photocurrent(lo);
     Hit out{false, false, lo, log_ratio(I_lo)};
     if (fires(lo, out)) return out;
     if (hi - lo == 1) return out;
-    int64_t pend_c[32];
-    float pend_Ic[32];
-    int sp = 0;
     int64_t a = lo, c = hi - 1;
-    float Ia = I_lo, Ic = photocurrent(seg, hi - 1);
+    float Ia = I_lo, Ic = photocurrent(hi - 1);
+    int64_t pend_c[24];  // a piece spans < 2^23 ns: at most 23 pending right siblings
+    float pend_Ic[24];
+    int sp = 0;
     while (true) {
-      bool descend = false;
       if (c - a == 1) {
         out.t = c;
         out.x = log_ratio(Ic);
         if (fires(c, out)) return out;
       } else {
         ++n_leaves;
-        if (may_fire(a, c, Ia, Ic)) descend = true;
-      }
-      if (descend) {
-        const int64_t m = a + (c - a) / 2;
-        const float Im = photocurrent(seg, m);
-        pend_c[sp] = c;
-        pend_Ic[sp] = Ic;
-        ++sp;
-        c = m;
-        Ic = Im;
-        continue;
+        if (may_fire(a, c, Ia, Ic)) {
+          const int64_t m = a + (c - a) / 2;
+          const float Im = photocurrent(m);
+          pend_c[sp] = c;
+          pend_Ic[sp] = Ic;
+          ++sp;
+          c = m;
+          Ic = Im;
+          continue;
+        }
       }
       if (sp == 0) break;
       --sp;
@@ -244,37 +212,136 @@ struct InstantLane {
     }
   }
 
-  // The call: pieces of one sample segment, cut at multiples of 2^23 ns.
+  using Prep = InstantPrep;
+
+  // The segment's values as segment_prep forms them (also the check of the fast path on the host).
+  BSX_HD BSX_INLINE Prep prep_segment() const {
+    constexpr float eps = FloatEps<float>::value;
+    Prep q;
+    const float Ia = photocurrent(ta), Ic = photocurrent(tb - 1);
+    q.x_a = log_ratio(Ia);
+    q.x_c = log_ratio(Ic);
+    q.x_hi = nan_<float>();
+    q.x_lo = nan_<float>();
+    if (leak != 0.0f && !(Ic >= Ia)) {
+      const float hi = vmax(Ia, Ic), lo = vmin(Ia, Ic);
+      const float margin = 64.0f * eps * (hi + 2.0f * 0.0f);
+      q.x_hi = log_ratio(vmin(hi + 0.125f * 0.0f, inf_<float>()) + margin);
+      const float lo_b = vmax(lo - 0.125f * 0.0f, dark) - margin;
+      if (lo_b > 0.0f) q.x_lo = log_ratio(lo_b);
+    }
+    return q;
+  }
+
+  // The whole-segment search from t_a on the prepared values: 0 no crossing in [t_a, t_b), 1 a
+  // crossing at t_a itself (the hit is filled), 2 undecided (the exact search must run).
+  BSX_HD BSX_INLINE int fast_search(const Prep& q, Hit& h) const {
+    constexpr float eps = FloatEps<float>::value;
+    h.x = q.x_a;
+    if (fires(ta, h)) return 1;
+    if (tb - ta == 1) return 0;
+    if (tb - ta == 2) return 2;  // the root is a leaf: the exact search tests it
+    const int64_t c = tb - 1;
+    if (leak == 0.0f) {
+      const float D = delta_at(c, q.x_c);
+      return D >= theta_on || D <= -theta_off ? 2 : 0;
+    }
+    if (q.x_hi != q.x_hi) return delta_at(c, q.x_c) >= theta_on ? 2 : 0;  // rising
+    const float dl_a = leak * ns_to_s64<float>(ta - t_ref) + dr_ref;
+    const float dl_c = leak * ns_to_s64<float>(c - t_ref) + dr_ref;
+    const float base = 0.0f - n3_ref - s3_ref;
+    const float x_hi = q.x_hi;
+    if (x_hi + base + dl_c + 16.0f * eps * (1.0f + fabs_(x_hi) + fabs_(s3_ref) + fabs_(dl_c)) >= theta_on) return 2;
+    if (q.x_lo != q.x_lo) return 2;
+    const float x_lo = q.x_lo;
+    return x_lo + base + dl_a - 16.0f * eps * (1.0f + fabs_(x_lo) + fabs_(s3_ref) + fabs_(dl_a)) <= -theta_off ? 2 : 0;
+  }
+
+  // One piece [t, e) of the current segment (the body of advance_instant's piece loop). With the
+  // segment's prepared values (q non-null), a search that starts at the segment's start and spans
+  // all of it is decided on them; anything else runs the exact search.
   template <class Sink>
-  BSX_HD BSX_INLINE void advance(int64_t t_end, Sink& sink) {
-    int seg = 0;  // the model time is sample 0's
-    int64_t t = t_state;
-    while (t < t_end) {
-      seg = segment_of(seg, t);
-      const int64_t e =
-          vmin(vmin(t_end, segment_end(seg)), ((t >> kPlainSegmentLog2) + 1) * kPlainSegmentNs);
-      ++n_substeps;
-      ++n_pieces;
-      while (true) {
-        if (blind_until != kNotBlind) {
-          if (blind_until >= e) break;
-          set_reference(blind_until, log_ratio(photocurrent(seg, blind_until)), 0.0f, -eps_ci);
-          blind_until = kNotBlind;
+  BSX_HD BSX_INLINE void piece(int64_t t, int64_t e, Sink& sink, const Prep* q = nullptr) {
+    ++n_substeps;
+    ++n_pieces;
+    while (true) {
+      if (blind_until != kNotBlind) {
+        if (blind_until >= e) break;
+        set_reference(blind_until, log_ratio(photocurrent(blind_until)), 0.0f, -eps_ci);
+        blind_until = kNotBlind;
+      }
+      const int64_t lo = vmax(t_search, t);
+      if (lo >= e) break;
+      if (q != nullptr && lo == ta && e == tb) {
+        Hit h{false, false, lo, 0.0f};
+        const int r = fast_search(*q, h);
+        if (r == 1) {
+          fire(h.t, h.on, h.x, 0.0f, sink);
+          continue;
         }
-        const int64_t lo = vmax(t_search, t);
-        if (lo >= e) break;
-        const Hit hit = search(seg, lo, e);
-        if (!hit.found) {
+        if (r == 0) {
+          if (tb - ta > 2) ++n_leaves;  // the root node the exact search would have counted
           t_search = vmax(t_search, e);
           break;
         }
-        fire(hit.t, hit.on, hit.x, 0.0f, sink);
       }
-      t = e;
+      const Hit hit = search(lo, e);
+      if (!hit.found) {
+        t_search = vmax(t_search, e);
+        break;
+      }
+      fire(hit.t, hit.on, hit.x, 0.0f, sink);
+    }
+  }
+
+  BSX_HD BSX_INLINE void load_sample(int i, int64_t& t, float* v) const {
+    t = ts[i];
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < G) v[g] = clampv(irr[static_cast<int64_t>(i) * G + g]);
+  }
+
+  // The call: the segments of the augmented samples in order, each cut into pieces at multiples of
+  // 2^23 ns; the next segment's end sample is fetched before the current one is worked on.
+  template <class Sink>
+  BSX_HD BSX_INLINE void advance(int64_t t_end, Sink& sink) {
+    int next = skip;  // the next input sample to become a segment end
+    ta = t_state;
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < G) va[g] = held[g];
+    if (skip) load_sample(0, ta, va);
+    int64_t nt = kNoBreakpoint;
+    float nv[kMaxG];
+    if (next < n) load_sample(next, nt, nv);
+    int64_t t = t_state;
+    Prep q{};
+    if (prep != nullptr && next < n) q = prep[next];
+    while (t < t_end) {
+      tb = nt;
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < G) vb[g] = nv[g];
+      const Prep cur = q;
+      const bool has_prep = prep != nullptr && tb != kNoBreakpoint;
+      ++next;
+      if (next < n) {  // prefetch the next segment's end sample and prepared values
+        load_sample(next, nt, nv);
+        if (prep != nullptr) q = prep[next];
+      } else {
+        nt = kNoBreakpoint;
+      }
+      const int64_t stop = vmin(t_end, tb);
+      while (t < stop) {
+        const int64_t e = vmin(stop, ((t >> kPlainSegmentLog2) + 1) * kPlainSegmentNs);
+        piece(t, e, sink, has_prep ? &cur : nullptr);
+        t = e;
+      }
+      if (tb == kNoBreakpoint) break;
+      ta = tb;
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < G) va[g] = vb[g];
     }
     if (n > 0) {
       for (int g = 0; g < kMaxG; ++g)
-        if (g < G) held[g] = sample_value(n_aug() - 1, g);
+        if (g < G) held[g] = clampv(irr[static_cast<int64_t>(n - 1) * G + g]);
     }
   }
 };

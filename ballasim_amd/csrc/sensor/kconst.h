@@ -83,6 +83,16 @@ struct RunsDev {
   const float* irr;
 };
 
+// The reference-independent values of a whole-segment search of the instantaneous response,
+// prepared for every segment by a data-parallel pass (InstantLane::prep_segment):
+// x_a = ln(I(t_a) / i_scale), x_c = ln(I(t_b - 1) / i_scale) and, for a falling segment under leak,
+// the logarithms x_hi and x_lo of instant_may_fire's bounds (NaN x_hi: not the bounds' branch; NaN
+// x_lo: the lower bound is not positive, so a crossing cannot be excluded). Each is the value the
+// exact search computes itself, by the same operations, so deciding on it changes no bit.
+struct InstantPrep {
+  float x_a, x_c, x_hi, x_lo;
+};
+
 struct RawEventDev {
   uint32_t pixel;
   uint8_t polarity;

@@ -10,9 +10,19 @@ namespace bsx {
 
 hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, const float* irr0, int64_t t0,
                        unsigned long long* counters, hipStream_t stream);
+// Phase A: the prepared values of every segment of every run (pos: the exclusive scan of the run
+// lengths, total = pos[n_runs]); phase B: the per-pixel walk (prep null: exact searches only).
+hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
+                               const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
+                               int64_t t_state, InstantPrep* out, hipStream_t stream);
 hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
-                          const RunsDev& runs, int64_t t_state, int64_t t_end, RawEventDev* ev,
-                          unsigned long long cap, unsigned long long* counters, int block, hipStream_t stream);
+                          const RunsDev& runs, const int64_t* pos, const InstantPrep* prep, int64_t t_state,
+                          int64_t t_end, RawEventDev* ev, unsigned long long cap, unsigned long long* counters,
+                          int block, hipStream_t stream);
+// Exclusive scan of int32 counts into int64 offsets (scratch: scan_scratch_count(n) values; the
+// total lands in scratch[scan_scratch_count(n) - 1]). Defined with the scene kernels.
+size_t scan_scratch_count(size_t n);
+hipError_t launch_exclusive_scan(const int32_t* counts, size_t n, int64_t* out, int64_t* scratch, hipStream_t stream);
 hipError_t launch_scatter_rows(const uint32_t* pixel, const int64_t* row_start, const int32_t* row_count, size_t n,
                                size_t n_samples, int32_t* row_of, uint32_t n_pixels, unsigned int* flags,
                                hipStream_t stream);

@@ -62,10 +62,12 @@ __global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateD
       const float v = clamp_irr(raw, kc.irr_max);
       if (!(v == raw)) ++clamps;
       L.held[g] = v;
+      L.va[g] = v;
       mean_I = mean_I + L.scale * v;
       st.held[static_cast<int64_t>(g) * kc.n_pixels + p] = v;
     }
-    L.scan_input();
+    L.ta = t0;
+    L.tb = kNoBreakpoint;  // the held segment: I = dark + s x held
     L.i_scale = mean_I;
     st.i_scale[p] = mean_I;
     st.z[p] = 1.0f / mean_I;
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateD
     st.s2[p] = 0.0f;
     st.s3[p] = 0.0f;
     st.tv[p] = 0.0f;
-    const float s3_ref = kc.instant ? L.log_ratio(L.photocurrent(0, t0)) : 0.0f;
+    const float s3_ref = kc.instant ? L.log_ratio(L.photocurrent(t0)) : 0.0f;
     st.t_ref[p] = t0;
     st.s3_ref[p] = s3_ref;
     st.n3_ref[p] = 0.0f;
@@ -87,10 +89,57 @@ __global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateD
   wave_add(counters + 1 + kStClamps, clamps);
 }
 
+// Phase A of the instantaneous response: one thread per segment of every run (position q of the
+// runs laid end to end, pos the exclusive scan of the run lengths), all runs at once, so the
+// transcendental work of the searches is spread evenly over the device whatever the runs' lengths.
+template <int kMaxG>
+__global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp, StateDev in, RunsDev runs,
+                                                           const uint32_t* pixel, const int64_t* pos, int64_t n_runs,
+                                                           int64_t total, int64_t t_state, InstantPrep* out) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  int64_t lo = 0, hi = n_runs;  // the run i with pos[i] <= q < pos[i + 1]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (pos[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  const int64_t i = lo, j = q - pos[i];
+  const uint32_t p = pixel[i];
+  const int64_t s = runs.row_start[i];
+  InstantLane<kMaxG> L;
+  L.G = kc.groups;
+  L.irr_max = kc.irr_max;
+  L.dark = mp.dark[p];
+  L.scale = mp.scale[p];
+  L.leak = mp.leak[p];
+  L.i_scale = in.i_scale[p];
+  const int64_t* t = runs.t + s;
+  const float* v = runs.irr + s * kc.groups;
+  if (j == 0) {
+    if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
+      out[q] = InstantPrep{0.0f, 0.0f, 0.0f, 0.0f};
+      return;
+    }
+    L.ta = t_state;
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
+  } else {
+    L.ta = t[j - 1];
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) L.va[g] = L.clampv(v[(j - 1) * kc.groups + g]);
+  }
+  L.tb = t[j];
+  for (int g = 0; g < kMaxG; ++g)
+    if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
+  out[q] = L.prep_segment();
+}
+
 template <int kMaxG>
 __global__ __launch_bounds__(256) void instant_kernel(KConst kc, MapsDev mp, StateDev in, StateDev out, RunsDev runs,
-                                                      int64_t t_state, int64_t t_end, RawEventDev* ev,
-                                                      unsigned long long cap, unsigned long long* counters) {
+                                                      const int64_t* pos, const InstantPrep* prep, int64_t t_state,
+                                                      int64_t t_end, RawEventDev* ev, unsigned long long cap,
+                                                      unsigned long long* counters) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   InstantLane<kMaxG> L;
   L.n_substeps = L.n_pieces = L.n_leaves = 0;
@@ -109,11 +158,13 @@ __global__ __launch_bounds__(256) void instant_kernel(KConst kc, MapsDev mp, Sta
     L.G = kc.groups;
     L.ideal_reset = kc.ideal_reset != 0;
     const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
+    L.prep = nullptr;
     if (row >= 0) {
       const int64_t start = runs.row_start[row];
       L.ts = runs.t + start;
       L.irr = runs.irr + start * kc.groups;
       L.n = runs.row_count[row];
+      if (prep != nullptr) L.prep = prep + pos[row];
     } else {
       L.ts = nullptr;
       L.irr = nullptr;
@@ -131,7 +182,6 @@ __global__ __launch_bounds__(256) void instant_kernel(KConst kc, MapsDev mp, Sta
     L.blind_until = in.blind_until[p];
     L.t_search = in.t_search[p];
     L.flags = in.flags[p];
-    L.scan_input();
     L.advance(t_end, sink);
     for (int g = 0; g < kMaxG; ++g)
       if (g < kc.groups) out.held[static_cast<int64_t>(g) * kc.n_pixels + p] = L.held[g];
@@ -217,18 +267,32 @@ hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, 
   return hipGetLastError();
 }
 
+hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
+                               const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
+                               int64_t t_state, InstantPrep* out, hipStream_t stream) {
+  if (total <= 0 || n_runs <= 0) return hipSuccess;
+  const dim3 grid = grid_for(static_cast<size_t>(total), 256);
+#define BSX_PREP(G) instant_prep_kernel<G><<<grid, 256, 0, stream>>>(kc, mp, in, runs, pixel, pos, n_runs, total, t_state, out)
+  if (kc.groups <= 1) BSX_PREP(1);
+  else if (kc.groups <= 2) BSX_PREP(2);
+  else if (kc.groups <= 4) BSX_PREP(4);
+  else BSX_PREP(8);
+#undef BSX_PREP
+  return hipGetLastError();
+}
+
 hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
-                          const RunsDev& runs, int64_t t_state, int64_t t_end, RawEventDev* ev,
-                          unsigned long long cap, unsigned long long* counters, int block, hipStream_t stream) {
+                          const RunsDev& runs, const int64_t* pos, const InstantPrep* prep, int64_t t_state,
+                          int64_t t_end, RawEventDev* ev, unsigned long long cap, unsigned long long* counters,
+                          int block, hipStream_t stream) {
   const dim3 grid = grid_for(kc.n_pixels, block);
-  if (kc.groups <= 1)
-    instant_kernel<1><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters);
-  else if (kc.groups <= 2)
-    instant_kernel<2><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters);
-  else if (kc.groups <= 4)
-    instant_kernel<4><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters);
-  else
-    instant_kernel<8><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters);
+#define BSX_ADV(G) \
+  instant_kernel<G><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, pos, prep, t_state, t_end, ev, cap, counters)
+  if (kc.groups <= 1) BSX_ADV(1);
+  else if (kc.groups <= 2) BSX_ADV(2);
+  else if (kc.groups <= 4) BSX_ADV(4);
+  else BSX_ADV(8);
+#undef BSX_ADV
   return hipGetLastError();
 }
 

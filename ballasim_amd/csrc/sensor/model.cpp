@@ -322,7 +322,9 @@ void SensorModel::advance_runs(const uint32_t* ids, size_t n, const int64_t* t, 
     hip_check(hipMemcpy(d_row_count_.get(), row_count, n * 4, hipMemcpyHostToDevice), "upload row_count");
   }
   hip_check(hipMemcpy(d_row_of_.get(), row_of.data(), P_ * 4, hipMemcpyHostToDevice), "upload row_of");
-  run_advance(runs_view(), t_end_ns, nullptr);
+  d_ids_.ensure(std::max<size_t>(n, 1) * 4);
+  if (n > 0) hip_check(hipMemcpy(d_ids_.get(), ids, n * 4, hipMemcpyHostToDevice), "upload ids");
+  run_advance(runs_view(), d_ids_.as<const uint32_t>(), n, t_end_ns, nullptr);
   t_state_ = t_end_ns;
   initialized_ = true;
   unsigned long long count = 0;
@@ -345,7 +347,8 @@ RunsDev SensorModel::runs_view() const {
   return r;
 }
 
-void SensorModel::run_advance(const RunsDev& runs, int64_t t_end, hipStream_t stream) {
+void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t t_end,
+                               hipStream_t stream) {
   if (params_.photoreceptor != BSX_PHOTORECEPTOR_INSTANTANEOUS) {
     throw std::logic_error("SensorModel: the cascade photoreceptor is not built into this backend yet");
   }
@@ -353,11 +356,32 @@ void SensorModel::run_advance(const RunsDev& runs, int64_t t_end, hipStream_t st
   d_events_.ensure(cap_ * sizeof(RawEventDev));
   const KConst kc = kconst();
   const MapsDev mp = maps_view(d_maps_, P_);
+  // The runs' segments laid end to end: pos = exclusive scan of the run lengths.
+  int64_t total = 0;
+  d_pos_.ensure(std::max<size_t>(n_runs, 1) * 8);
+  if (n_runs > 0) {
+    d_scan_.ensure(scan_scratch_count(n_runs) * 8);
+    hip_check(launch_exclusive_scan(runs.row_count, n_runs, d_pos_.as<int64_t>(), d_scan_.as<int64_t>(), stream),
+              "scan runs");
+    hip_check(hipMemcpyAsync(&total, d_scan_.as<int64_t>() + scan_scratch_count(n_runs) - 1, 8,
+                             hipMemcpyDeviceToHost, stream),
+              "read total");
+    hip_check(hipStreamSynchronize(stream), "scan runs");
+  }
+  d_prep_.ensure(std::max<int64_t>(total, 1) * sizeof(InstantPrep));
   unsigned long long c[1 + kStCount];
+  bool prepared = false;
   while (true) {
     hip_check(launch_fill_u64(d_counters_.as<unsigned long long>(), 1 + kStCount, 0, stream), "fill counters");
     hip_check(hipEventRecord(ev0_, stream), "event record");
-    hip_check(launch_instant(kc, mp, state_view(cur_), state_view(1 - cur_), runs, t_state_, t_end,
+    if (!prepared) {  // phase A reads only the call's input state: a re-run reuses it
+      hip_check(launch_instant_prep(kc, mp, state_view(cur_), runs, pixel, d_pos_.as<const int64_t>(),
+                                    static_cast<int64_t>(n_runs), total, t_state_, d_prep_.as<InstantPrep>(), stream),
+                "instant prep kernel");
+      prepared = true;
+    }
+    hip_check(launch_instant(kc, mp, state_view(cur_), state_view(1 - cur_), runs, d_pos_.as<const int64_t>(),
+                             total > 0 ? d_prep_.as<const InstantPrep>() : nullptr, t_state_, t_end,
                              d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(), block_, stream),
               "instant kernel");
     hip_check(hipEventRecord(ev1_, stream), "event record");
@@ -417,7 +441,7 @@ bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64
   r.t = runs.t;
   r.irr = runs.irr;
   initialized_ = false;
-  run_advance(r, t_end_ns, stream);
+  run_advance(r, runs.pixel, runs.n, t_end_ns, stream);
   t_state_ = t_end_ns;
   initialized_ = true;
   unsigned long long count = 0;

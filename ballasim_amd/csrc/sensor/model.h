@@ -106,7 +106,8 @@ class SensorModel {
   StateDev state_view(int which) const;
   RunsDev runs_view() const;
   // The launch loop of one call on device runs (row_of built), with overflow re-runs.
-  void run_advance(const RunsDev& runs, int64_t t_end, hipStream_t stream);
+  // pixel: the runs' pixel ids in device memory (run i is pixel[i]), n_runs of them.
+  void run_advance(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t t_end, hipStream_t stream);
   void finish_host_events(const RawEventDev* raw, size_t n);
   KConst kconst() const;
 
@@ -130,7 +131,8 @@ class SensorModel {
   DeviceBuffer d_counters_;   // 1 + kStCount uint64
   DeviceBuffer d_flags_;      // scatter flags + check counters
   DeviceBuffer d_row_of_;
-  DeviceBuffer d_t_, d_irr_, d_row_start_, d_row_count_, d_frame_;
+  DeviceBuffer d_t_, d_irr_, d_row_start_, d_row_count_, d_frame_, d_ids_;
+  DeviceBuffer d_pos_, d_scan_, d_prep_;  // the runs' segment positions and the prepared segment values
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 
   int block_ = 256;

@@ -209,6 +209,14 @@ class SensorModel:
                                                        C.byref(out)))
         return out.data, out.count
 
+    def advance_device_runs(self, runs, t_end_ns, stream=0):
+        """advance_device on a bsx_device_runs struct (e.g. Scene.sample_window's); (data_ptr, count)."""
+        out = _abi.DeviceEvents()
+        st = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
+        _abi.check(self._lib.bsx_sensor_advance_device(self._h, C.byref(runs), int(t_end_ns), C.c_void_p(st),
+                                                       C.byref(out)))
+        return out.data, out.count
+
     def download_events(self, data_ptr, count):
         ev = _abi.DeviceEvents(data_ptr, count)
         _abi.check(self._lib.bsx_sensor_download_events(self._h, C.byref(ev)))

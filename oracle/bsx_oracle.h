@@ -90,6 +90,17 @@ float orc_resid_expf(int j, float y);
 float orc_erfcxf(float x);
 float orc_dawsonf(float x);
 float orc_z_step(float z0, float I0, float I1, float h, float k);
+/* The comparator input g(u) = c0 + c1 u + c2 u^2 + c3 u^3 + C0 R_j(a0 u) + C1 R_j(a1 u) (j = order) and
+ * its first firing: g at u, its critical points on (lo, hi) (returns their count, at most 6, into out),
+ * and the first integer ns in [max(lo, e0), e1) where g - th_on >= 0 or -g - th_off >= 0 (1 with *on and
+ * *t, or 0). */
+float orc_xc_eval(const float c[4], const float C[2], const float a[2], int order, float u);
+void orc_xc_eval_n(const float c[4], const float C[2], const float a[2], int order, const float* u, size_t n,
+                   float* out);
+int orc_xc_critical(const float c[4], const float C[2], const float a[2], int order, float lo, float hi,
+                    float* out);
+int orc_xc_search(const float c[4], const float C[2], const float a[2], int order, int64_t e0, int64_t e1,
+                  int64_t lo, float th_on, float th_off, int* on, int64_t* t);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,11 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
 
 
+STAT_NAMES_ORACLE = ("substeps", "substep_attempts", "settled_substeps", "leaf_intervals", "fallbacks",
+                     "input_clamps", "quarantines", "pieces")
+STAT_NAMES_REF = ("substeps", "substep_attempts", "settled_substeps", "cells", "refinements", "leaf_intervals",
+                  "fallbacks", "input_clamps", "quarantines", "pieces")
+
 _oracle = None
 _ref = None
 
@@ -82,6 +87,12 @@ def oracle_lib():
         lib.orc_sincosf.argtypes = [C.c_float, P, P]
         lib.orc_z_step.restype = C.c_float
         lib.orc_z_step.argtypes = [C.c_float] * 5
+        lib.orc_xc_eval.restype = C.c_float
+        lib.orc_xc_eval.argtypes = [P, P, P, C.c_int, C.c_float]
+        lib.orc_xc_eval_n.argtypes = [P, P, P, C.c_int, P, C.c_size_t, P]
+        lib.orc_xc_critical.argtypes = [P, P, P, C.c_int, C.c_float, C.c_float, P]
+        lib.orc_xc_search.argtypes = [P, P, P, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_float,
+                                      P, P]
         _oracle = lib
     return _oracle
 
@@ -189,6 +200,11 @@ class OracleModel(_Base):
         self.lib.orc_states(self.h, _ptr(out))
         return out
 
+    def stats(self):
+        v = np.zeros(8, np.uint64)
+        self.lib.orc_get_stats(self.h, _ptr(v))
+        return dict(zip(STAT_NAMES_ORACLE, (int(x) for x in v)))
+
 
 class RefModel(_Base):
     def __init__(self, params, width, height, seed, groups=1, camera=0, num_threads=0, fp32=True):
@@ -236,3 +252,8 @@ class RefModel(_Base):
         out = np.zeros(self.width * self.height, REF_STATE_DTYPE)
         self.lib.ref_states(self.h, _ptr(out))
         return out
+
+    def stats(self):
+        v = np.zeros(10, np.uint64)
+        self.lib.ref_stats(self.h, _ptr(v))
+        return dict(zip(STAT_NAMES_REF, (int(x) for x in v)))

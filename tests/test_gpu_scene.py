@@ -1,0 +1,150 @@
+"""The device scene and sample-time controller (GPU): runs are well formed, the refinement obeys the
+controller's rules on the values it rendered (re-derived on the CPU from the run itself), the scene
+values agree with a numpy evaluation of the same shapes, and the sensor on these runs equals the
+oracle on their host copy bit for bit."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import events_from_dict
+
+pytestmark = pytest.mark.gpu
+
+WINDOW = 1 << 23
+
+
+def _small_scene(seed=3, n=6, width=96, height=64):
+    from ballasim_amd import scene as S
+    shapes, bg = S.config2_shapes(seed, n=n, width=width, height=height, v_max=1500.0, w_max=2.0)
+    return S.Scene(shapes, bg), S.view(width, height), shapes, bg
+
+
+def _eta(dark, a, m, b):
+    a, m, b = np.float32(a), np.float32(m), np.float32(b)
+    dev = np.float32(abs((a + b) * np.float32(-0.5) + m))
+    low = min(min(a, m), b)
+    return np.float32(np.float32(dev) / np.float32(np.float32(dark) + low))
+
+
+def _expected_offsets(vals, level, dark, tol, max_levels=4, min_split=2048):
+    """The controller's sample offsets for one pixel, re-derived from the rendered values alone
+    (signature flags off): vals maps offset -> value (the run plus the carried sample at 0)."""
+    n, step = 1 << level, WINDOW >> level
+    e = [vals.get(j * step) for j in range(n + 1)]
+    if any(v is None for v in e):
+        return None
+    out = []
+
+    def refine(lo, hi, depth):
+        g = (lo + hi) >> 1
+        if g not in vals:
+            raise KeyError(g)
+        bad = _eta(dark, vals[lo], vals[g], vals[hi]) > np.float32(tol)
+        if bad and g - lo >= min_split and depth < max_levels:
+            refine(lo, g, depth + 1)
+        out.append(g)
+        if bad and hi - g >= min_split and depth < max_levels:
+            refine(g, hi, depth + 1)
+
+    for j in range(1, n + 1):
+        m = (j + 1 <= n and _eta(dark, e[j - 1], e[j], e[j + 1]) > np.float32(tol)) or \
+            (j >= 2 and _eta(dark, e[j - 2], e[j - 1], e[j]) > np.float32(tol))
+        if m and step >= min_split:
+            try:
+                refine((j - 1) * step, j * step, 1)
+            except KeyError:
+                return None
+        out.append(j * step)
+    return out
+
+
+def test_runs_well_formed_and_rules_hold():
+    """Every run is sorted inside (t_w, t_w + 2^23], on the 1024 ns grid, and is exactly the sample
+    set the controller's rules give on its own values at some lattice level (signature flags off)."""
+    from ballasim_amd import scene as S
+    sc, view, _, _ = _small_scene()
+    dark, tol = 0.0021, 0.01
+    rules = S.rules(dark, tol=tol, signature=False)
+    checked = refined = 0
+    for w in range(3):
+        runs, total = sc.sample_window(view, rules, w * WINDOW, slot=0)
+        ids, t, irr, offs = S.runs_to_host(runs)
+        carried = sc.render(view, w * WINDOW)
+        assert np.array_equal(ids, np.arange(view.width * view.height))
+        assert total == len(t) == offs[-1]
+        for p in range(len(ids)):
+            tt = t[offs[p]:offs[p + 1]] - w * WINDOW
+            if len(tt) == 0:
+                continue
+            assert np.all(np.diff(tt) > 0) and tt[0] > 0 and tt[-1] == WINDOW and np.all(tt % 1024 == 0)
+            vals = {0: carried[p]}
+            vals.update({int(d): v for d, v in zip(tt, irr[offs[p]:offs[p + 1], 0])})
+            ok = False
+            for level in range(1, 14):
+                exp = _expected_offsets(vals, level, dark, tol)
+                if exp is not None and exp == [int(x) for x in tt]:
+                    ok = True
+                    refined += len(tt) > (1 << level)
+                    break
+            assert ok, f"window {w} pixel {p}: the run is not the controller's sample set"
+            checked += 1
+    assert checked > 100 and refined > 10
+
+
+def test_scene_values_match_numpy_evaluation():
+    from ballasim_amd import scene as S
+    sc, view, shapes, bg = _small_scene(seed=5)
+    for t_ns in (0, 3 * WINDOW + 12345):
+        img = sc.render(view, t_ns).reshape(view.height, view.width)
+        t = t_ns * 1e-9
+        ys, xs = np.mgrid[0:view.height, 0:view.width]
+        x, y = xs + 0.5, ys + 0.5
+        owner = np.full(x.shape, -1)
+        for i, s in enumerate(shapes):
+            dx = x - (s.cx + s.vx * t)
+            dy = y - (s.cy + s.vy * t)
+            th = s.angle + s.omega * t
+            u = np.cos(th) * dx + np.sin(th) * dy
+            v = -np.sin(th) * dx + np.cos(th) * dy
+            inside = (np.abs(u) <= s.hx) & (np.abs(v) <= s.hy)
+            owner[inside] = i
+        bg_mask = owner < 0
+        assert np.allclose(img[bg_mask], bg)
+        for i, s in enumerate(shapes):
+            vals = img[owner == i]
+            if len(vals):
+                assert np.all(vals >= s.lux * (1 - s.contrast) * 0.999) and np.all(vals <= s.lux * (1 + s.contrast) * 1.001)
+
+
+def test_sensor_on_scene_runs_matches_oracle():
+    import ballasim_amd as bsx
+    from ballasim_amd import scene as S
+    sc, view, _, _ = _small_scene(seed=7, n=8, width=128, height=80)
+    p = bsx.preset("prophesee_gen3")
+    p.photoreceptor = "instantaneous"
+    p.noise = False
+    p.theta_on = p.theta_off = 0.15
+    p.refractory_s = 50e-6
+    p.leak_rate_hz = 0.05
+    p.illuminance_scale = S.LUX_TO_FA
+    W, H = view.width, view.height
+    gpu = bsx.SensorModel(p, W, H, 9, device="cuda")
+    ref = O.OracleModel(p.c, W, H, 9, 1, 0)
+    frame = sc.render(view, 0).reshape(-1, 1)
+    gpu.init(0, frame)
+    ref.init(0, frame)
+    rules = S.rules(float(p.dark_current_fA) / S.LUX_TO_FA, tol=0.01)
+    total_events = 0
+    for w in range(4):
+        runs, _ = sc.sample_window(view, rules, w * WINDOW, slot=w % 2)
+        ptr, count = gpu.advance_device_runs(runs, (w + 1) * WINDOW)
+        ev = gpu.download_events(ptr, count)
+        ids, t, irr, offs = S.runs_to_host(runs)
+        want, rep = ref.advance_runs(ids, t, irr, offs, (w + 1) * WINDOW)
+        got = events_from_dict(ev, W)
+        assert np.array_equal(got, want), f"window {w}"
+        assert np.array_equal(ev["t_reported_ns"], rep)
+        total_events += len(want)
+    assert total_events > 100
