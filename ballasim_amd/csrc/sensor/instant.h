@@ -118,33 +118,33 @@ struct InstantLane {
   }
 
   // First integer t in [lo, hi) of the current segment where Delta fires: time-ordered depth-first
-  // bisection of (lo, hi - 1]. The pending right siblings are kept as their right ends only: in time
-  // order a popped node always starts where the last finished one ended.
-  BSX_HD BSX_INLINE Hit search(int64_t lo, int64_t hi) {
+  // bisection of (lo, hi - 1]. The pending right siblings are kept as their right ends only (offsets
+  // from lo: a piece spans less than 2^23 ns), in the lane's column of a stack with `stride` entries
+  // per level (kSearchDepth levels): in time order a popped node always starts where the last
+  // finished one ended, and its I at the right end is recomputed, as photocurrent is a pure function.
+  static constexpr int kSearchDepth = 24;
+
+  BSX_HD BSX_INLINE Hit search(int64_t lo, int64_t hi, uint32_t* stack, int stride) {
     const float I_lo = photocurrent(lo);
     Hit out{false, false, lo, log_ratio(I_lo)};
     if (fires(lo, out)) return out;
     if (hi - lo == 1) return out;
-    int64_t a = lo, c = hi - 1;
+    uint32_t a = 0, c = static_cast<uint32_t>(hi - 1 - lo);
     float Ia = I_lo, Ic = photocurrent(hi - 1);
-    int64_t pend_c[24];  // a piece spans < 2^23 ns: at most 23 pending right siblings
-    float pend_Ic[24];
     int sp = 0;
     while (true) {
       if (c - a == 1) {
-        out.t = c;
+        out.t = lo + c;
         out.x = log_ratio(Ic);
-        if (fires(c, out)) return out;
+        if (fires(lo + c, out)) return out;
       } else {
         ++n_leaves;
-        if (may_fire(a, c, Ia, Ic)) {
-          const int64_t m = a + (c - a) / 2;
-          const float Im = photocurrent(m);
-          pend_c[sp] = c;
-          pend_Ic[sp] = Ic;
+        if (may_fire(lo + a, lo + c, Ia, Ic)) {
+          const uint32_t m = a + (c - a) / 2;
+          stack[sp * stride] = c;
           ++sp;
           c = m;
-          Ic = Im;
+          Ic = photocurrent(lo + m);
           continue;
         }
       }
@@ -152,8 +152,8 @@ struct InstantLane {
       --sp;
       a = c;
       Ia = Ic;
-      c = pend_c[sp];
-      Ic = pend_Ic[sp];
+      c = stack[sp * stride];
+      Ic = photocurrent(lo + c);
     }
     out.found = false;
     return out;
@@ -257,21 +257,20 @@ struct InstantLane {
     return x_lo + base + dl_a - 16.0f * eps * (1.0f + fabs_(x_lo) + fabs_(s3_ref) + fabs_(dl_a)) <= -theta_off ? 2 : 0;
   }
 
-  // One piece [t, e) of the current segment (the body of advance_instant's piece loop). With the
-  // segment's prepared values (q non-null), a search that starts at the segment's start and spans
-  // all of it is decided on them; anything else runs the exact search.
+  // The fast part of one piece [t, e) of the current segment (the body of advance_instant's piece
+  // loop up to its exact search). With the segment's prepared values (q non-null), a search that
+  // starts at the segment's start and spans all of it is decided on them. Returns true when the
+  // piece is finished, false when the exact search of [s_lo, e) must run next (search_step).
   template <class Sink>
-  BSX_HD BSX_INLINE void piece(int64_t t, int64_t e, Sink& sink, const Prep* q = nullptr) {
-    ++n_substeps;
-    ++n_pieces;
+  BSX_HD BSX_INLINE bool piece_fast(int64_t t, int64_t e, Sink& sink, const Prep* q) {
     while (true) {
       if (blind_until != kNotBlind) {
-        if (blind_until >= e) break;
+        if (blind_until >= e) return true;
         set_reference(blind_until, log_ratio(photocurrent(blind_until)), 0.0f, -eps_ci);
         blind_until = kNotBlind;
       }
       const int64_t lo = vmax(t_search, t);
-      if (lo >= e) break;
+      if (lo >= e) return true;
       if (q != nullptr && lo == ta && e == tb) {
         Hit h{false, false, lo, 0.0f};
         const int r = fast_search(*q, h);
@@ -282,15 +281,11 @@ struct InstantLane {
         if (r == 0) {
           if (tb - ta > 2) ++n_leaves;  // the root node the exact search would have counted
           t_search = vmax(t_search, e);
-          break;
+          return true;
         }
       }
-      const Hit hit = search(lo, e);
-      if (!hit.found) {
-        t_search = vmax(t_search, e);
-        break;
-      }
-      fire(hit.t, hit.on, hit.x, 0.0f, sink);
+      s_lo = lo;
+      return false;
     }
   }
 
@@ -300,49 +295,110 @@ struct InstantLane {
       if (g < G) v[g] = clampv(irr[static_cast<int64_t>(i) * G + g]);
   }
 
-  // The call: the segments of the augmented samples in order, each cut into pieces at multiples of
-  // 2^23 ns; the next segment's end sample is fetched before the current one is worked on.
-  template <class Sink>
-  BSX_HD BSX_INLINE void advance(int64_t t_end, Sink& sink) {
-    int next = skip;  // the next input sample to become a segment end
+  // ---- The call as a resumable lane program --------------------------------------------------
+  // advance_instant walks the segments of the augmented samples in order, cutting each into pieces at
+  // multiples of 2^23 ns. Here that walk is split at its exact searches: walk() runs until the next
+  // exact search is due (kSearch), the call is finished (kDone) or `budget` segments are done
+  // (kWalk); search_step() runs the due search and its event. Run in that alternation, each lane
+  // performs exactly advance_instant's operations in its order, so a scheduler may interleave the
+  // lanes of a wave freely (the kernel gathers the searches of many lanes into one pass).
+  static constexpr int kWalk = 0, kSearch = 1, kDone = 2;
+
+  int next_i;          // the next input sample to become a segment end
+  int64_t nt;          // the prefetched segment end (kNoBreakpoint: none)
+  float nv[kMaxG];
+  Prep q_next, q_cur;  // the prefetched and the current segment's prepared values
+  bool has_prep, seg_open, in_piece;
+  int64_t t_cur, stop, t_end_, s_lo, e_cur;
+
+  BSX_HD BSX_INLINE void begin(int64_t t_end) {
+    t_end_ = t_end;
+    next_i = skip;
     ta = t_state;
     for (int g = 0; g < kMaxG; ++g)
       if (g < G) va[g] = held[g];
     if (skip) load_sample(0, ta, va);
-    int64_t nt = kNoBreakpoint;
-    float nv[kMaxG];
-    if (next < n) load_sample(next, nt, nv);
-    int64_t t = t_state;
-    Prep q{};
-    if (prep != nullptr && next < n) q = prep[next];
-    while (t < t_end) {
-      tb = nt;
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < G) vb[g] = nv[g];
-      const Prep cur = q;
-      const bool has_prep = prep != nullptr && tb != kNoBreakpoint;
-      ++next;
-      if (next < n) {  // prefetch the next segment's end sample and prepared values
-        load_sample(next, nt, nv);
-        if (prep != nullptr) q = prep[next];
-      } else {
-        nt = kNoBreakpoint;
+    nt = kNoBreakpoint;
+    if (next_i < n) load_sample(next_i, nt, nv);
+    t_cur = t_state;
+    q_next = Prep{};
+    if (prep != nullptr && next_i < n) q_next = prep[next_i];
+    seg_open = false;
+    in_piece = false;
+    tb = kNoBreakpoint;
+  }
+
+  template <class Sink>
+  BSX_HD BSX_INLINE int walk(Sink& sink, int budget) {
+    while (true) {
+      if (!seg_open) {
+        if (!(t_cur < t_end_)) return kDone;
+        tb = nt;
+        for (int g = 0; g < kMaxG; ++g)
+          if (g < G) vb[g] = nv[g];
+        q_cur = q_next;
+        has_prep = prep != nullptr && tb != kNoBreakpoint;
+        ++next_i;
+        if (next_i < n) {  // prefetch the next segment's end sample and prepared values
+          load_sample(next_i, nt, nv);
+          if (prep != nullptr) q_next = prep[next_i];
+        } else {
+          nt = kNoBreakpoint;
+        }
+        stop = vmin(t_end_, tb);
+        seg_open = true;
       }
-      const int64_t stop = vmin(t_end, tb);
-      while (t < stop) {
-        const int64_t e = vmin(stop, ((t >> kPlainSegmentLog2) + 1) * kPlainSegmentNs);
-        piece(t, e, sink, has_prep ? &cur : nullptr);
-        t = e;
+      if (t_cur < stop) {
+        e_cur = vmin(stop, ((t_cur >> kPlainSegmentLog2) + 1) * kPlainSegmentNs);
+        if (!in_piece) {
+          ++n_substeps;
+          ++n_pieces;
+          in_piece = true;
+        }
+        if (!piece_fast(t_cur, e_cur, sink, has_prep ? &q_cur : nullptr)) return kSearch;
+        t_cur = e_cur;
+        in_piece = false;
+        continue;
       }
-      if (tb == kNoBreakpoint) break;
+      if (tb == kNoBreakpoint) return kDone;
       ta = tb;
       for (int g = 0; g < kMaxG; ++g)
         if (g < G) va[g] = vb[g];
+      seg_open = false;
+      if (--budget <= 0) return kWalk;
     }
+  }
+
+  // The exact search of [s_lo, e_cur) that walk() stopped at; its event (if any) keeps the piece
+  // open, as the piece loop continues after a crossing.
+  template <class Sink>
+  BSX_HD BSX_INLINE void search_step(Sink& sink, uint32_t* stack, int stride) {
+    const Hit hit = search(s_lo, e_cur, stack, stride);
+    if (!hit.found) {
+      t_search = vmax(t_search, e_cur);
+      t_cur = e_cur;
+      in_piece = false;
+      return;
+    }
+    fire(hit.t, hit.on, hit.x, 0.0f, sink);
+  }
+
+  // After the call: the last input sample is held until the next call.
+  BSX_HD BSX_INLINE void end() {
     if (n > 0) {
       for (int g = 0; g < kMaxG; ++g)
         if (g < G) held[g] = clampv(irr[static_cast<int64_t>(n - 1) * G + g]);
     }
+  }
+
+  // The whole call on one lane, in order.
+  template <class Sink>
+  BSX_HD BSX_INLINE void advance(int64_t t_end, Sink& sink) {
+    uint32_t stack[kSearchDepth];
+    begin(t_end);
+    for (int m = walk(sink, 1 << 30); m != kDone; m = walk(sink, 1 << 30))
+      if (m == kSearch) search_step(sink, stack, 1);
+    end();
   }
 };
 

@@ -135,11 +135,20 @@ __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp
   out[q] = L.prep_segment();
 }
 
+// Wave scheduling of the instantaneous walk: segments a walking lane does before the wave looks
+// again, and waiting lanes that start a joint search pass.
+constexpr int kWalkBudget = 8;
+constexpr int kSearchBatch = 8;
+constexpr int kInstantBlock = 256;
+
 template <int kMaxG>
-__global__ __launch_bounds__(256) void instant_kernel(KConst kc, MapsDev mp, StateDev in, StateDev out, RunsDev runs,
+__global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsDev mp, StateDev in, StateDev out, RunsDev runs,
                                                       const int64_t* pos, const InstantPrep* prep, int64_t t_state,
                                                       int64_t t_end, RawEventDev* ev, unsigned long long cap,
                                                       unsigned long long* counters) {
+  // The lanes' bisection stacks, one column per lane (conflict-free: a level is a row of the block;
+  // blockDim.x <= kInstantBlock by the launch bound).
+  __shared__ uint32_t stacks[InstantLane<kMaxG>::kSearchDepth * kInstantBlock];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   InstantLane<kMaxG> L;
   L.n_substeps = L.n_pieces = L.n_leaves = 0;
@@ -182,7 +191,27 @@ __global__ __launch_bounds__(256) void instant_kernel(KConst kc, MapsDev mp, Sta
     L.blind_until = in.blind_until[p];
     L.t_search = in.t_search[p];
     L.flags = in.flags[p];
-    L.advance(t_end, sink);
+    L.begin(t_end);
+  }
+  // The wave's lanes walk their pixels; a lane that reaches an exact search waits until enough lanes
+  // wait (or none walks), and then all waiting lanes search together: the rare deep bisections of
+  // different pixels run side by side instead of stalling the wave one at a time.
+  int mode = p < kc.n_pixels ? L.walk(sink, kWalkBudget) : InstantLane<kMaxG>::kDone;
+  while (true) {
+    const unsigned long long searching = __ballot(mode == InstantLane<kMaxG>::kSearch);
+    const unsigned long long walking = __ballot(mode == InstantLane<kMaxG>::kWalk);
+    if ((searching | walking) == 0) break;
+    if (searching != 0 && (walking == 0 || __popcll(searching) >= kSearchBatch)) {
+      if (mode == InstantLane<kMaxG>::kSearch) {
+        L.search_step(sink, stacks + threadIdx.x, blockDim.x);
+        mode = InstantLane<kMaxG>::kWalk;
+      }
+    } else if (mode == InstantLane<kMaxG>::kWalk) {
+      mode = L.walk(sink, kWalkBudget);
+    }
+  }
+  if (p < kc.n_pixels) {
+    L.end();
     for (int g = 0; g < kMaxG; ++g)
       if (g < kc.groups) out.held[static_cast<int64_t>(g) * kc.n_pixels + p] = L.held[g];
     out.z[p] = in.z[p];

@@ -596,8 +596,8 @@ void SensorModel::load_state(const uint8_t* data, size_t n) {
 void SensorModel::reset_stats() { stats_ = bsx_stats{}; }
 
 void SensorModel::set_policy(const bsx_launch_policy& p) {
-  if (p.block_size != 0 && (p.block_size % 64 != 0 || p.block_size < 64 || p.block_size > 1024)) {
-    throw std::invalid_argument("set_launch_policy: block_size must be a multiple of 64 in [64, 1024]");
+  if (p.block_size != 0 && (p.block_size % 64 != 0 || p.block_size < 64 || p.block_size > 256)) {
+    throw std::invalid_argument("set_launch_policy: block_size must be a multiple of 64 in [64, 256]");
   }
   if (p.event_capacity != 0) cap_ = p.event_capacity;
   if (p.block_size != 0) block_ = p.block_size;

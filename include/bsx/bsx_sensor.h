@@ -146,7 +146,7 @@ typedef struct bsx_stats {
  * capacity (0 = automatic), which the model grows on overflow. */
 typedef struct bsx_launch_policy {
   uint64_t event_capacity;
-  int32_t block_size; /* threads per workgroup, a multiple of 64 in [64, 1024]; 0 = automatic */
+  int32_t block_size; /* threads per workgroup, a multiple of 64 in [64, 256]; 0 = automatic (256) */
   int32_t reserved;
 } bsx_launch_policy;
 

@@ -31,6 +31,13 @@ CASES = {
                                               "initial_phase": 1, "readout_quantum_s": 1e-6,
                                               "readout_delay_s": 3e-6, "clock_offset_ns": 1234,
                                               "clock_drift_ppm": 2.5}},
+    "cascade_prophesee_physical": {"preset": "prophesee_gen3", "width": 16, "height": 12, "groups": 1, "seed": 21,
+                                   "camera": 0, "t0": 3 * WINDOW + 5, "calls": 3, "input_seed": 4,
+                                   "overrides": {"noise": 0}},
+    "cascade_samsung_ideal_leak": {"preset": "samsung_gen3", "width": 16, "height": 12, "groups": 2, "seed": 22,
+                                   "camera": 6, "t0": -2 * WINDOW, "calls": 3, "input_seed": 5,
+                                   "overrides": {"noise": 0, "reset_mode": 1, "refractory_s": 2e-5,
+                                                 "leak_rate_hz": 1.5}},
 }
 
 

@@ -74,6 +74,51 @@ def test_instant_random_sweep(trial):
         assert np.array_equal(sa[f], sb[f]), f
 
 
+def _random_cascade_params(rng, family):
+    p = preset(family)
+    p.noise = 0
+    p.reset_mode = int(rng.integers(0, 2))
+    p.refractory_s = float(rng.choice([0.0, 2e-5, 1e-4]))
+    p.leak_rate_hz = float(rng.choice([0.0, 0.1, 3.0]))
+    p.theta_on = float(rng.uniform(0.1, 0.35))
+    p.theta_off = float(rng.uniform(0.1, 0.35))
+    p.eps_ci = float(rng.uniform(-0.004, 0.004))
+    p.illuminance_scale = float(np.exp(rng.uniform(-1, 3)))
+    return p
+
+
+@pytest.mark.parametrize("trial", range(6))
+def test_cascade_random_sweep(trial):
+    """The cascade without noise, bit for bit: events, reported times, the full state and the work
+    counters (sub-steps, attempts, settled steps, leaf intervals, pieces)."""
+    rng = np.random.default_rng(3000 + trial)
+    p = _random_cascade_params(rng, ["prophesee_gen3", "samsung_gen3"][trial % 2])
+    W, H, G = 14, 9, int(rng.integers(1, 3))
+    seed = int(rng.integers(0, 2 ** 63))
+    a = O.OracleModel(p, W, H, seed, G, 4, 2)
+    b = O.RefModel(p, W, H, seed, G, 4, 2, True)
+    irr0 = np.exp(rng.normal(2, 1.5, (W * H, G))).astype(np.float32)
+    t0 = int(rng.integers(-10 ** 9, 10 ** 9))
+    a.init(t0, irr0)
+    b.init(t0, irr0)
+    t = t0
+    for _ in range(3):
+        t1 = t + int(rng.integers(1, 3 * 2 ** 23))
+        ids, ts, ir, offs = random_runs(rng, W * H, t, t1, G=G, log_sigma=2.0)
+        ea, ra = a.advance_runs(ids, ts, ir, offs, t1)
+        eb, rb = b.advance_runs(ids, ts, ir, offs, t1)
+        assert np.array_equal(ea, eb) and np.array_equal(ra, rb)
+        t = t1
+    sa, sb = a.states(), b.states()
+    for f in ("z", "x1", "s2", "s3", "tv", "i_scale", "s3_ref", "dr_ref", "t_ref", "t_search", "blind_until", "flags"):
+        assert np.array_equal(sa[f], sb[f]), f
+    ka, kb = a.stats(), b.stats()
+    for f in ("substeps", "substep_attempts", "settled_substeps", "leaf_intervals", "fallbacks", "quarantines",
+              "pieces"):
+        assert ka[f] == kb[f], f
+    assert ka["substeps"] > 0
+
+
 @pytest.mark.parametrize("name", ["prophesee_gen3", "samsung_gen3"])
 def test_maps_random_seeds(name):
     rng = np.random.default_rng(5)
