@@ -121,6 +121,7 @@ P = C.c_void_p
 SIGNATURES = {
     "bsx_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
     "bsx_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+    "bsx_math_selfcheck": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "bsx_params_preset": (C.c_int, [C.c_char_p, C.POINTER(Params)]),
     "bsx_preset_names": (C.c_int, [C.c_char_p, C.c_size_t]),
     "bsx_params_validate": (C.c_int, [C.POINTER(Params)]),

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../include/bsx/bsx_sensor.h"
+#include "sensor/kernels.h"
 #include "sensor/model.h"
 #include "sensor/params.h"
 
@@ -75,6 +76,24 @@ int bsx_device_count(int32_t* out) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
     *out = n;
+  });
+}
+
+int bsx_math_selfcheck(uint64_t* log_violations, uint64_t* log_hash) {
+  return guarded([&] {
+    need_ptr(log_violations, "log_violations");
+    need_ptr(log_hash, "log_hash");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) throw std::runtime_error("bsx_math_selfcheck: no HIP device");
+    unsigned long long* d = nullptr;
+    bsx::hip_check(hipMalloc(&d, 16), "bsx_math_selfcheck");
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = bsx::launch_math_selfcheck(d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(h, d, 16, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    bsx::hip_check(e, "bsx_math_selfcheck");
+    *log_violations = h[0];
+    *log_hash = h[1];
   });
 }
 

@@ -117,6 +117,144 @@ struct InstantLane {
     return h.found;
   }
 
+  // ---- The first crossing by monotone transition searches ---------------------------------------
+  // The bisection below (the reference's instant_search) returns the first integer t of [lo, hi)
+  // where Delta fires, whenever its exclusion test is conservative: a node (a, c] holding a firing
+  // integer is never dropped, so the time-ordered walk reaches the first one. It is conservative
+  // here because every step of Delta(t) is monotone in t on a segment whose groups all rise or all
+  // fall: the interpolation fraction, the products and sums of I(t) and its division by i_scale are
+  // correctly rounded operations of monotone arguments; pm::log is monotone on every positive float
+  // (checked exhaustively); the leak term leak * (float)((double)(t - t_ref) * 1e-9) rises with t
+  // (leak >= 0); and with n3_ref = 0 the bound of the falling-segment test is formed by the same
+  // operations as Delta(t) with larger (smaller) arguments. So the first crossing may be found by any
+  // means, and these searches find it from a few evaluations of the same fp32 Delta(t):
+  //   - Delta non-decreasing (rising or flat I) or non-increasing (falling I without leak): only
+  //     one level can be crossed after lo, and "crossed at t" is monotone in t: the transition is
+  //     bracketed by interpolation and bisection;
+  //   - falling I with leak: Delta(t) lies between fl(S(t) + dl(lo)) and fl(S(t) + dl(c)), both
+  //     non-increasing (S = the leak-free part), so the first crossing of the lower level lies
+  //     between their transitions, which are found as above and scanned in between; a segment where
+  //     the upper envelope allows an ON crossing, or the gap exceeds kScanLimit, takes the bisection.
+  // Returns false when the bisection must decide (mixed group slopes, n3_ref != 0).
+  static constexpr int64_t kScanLimit = 64;
+
+  BSX_HD BSX_INLINE float s_at(int64_t t, float& x) const {
+    x = log_ratio(photocurrent(t));
+    return (x - s3_ref) + (0.0f - n3_ref);
+  }
+  BSX_HD BSX_INLINE float dl_at(int64_t t) const { return leak * ns_to_s64<float>(t - t_ref) + dr_ref; }
+
+  // The first t of (a, b] where pred holds, pred monotone (false at a, true at b); v_a, v_b the
+  // values of the bracket ends, interpolated towards `level` (a guess only; the predicate decides).
+  template <class F>
+  BSX_HD BSX_INLINE int64_t first_true(int64_t a, float v_a, int64_t b, float v_b, float level, F&& pred) {
+    bool interp = true;
+    while (b - a > 1) {
+      int64_t m = a + (b - a) / 2;
+      if (interp) {
+        const float frac = (level - v_a) / (v_b - v_a);
+        if (frac > 0.0f && frac < 1.0f) {
+          m = a + static_cast<int64_t>(frac * static_cast<float>(b - a));
+          m = vmin(vmax(m, a + 1), b - 1);
+        }
+      }
+      const int64_t width = b - a;
+      float v_m;
+      ++n_leaves;
+      if (pred(m, v_m)) {
+        b = m;
+        v_b = v_m;
+      } else {
+        a = m;
+        v_a = v_m;
+      }
+      interp = (b - a) * 2 <= width;  // bisect next when interpolation stalled
+    }
+    return b;
+  }
+
+  BSX_HD BSX_INLINE bool transition_search(int64_t lo, int64_t hi, Hit& out) {
+    if (n3_ref != 0.0f || !(leak >= 0.0f)) return false;
+    bool up = true, down = true;
+    if (tb != kNoBreakpoint) {
+      for (int g = 0; g < kMaxG; ++g) {
+        if (g < G) {
+          if (vb[g] < va[g]) up = false;
+          if (vb[g] > va[g]) down = false;
+        }
+      }
+    }
+    if (!up && !down) return false;
+    float x_lo;
+    const float S_lo = s_at(lo, x_lo);
+    const float dl_lo = dl_at(lo);
+    const float D_lo = S_lo + dl_lo;
+    ++n_leaves;
+    out.t = lo;
+    out.x = x_lo;
+    out.found = D_lo >= theta_on || D_lo <= -theta_off;
+    out.on = D_lo >= theta_on;
+    if (out.found) return true;
+    if (hi - lo == 1) return true;
+    const int64_t c = hi - 1;
+    float x_c;
+    const float S_c = s_at(c, x_c);
+    ++n_leaves;
+    if (up || leak == 0.0f) {
+      const float D_c = S_c + dl_at(c);
+      const bool on = up;
+      const float level = on ? theta_on : -theta_off;
+      if (on ? !(D_c >= theta_on) : !(D_c <= -theta_off)) return true;  // no crossing
+      const int64_t t = first_true(lo, D_lo, c, D_c, level, [&](int64_t m, float& v) {
+        float xm;
+        v = s_at(m, xm) + dl_at(m);
+        return on ? v >= theta_on : v <= -theta_off;
+      });
+      out.found = true;
+      out.on = on;
+      out.t = t;
+      out.x = log_ratio(photocurrent(t));
+      return true;
+    }
+    // Falling I under leak: OFF only, unless the upper envelope reaches theta_on somewhere.
+    const float dl_c = dl_at(c);
+    if (S_lo + dl_c >= theta_on) return false;
+    const float L_c = S_c + dl_lo;
+    if (!(L_c <= -theta_off)) return true;  // the lower envelope stays above: no crossing
+    const int64_t tL = first_true(lo, D_lo, c, L_c, -theta_off, [&](int64_t m, float& v) {
+      float xm;
+      v = s_at(m, xm) + dl_lo;
+      return v <= -theta_off;
+    });
+    int64_t tU = c + 1;
+    const float U_c = S_c + dl_c;
+    if (U_c <= -theta_off) {
+      float x_l;
+      const float U_l = s_at(tL, x_l) + dl_c;
+      ++n_leaves;
+      tU = U_l <= -theta_off ? tL : first_true(tL, U_l, c, U_c, -theta_off, [&](int64_t m, float& v) {
+        float xm;
+        v = s_at(m, xm) + dl_c;
+        return v <= -theta_off;
+      });
+    }
+    const int64_t last = vmin(tU, c);
+    if (last - tL > kScanLimit) return false;
+    for (int64_t t = tL; t <= last; ++t) {
+      float xt;
+      const float D = s_at(t, xt) + dl_at(t);
+      ++n_leaves;
+      if (D <= -theta_off) {
+        out.found = true;
+        out.on = false;
+        out.t = t;
+        out.x = xt;
+        return true;
+      }
+    }
+    return true;  // tU = c + 1: no crossing (a finite tU always holds one)
+  }
+
   // First integer t in [lo, hi) of the current segment where Delta fires: time-ordered depth-first
   // bisection of (lo, hi - 1]. The pending right siblings are kept as their right ends only (offsets
   // from lo: a piece spans less than 2^23 ns), in the lane's column of a stack with `stride` entries
@@ -373,7 +511,8 @@ struct InstantLane {
   // open, as the piece loop continues after a crossing.
   template <class Sink>
   BSX_HD BSX_INLINE void search_step(Sink& sink, uint32_t* stack, int stride) {
-    const Hit hit = search(s_lo, e_cur, stack, stride);
+    Hit hit{false, false, s_lo, 0.0f};
+    if (!transition_search(s_lo, e_cur, hit)) hit = search(s_lo, e_cur, stack, stride);
     if (!hit.found) {
       t_search = vmax(t_search, e_cur);
       t_cur = e_cur;

@@ -14,11 +14,19 @@ hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, 
 // lengths, total = pos[n_runs]); phase B: the per-pixel walk (prep null: exact searches only).
 hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
                                const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
-                               int64_t t_state, InstantPrep* out, hipStream_t stream);
+                               int64_t t_state, InstantPrep* out, float* work, hipStream_t stream);
+// order (null: pixel order) lists the pixels in the order the lanes take them.
 hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
                           const RunsDev& runs, const int64_t* pos, const InstantPrep* prep, int64_t t_state,
                           int64_t t_end, RawEventDev* ev, unsigned long long cap, unsigned long long* counters,
-                          int block, hipStream_t stream);
+                          const uint32_t* order, int block, hipStream_t stream);
+// Heavy-first pixel order from the runs' estimated costs (work[run], zero-filled before phase A
+// adds to it): key (n_pixels bytes) and hist (128 words) are scratch, perm receives the order.
+hipError_t launch_cost_order(const int32_t* row_of, const float* work, uint32_t n_pixels, uint8_t* key,
+                             uint32_t* hist, uint32_t* perm, hipStream_t stream);
+hipError_t launch_fill_f32(float* dst, size_t n, float v, hipStream_t stream);
+// out[0] = pm::log monotonicity violations, out[1] = hash of its bits over every positive float.
+hipError_t launch_math_selfcheck(unsigned long long* out, hipStream_t stream);
 // Exclusive scan of int32 counts into int64 offsets (scratch: scan_scratch_count(n) values; the
 // total lands in scratch[scan_scratch_count(n) - 1]). Defined with the scene kernels.
 size_t scan_scratch_count(size_t n);

@@ -89,50 +89,127 @@ __global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateD
   wave_add(counters + 1 + kStClamps, clamps);
 }
 
+// Sum of v over the lanes of the wave that share `key` (keys non-decreasing along the lanes), added
+// once per key to dst[key] by the key's last lane (key < 0: no contribution).
+__device__ __forceinline__ void wave_segmented_add(float* dst, int key, float v) {
+  const int lane = __lane_id();
+  for (int off = 1; off < 64; off <<= 1) {
+    const float vu = __shfl_up(v, off);
+    const int ku = __shfl_up(key, off);
+    if (lane >= off && ku == key) v += vu;
+  }
+  const int kn = __shfl_down(key, 1);
+  if (key >= 0 && (lane == 63 || kn != key)) atomicAdd(dst + key, v);
+}
+
 // Phase A of the instantaneous response: one thread per segment of every run (position q of the
 // runs laid end to end, pos the exclusive scan of the run lengths), all runs at once, so the
 // transcendental work of the searches is spread evenly over the device whatever the runs' lengths.
+// It also estimates each run's walk cost for the heavy-first pixel order: one unit per segment plus
+// kSearchWeight per threshold's worth of log-intensity change (where the searches and events are).
+constexpr float kSearchWeight = 32.0f;
+
 template <int kMaxG>
 __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp, StateDev in, RunsDev runs,
                                                            const uint32_t* pixel, const int64_t* pos, int64_t n_runs,
-                                                           int64_t total, int64_t t_state, InstantPrep* out) {
+                                                           int64_t total, int64_t t_state, InstantPrep* out,
+                                                           float* work) {
   const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= total) return;
-  int64_t lo = 0, hi = n_runs;  // the run i with pos[i] <= q < pos[i + 1]
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (pos[mid] <= q) lo = mid;
-    else hi = mid;
-  }
-  const int64_t i = lo, j = q - pos[i];
-  const uint32_t p = pixel[i];
-  const int64_t s = runs.row_start[i];
-  InstantLane<kMaxG> L;
-  L.G = kc.groups;
-  L.irr_max = kc.irr_max;
-  L.dark = mp.dark[p];
-  L.scale = mp.scale[p];
-  L.leak = mp.leak[p];
-  L.i_scale = in.i_scale[p];
-  const int64_t* t = runs.t + s;
-  const float* v = runs.irr + s * kc.groups;
-  if (j == 0) {
-    if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
-      out[q] = InstantPrep{0.0f, 0.0f, 0.0f, 0.0f};
-      return;
+  int run = -1;
+  float cost = 0.0f;
+  if (q < total) {
+    int64_t lo = 0, hi = n_runs;  // the run i with pos[i] <= q < pos[i + 1]
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (pos[mid] <= q) lo = mid;
+      else hi = mid;
     }
-    L.ta = t_state;
-    for (int g = 0; g < kMaxG; ++g)
-      if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
-  } else {
-    L.ta = t[j - 1];
-    for (int g = 0; g < kMaxG; ++g)
-      if (g < kc.groups) L.va[g] = L.clampv(v[(j - 1) * kc.groups + g]);
+    const int64_t i = lo, j = q - pos[i];
+    run = static_cast<int>(i);
+    const uint32_t p = pixel[i];
+    const int64_t s = runs.row_start[i];
+    InstantLane<kMaxG> L;
+    L.G = kc.groups;
+    L.irr_max = kc.irr_max;
+    L.dark = mp.dark[p];
+    L.scale = mp.scale[p];
+    L.leak = mp.leak[p];
+    L.i_scale = in.i_scale[p];
+    const int64_t* t = runs.t + s;
+    const float* v = runs.irr + s * kc.groups;
+    bool empty = false;
+    if (j == 0) {
+      if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
+        out[q] = InstantPrep{0.0f, 0.0f, 0.0f, 0.0f};
+        empty = true;
+      } else {
+        L.ta = t_state;
+        for (int g = 0; g < kMaxG; ++g)
+          if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
+      }
+    } else {
+      L.ta = t[j - 1];
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < kc.groups) L.va[g] = L.clampv(v[(j - 1) * kc.groups + g]);
+    }
+    if (!empty) {
+      L.tb = t[j];
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
+      const InstantPrep r = L.prep_segment();
+      out[q] = r;
+      const float theta = vmin(mp.theta_on[p], mp.theta_off[p]);
+      cost = 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) / theta;
+    }
   }
-  L.tb = t[j];
-  for (int g = 0; g < kMaxG; ++g)
-    if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
-  out[q] = L.prep_segment();
+  if (work != nullptr) wave_segmented_add(work, run, cost);
+}
+
+// The pixel order of a walk: pixels by estimated cost, heaviest first (64 logarithmic buckets, bucket
+// 0 = no run), so that the long walks start first and waves hold pixels of like cost. A schedule
+// only: every pixel's result is independent of where and when it runs.
+__device__ __forceinline__ int cost_bucket(float w) {
+  if (!(w > 0.0f)) return 0;
+  const int b = 1 + static_cast<int>(4.0f * __log2f(1.0f + w));
+  return b < 1 ? 1 : (b > 63 ? 63 : b);
+}
+
+__global__ __launch_bounds__(256) void order_count_kernel(const int32_t* row_of, const float* work, uint32_t n,
+                                                          uint8_t* key, uint32_t* hist) {
+  __shared__ uint32_t cnt[64];
+  if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) {
+    const int32_t r = row_of != nullptr ? row_of[p] : -1;
+    const int b = r >= 0 ? cost_bucket(work[r]) : 0;
+    key[p] = static_cast<uint8_t>(b);
+    atomicAdd(&cnt[b], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64 && cnt[threadIdx.x] != 0) atomicAdd(hist + threadIdx.x, cnt[threadIdx.x]);
+}
+
+// cursor[b] = the number of pixels in buckets above b (one wave).
+__global__ void order_offsets_kernel(const uint32_t* hist, uint32_t* cursor) {
+  const int b = threadIdx.x;
+  uint32_t above = 0;
+  for (int k = b + 1; k < 64; ++k) above += hist[k];
+  cursor[b] = above;
+}
+
+__global__ __launch_bounds__(256) void order_scatter_kernel(const uint8_t* key, uint32_t n, uint32_t* cursor,
+                                                            uint32_t* perm) {
+  __shared__ uint32_t cnt[64], base[64];
+  if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = p < n ? key[p] : 0;
+  const uint32_t local = p < n ? atomicAdd(&cnt[b], 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x < 64 && cnt[threadIdx.x] != 0) base[threadIdx.x] = atomicAdd(cursor + threadIdx.x, cnt[threadIdx.x]);
+  __syncthreads();
+  if (p < n) perm[base[b] + local] = p;
 }
 
 // Wave scheduling of the instantaneous walk: segments a walking lane does before the wave looks
@@ -145,11 +222,12 @@ template <int kMaxG>
 __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsDev mp, StateDev in, StateDev out, RunsDev runs,
                                                       const int64_t* pos, const InstantPrep* prep, int64_t t_state,
                                                       int64_t t_end, RawEventDev* ev, unsigned long long cap,
-                                                      unsigned long long* counters) {
+                                                      unsigned long long* counters, const uint32_t* order) {
   // The lanes' bisection stacks, one column per lane (conflict-free: a level is a row of the block;
   // blockDim.x <= kInstantBlock by the launch bound).
   __shared__ uint32_t stacks[InstantLane<kMaxG>::kSearchDepth * kInstantBlock];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane_index = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = order != nullptr && lane_index < kc.n_pixels ? static_cast<int>(order[lane_index]) : lane_index;
   InstantLane<kMaxG> L;
   L.n_substeps = L.n_pieces = L.n_leaves = 0;
   WaveSink sink{ev, cap, counters};
@@ -281,6 +359,33 @@ __global__ void check_samples_kernel(const float* irr, size_t count, float vmax_
   wave_add(out + 1, clamped);
 }
 
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Every positive finite float: monotonicity violations of pm::log and the hash of its bits
+// (grid-stride over the bit patterns u = 1 .. 0x7f7fffff).
+__global__ __launch_bounds__(256) void log_check_kernel(unsigned long long* out) {
+  uint64_t bad = 0, hash = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t u = 1 + blockIdx.x * blockDim.x + threadIdx.x; u < 0x7f800000u; u += stride) {
+    const float y = pm::log(__uint_as_float(u));
+    hash += splitmix64((static_cast<uint64_t>(u) << 32) | __float_as_uint(y));
+    if (u + 1 < 0x7f800000u && pm::log(__uint_as_float(u + 1)) < y) ++bad;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    bad += __shfl_xor(bad, off);
+    hash += __shfl_xor(hash, off);
+  }
+  if (__lane_id() == 0) {
+    atomicAdd(out + 0, static_cast<unsigned long long>(bad));
+    atomicAdd(out + 1, static_cast<unsigned long long>(hash));
+  }
+}
+
 inline dim3 grid_for(size_t n, int block) { return dim3(static_cast<unsigned>((n + block - 1) / block)); }
 
 }  // namespace
@@ -298,10 +403,11 @@ hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, 
 
 hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
                                const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
-                               int64_t t_state, InstantPrep* out, hipStream_t stream) {
+                               int64_t t_state, InstantPrep* out, float* work, hipStream_t stream) {
   if (total <= 0 || n_runs <= 0) return hipSuccess;
   const dim3 grid = grid_for(static_cast<size_t>(total), 256);
-#define BSX_PREP(G) instant_prep_kernel<G><<<grid, 256, 0, stream>>>(kc, mp, in, runs, pixel, pos, n_runs, total, t_state, out)
+#define BSX_PREP(G) \
+  instant_prep_kernel<G><<<grid, 256, 0, stream>>>(kc, mp, in, runs, pixel, pos, n_runs, total, t_state, out, work)
   if (kc.groups <= 1) BSX_PREP(1);
   else if (kc.groups <= 2) BSX_PREP(2);
   else if (kc.groups <= 4) BSX_PREP(4);
@@ -313,15 +419,39 @@ hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateD
 hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
                           const RunsDev& runs, const int64_t* pos, const InstantPrep* prep, int64_t t_state,
                           int64_t t_end, RawEventDev* ev, unsigned long long cap, unsigned long long* counters,
-                          int block, hipStream_t stream) {
+                          const uint32_t* order, int block, hipStream_t stream) {
   const dim3 grid = grid_for(kc.n_pixels, block);
-#define BSX_ADV(G) \
-  instant_kernel<G><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, pos, prep, t_state, t_end, ev, cap, counters)
+#define BSX_ADV(G)                                                                                              \
+  instant_kernel<G><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, pos, prep, t_state, t_end, ev, cap, counters, \
+                                                order)
   if (kc.groups <= 1) BSX_ADV(1);
   else if (kc.groups <= 2) BSX_ADV(2);
   else if (kc.groups <= 4) BSX_ADV(4);
   else BSX_ADV(8);
 #undef BSX_ADV
+  return hipGetLastError();
+}
+
+hipError_t launch_cost_order(const int32_t* row_of, const float* work, uint32_t n_pixels, uint8_t* key,
+                             uint32_t* hist, uint32_t* perm, hipStream_t stream) {
+  if (n_pixels == 0) return hipSuccess;
+  const int block = 256;
+  fill_i32_kernel<<<1, 128, 0, stream>>>(reinterpret_cast<int32_t*>(hist), 128, 0);
+  order_count_kernel<<<grid_for(n_pixels, block), block, 0, stream>>>(row_of, work, n_pixels, key, hist);
+  order_offsets_kernel<<<1, 64, 0, stream>>>(hist, hist + 64);
+  order_scatter_kernel<<<grid_for(n_pixels, block), block, 0, stream>>>(key, n_pixels, hist + 64, perm);
+  return hipGetLastError();
+}
+
+hipError_t launch_math_selfcheck(unsigned long long* out, hipStream_t stream) {
+  fill_u64_kernel<<<1, 64, 0, stream>>>(out, 2, 0);
+  log_check_kernel<<<2048, 256, 0, stream>>>(out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* dst, size_t n, float v, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  fill_i32_kernel<<<grid_for(n, 256), 256, 0, stream>>>(reinterpret_cast<int32_t*>(dst), n, __builtin_bit_cast(int32_t, v));
   return hipGetLastError();
 }
 

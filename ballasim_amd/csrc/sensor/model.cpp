@@ -369,20 +369,30 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
     hip_check(hipStreamSynchronize(stream), "scan runs");
   }
   d_prep_.ensure(std::max<int64_t>(total, 1) * sizeof(InstantPrep));
+  d_work_.ensure(std::max<size_t>(n_runs, 1) * sizeof(float));
+  d_key_.ensure(P_);
+  d_perm_.ensure(P_ * sizeof(uint32_t));
+  d_order_hist_.ensure(128 * sizeof(uint32_t));
   unsigned long long c[1 + kStCount];
   bool prepared = false;
   while (true) {
     hip_check(launch_fill_u64(d_counters_.as<unsigned long long>(), 1 + kStCount, 0, stream), "fill counters");
     hip_check(hipEventRecord(ev0_, stream), "event record");
-    if (!prepared) {  // phase A reads only the call's input state: a re-run reuses it
+    if (!prepared) {  // phase A and the pixel order read only the call's input: a re-run reuses them
+      hip_check(launch_fill_f32(d_work_.as<float>(), n_runs, 0.0f, stream), "fill work");
       hip_check(launch_instant_prep(kc, mp, state_view(cur_), runs, pixel, d_pos_.as<const int64_t>(),
-                                    static_cast<int64_t>(n_runs), total, t_state_, d_prep_.as<InstantPrep>(), stream),
+                                    static_cast<int64_t>(n_runs), total, t_state_, d_prep_.as<InstantPrep>(),
+                                    d_work_.as<float>(), stream),
                 "instant prep kernel");
+      hip_check(launch_cost_order(runs.row_of, d_work_.as<const float>(), static_cast<uint32_t>(P_),
+                                  d_key_.as<uint8_t>(), d_order_hist_.as<uint32_t>(), d_perm_.as<uint32_t>(), stream),
+                "pixel order");
       prepared = true;
     }
     hip_check(launch_instant(kc, mp, state_view(cur_), state_view(1 - cur_), runs, d_pos_.as<const int64_t>(),
                              total > 0 ? d_prep_.as<const InstantPrep>() : nullptr, t_state_, t_end,
-                             d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(), block_, stream),
+                             d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(),
+                             d_perm_.as<const uint32_t>(), block_, stream),
               "instant kernel");
     hip_check(hipEventRecord(ev1_, stream), "event record");
     hip_check(hipMemcpyAsync(c, d_counters_.get(), sizeof(c), hipMemcpyDeviceToHost, stream), "read counters");

@@ -169,6 +169,13 @@ int bsx_last_error(char* buf, size_t n);
 /* Visible HIP devices. Reference: sensor_cuda_available(), src/sensor/sensor_model.h:52. */
 int bsx_device_count(int32_t* out);
 
+/* Self-check of the device math on the current device (no reference counterpart). Over every
+ * positive finite fp32 x (bit patterns 1 .. 0x7f7fffff): the number of x where the device log of
+ * the next float is below the log of x (the exact searches of the instantaneous response rely on
+ * none), and an order-independent hash of all log bit patterns, sum of splitmix64((u << 32) | bits)
+ * mod 2^64, to compare with a host evaluation. */
+int bsx_math_selfcheck(uint64_t* log_violations, uint64_t* log_hash);
+
 /* Reference: preset(name), src/sensor/params.h:176; preset_names(), :177 (names joined by ','). */
 int bsx_params_preset(const char* name, bsx_params* out);
 int bsx_preset_names(char* buf, size_t n);

@@ -312,6 +312,57 @@ float orc_dawsonf(float x) {
   return copysignf((float)0x1.20dd750429b6dp-1 * orc_expf(-xp * xp) * sum, x);
 }
 
+static uint64_t splitmix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+typedef struct {
+  uint32_t u0, u1;
+  uint64_t bad, hash;
+} logchk;
+
+static void* log_check_worker(void* arg) {
+  logchk* c = (logchk*)arg;
+  for (uint32_t u = c->u0; u < c->u1; ++u) {
+    float x, x1;
+    memcpy(&x, &u, 4);
+    const float y = orc_logf(x);
+    uint32_t b;
+    memcpy(&b, &y, 4);
+    c->hash += splitmix64(((uint64_t)u << 32) | b);
+    if (u + 1 < 0x7f800000u) {
+      const uint32_t v = u + 1;
+      memcpy(&x1, &v, 4);
+      if (orc_logf(x1) < y) ++c->bad;
+    }
+  }
+  return NULL;
+}
+
+void orc_log_check(int threads, uint64_t* violations, uint64_t* hash) {
+  if (threads < 1) threads = 1;
+  if (threads > 64) threads = 64;
+  pthread_t th[64];
+  logchk c[64];
+  const uint32_t n = 0x7f800000u - 1u;
+  for (int i = 0; i < threads; ++i) {
+    c[i].u0 = 1u + (uint32_t)((uint64_t)n * i / threads);
+    c[i].u1 = 1u + (uint32_t)((uint64_t)n * (i + 1) / threads);
+    c[i].bad = c[i].hash = 0;
+    pthread_create(&th[i], NULL, log_check_worker, &c[i]);
+  }
+  *violations = 0;
+  *hash = 0;
+  for (int i = 0; i < threads; ++i) {
+    pthread_join(th[i], NULL);
+    *violations += c[i].bad;
+    *hash += c[i].hash;
+  }
+}
+
 /* ------------------------------------------------------------------------------------------------
  * The model
  * ---------------------------------------------------------------------------------------------- */

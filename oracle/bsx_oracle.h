@@ -90,6 +90,9 @@ float orc_resid_expf(int j, float y);
 float orc_erfcxf(float x);
 float orc_dawsonf(float x);
 float orc_z_step(float z0, float I0, float I1, float h, float k);
+/* Over every positive finite float (bit patterns 1 .. 0x7f7fffff): monotonicity violations of
+ * orc_logf and the sum mod 2^64 of splitmix64((u << 32) | bits(orc_logf)), on `threads` threads. */
+void orc_log_check(int threads, uint64_t* violations, uint64_t* hash);
 /* The comparator input g(u) = c0 + c1 u + c2 u^2 + c3 u^3 + C0 R_j(a0 u) + C1 R_j(a1 u) (j = order) and
  * its first firing: g at u, its critical points on (lo, hi) (returns their count, at most 6, into out),
  * and the first integer ns in [max(lo, e0), e1) where g - th_on >= 0 or -g - th_off >= 0 (1 with *on and

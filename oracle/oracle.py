@@ -87,6 +87,7 @@ def oracle_lib():
         lib.orc_sincosf.argtypes = [C.c_float, P, P]
         lib.orc_z_step.restype = C.c_float
         lib.orc_z_step.argtypes = [C.c_float] * 5
+        lib.orc_log_check.argtypes = [C.c_int, P, P]
         lib.orc_xc_eval.restype = C.c_float
         lib.orc_xc_eval.argtypes = [P, P, P, C.c_int, C.c_float]
         lib.orc_xc_eval_n.argtypes = [P, P, P, C.c_int, P, C.c_size_t, P]

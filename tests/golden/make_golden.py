@@ -64,6 +64,12 @@ def main():
         manifest["cases"][name] = {"params": params_dict(p), **{k: v for k, v in spec.items() if k != "overrides"},
                                    "overrides": spec.get("overrides", {}),
                                    "events": int(sum(len(arrays[f"events_{k}"]) for k in range(len(calls))))}
+    # The oracle's log over every positive float: monotone, and the hash the device log must match.
+    v = np.zeros(1, np.uint64)
+    h = np.zeros(1, np.uint64)
+    O.oracle_lib().orc_log_check(os.cpu_count() or 1, v.ctypes.data_as(C.c_void_p), h.ctypes.data_as(C.c_void_p))
+    manifest["log_check"] = {"violations": int(v[0]), "hash": "0x%016x" % int(h[0]),
+                             "source": "oracle orc_logf over bit patterns 1 .. 0x7f7fffff"}
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
     print("wrote", len(manifest["cases"]), "cases and", len(manifest["maps"]), "map sets")

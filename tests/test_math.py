@@ -135,3 +135,16 @@ def test_special_functions():
                 exact = sum((-y) ** i / math.factorial(i) for i in range(j, j + 25))
             got = lib.orc_resid_expf(j, float(y))
             assert abs(got - exact) <= 2e-6 * abs(exact) + 1e-30, (j, y)
+
+
+def test_oracle_log_monotone_on_every_float():
+    """orc_logf never decreases from one positive float to the next (all 2^31 - 2^23 of them), and its
+    bits hash to the value recorded in the golden manifest (the device log is checked against it)."""
+    import json
+    import os
+    man = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "manifest.json")))["log_check"]
+    v = np.zeros(1, np.uint64)
+    h = np.zeros(1, np.uint64)
+    O.oracle_lib().orc_log_check(os.cpu_count() or 1, v.ctypes.data_as(C.c_void_p), h.ctypes.data_as(C.c_void_p))
+    assert int(v[0]) == 0 == man["violations"]
+    assert "0x%016x" % int(h[0]) == man["hash"]
