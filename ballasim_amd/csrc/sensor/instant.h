@@ -19,6 +19,7 @@
 //     dark current > 0, which leaves it bit for bit unchanged.
 #pragma once
 
+#include "../core/philox.h"
 #include "../core/pmath.h"
 #include "kconst.h"
 
@@ -48,6 +49,12 @@ struct InstantLane {
   // The current segment [ta, tb) with its end values (tb = kNoBreakpoint: the held segment).
   int64_t ta, tb;
   float va[kMaxG], vb[kMaxG];
+  // The rate-specified shot-noise process (this library's extension; rate 0 = off).
+  double shot_rate;
+  uint64_t seed;
+  uint32_t camera;
+  int64_t shot_next;
+  uint32_t shot_index;
   // Counters.
   uint32_t n_substeps, n_pieces, n_leaves;
 
@@ -311,8 +318,9 @@ struct InstantLane {
     return kLabelNoise;
   }
 
+  // shot: an event of the shot-noise process (labelled NOISE, flagged kFlagShot, reset as any event).
   template <class Sink>
-  BSX_HD BSX_INLINE void fire(int64_t t, bool on, float s3, float n3, Sink& sink) {
+  BSX_HD BSX_INLINE void fire(int64_t t, bool on, float s3, float n3, Sink& sink, bool shot = false) {
     const float ds = s3 - s3_ref;
     const float dn = n3 - n3_ref;
     const float l = leak * ns_to_s64<float>(t - t_ref);
@@ -320,7 +328,9 @@ struct InstantLane {
     const float D = ds + dn + dl;
     const bool first = (flags & kFlagFirstAfterInit) != 0;
     uint8_t label;
-    if (first) {
+    if (shot) {
+      label = kLabelNoise;
+    } else if (first) {
       const float Dl = ds + dn + l;
       label = (on ? Dl > 0.0f : Dl < 0.0f) ? split_label(ds, l, Dl, label_cut) : kLabelNoise;
     } else {
@@ -330,7 +340,7 @@ struct InstantLane {
     ev.pixel = pixel;
     ev.polarity = on ? 1 : 0;
     ev.label = label;
-    ev.flags = static_cast<uint16_t>(flags);
+    ev.flags = static_cast<uint16_t>(flags | (shot ? kFlagShot : 0u));
     ev.t_cross = t;
     ev.ds = ds;
     ev.dn = dn;
@@ -433,6 +443,34 @@ struct InstantLane {
       if (g < G) v[g] = clampv(irr[static_cast<int64_t>(i) * G + g]);
   }
 
+  // ---- Shot noise ---------------------------------------------------------------------------------
+  // Per pixel a Poisson process of rate r: arrival k follows arrival k - 1 (or the init time) after
+  // ceil(-ln(U) / r * 1e9) ns (at least 1), U the open uniform of word 0 of
+  // Philox(seed, kPurposeShot, camera, pixel, k), whose word 1 gives the polarity (low bit). The
+  // arithmetic is fp64 with correctly rounded operations and the library's log, so host and device
+  // draw the same schedule.
+  BSX_HD BSX_INLINE static int64_t shot_gap(double rate, uint64_t seed, uint32_t camera, uint32_t pixel,
+                                            uint64_t index, bool* on) {
+    const Word4 w = draw(seed, kPurposeShot, camera, pixel, index);
+    const double e = -pm::log(open_uniform64(w.v[0]));
+    const double ns = __builtin_ceil(e / rate * 1e9);
+    if (on != nullptr) *on = (w.v[1] & 1u) != 0;
+    if (!(ns >= 1.0)) return 1;
+    if (ns >= 0x1p62) return int64_t{1} << 62;
+    return static_cast<int64_t>(ns);
+  }
+
+  // The arrival at T = shot_next (T inside the current segment): an event unless T falls in the
+  // dead time (T < t_search, which covers the physical blind interval), then the next arrival.
+  template <class Sink>
+  BSX_HD BSX_INLINE void shot_arrival(int64_t T, Sink& sink) {
+    bool on = false;
+    (void)shot_gap(shot_rate, seed, camera, pixel, shot_index, &on);
+    if (T >= t_search) fire(T, on, log_ratio(photocurrent(T)), 0.0f, sink, true);
+    shot_index += 1;
+    shot_next = T + shot_gap(shot_rate, seed, camera, pixel, shot_index, nullptr);
+  }
+
   // ---- The call as a resumable lane program --------------------------------------------------
   // advance_instant walks the segments of the augmented samples in order, cutting each into pieces at
   // multiples of 2^23 ns. Here that walk is split at its exact searches: walk() runs until the next
@@ -487,7 +525,11 @@ struct InstantLane {
         seg_open = true;
       }
       if (t_cur < stop) {
-        e_cur = vmin(stop, ((t_cur >> kPlainSegmentLog2) + 1) * kPlainSegmentNs);
+        if (!in_piece && shot_next == t_cur) {
+          shot_arrival(t_cur, sink);
+          continue;
+        }
+        e_cur = vmin(vmin(stop, ((t_cur >> kPlainSegmentLog2) + 1) * kPlainSegmentNs), shot_next);
         if (!in_piece) {
           ++n_substeps;
           ++n_pieces;

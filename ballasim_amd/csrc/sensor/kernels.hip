@@ -83,7 +83,9 @@ __global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateD
     st.blind_until[p] = kNotBlind;
     st.t_search[p] = t0 + 1;
     st.flags[p] = kFlagFirstAfterInit;
-    st.shot_next[p] = INT64_MAX;
+    st.shot_next[p] = kc.shot_rate_hz > 0.0 ? t0 + InstantLane<kMaxG>::shot_gap(kc.shot_rate_hz, kc.seed, kc.camera,
+                                                                                   static_cast<uint32_t>(p), 0, nullptr)
+                                            : INT64_MAX;
     st.shot_index[p] = 0;
   }
   wave_add(counters + 1 + kStClamps, clamps);
@@ -269,6 +271,11 @@ __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsD
     L.blind_until = in.blind_until[p];
     L.t_search = in.t_search[p];
     L.flags = in.flags[p];
+    L.shot_rate = kc.shot_rate_hz;
+    L.seed = kc.seed;
+    L.camera = kc.camera;
+    L.shot_next = in.shot_next[p];
+    L.shot_index = in.shot_index[p];
     L.begin(t_end);
   }
   // The wave's lanes walk their pixels; a lane that reaches an exact search waits until enough lanes
@@ -305,8 +312,8 @@ __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsD
     out.blind_until[p] = L.blind_until;
     out.t_search[p] = L.t_search;
     out.flags[p] = static_cast<uint16_t>(L.flags);
-    out.shot_next[p] = in.shot_next[p];
-    out.shot_index[p] = in.shot_index[p];
+    out.shot_next[p] = L.shot_next;
+    out.shot_index[p] = L.shot_index;
   }
   wave_add(counters + 1 + kStSubsteps, L.n_substeps);
   wave_add(counters + 1 + kStPieces, L.n_pieces);

@@ -187,6 +187,8 @@ void validate(const SensorParams& p) {
   need(p.limiter == 0, "the rate limiter is not supported by the bsx backend");
   need(p.initial_state == 0, "initial_state stationary is not supported by the bsx backend");
   need(!noise_on(p), "the cascade's in-path noise (noise on) is not supported by the bsx backend yet");
+  need(p.shot_rate_hz == 0 || p.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS,
+       "shot_rate_hz > 0 (the shot-noise process) needs the instantaneous photoreceptor");
 }
 
 double arrhenius(double ea_eV, double temperature_K, double reference_K) {

@@ -3,7 +3,8 @@
 Workload (the GPU configuration of the metric): a 1280x720 sensor watching 64 textured rectangles
 with affine motion (velocity U(+-2000 px/s), rotation U(+-3 rad/s), contrast U[0.1, 0.9]) over a
 log-uniform 10-1000 lux background, for 500 ms, sampled by the plain error-tolerance controller
-(refine_tol_log 0.01) on the GPU. Thresholds 0.15 +- 0.03, refractory 50 us, leak 0.05 Hz/px.
+(refine_tol_log 0.01) on the GPU. Thresholds 0.15 +- 0.03, refractory 50 us, leak 0.05 Hz/px, shot
+noise 0.1 Hz/px (Philox-keyed Poisson arrivals per pixel).
 A step is one advance of every pixel over one 2^23 ns window; all windows' runs are generated on the
 device and resident in HBM before the timed region. value = pixel-samples advanced per second.
 
@@ -34,6 +35,9 @@ def sensor_params(bsx, photoreceptor):
     p.sigma_theta_on = p.sigma_theta_off = 0.03
     p.refractory_s = 50e-6
     p.leak_rate_hz = 0.05
+    # shot noise 0.1 Hz/px: the library's rate-specified Philox process (the instantaneous response
+    # only; the cascade runs noise-free)
+    p.shot_rate_hz = 0.1 if photoreceptor == "instantaneous" else 0.0
     from ballasim_amd.scene import LUX_TO_FA
     p.illuminance_scale = LUX_TO_FA
     return p
@@ -137,8 +141,9 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (procedural scene, seed %d)" % args.seed,
         "config": {"workload": "1280x720 sensor, 64 textured affine rectangles, %d windows of 2^23 ns "
-                               "(%.1f ms), plain controller tol 0.01, %s response, noise off"
-                               % (args.steps, args.steps * WINDOW / 1e6, args.photoreceptor),
+                               "(%.1f ms), plain controller tol 0.01, %s response, leak 0.05 Hz/px, %s"
+                               % (args.steps, args.steps * WINDOW / 1e6, args.photoreceptor,
+                                  "shot noise 0.1 Hz/px" if p.shot_rate_hz > 0 else "noise off"),
                    "pixels": P, "samples": samples, "events": events},
         "events_per_s": events / elapsed,
         "samples_per_event": samples / max(events, 1),

@@ -460,8 +460,10 @@ orc_model* orc_create(const bsx_params* p, uint32_t width, uint32_t height, uint
     return NULL;
   }
   if (p->veiling_glare != 0 || p->tau_hp != 0 || p->hot_fraction != 0 ||
-      (p->noise && p->photoreceptor == BSX_PHOTORECEPTOR_CASCADE)) {
-    set_err(err, err_n, "orc_create: glare, high-pass filter, hot pixels and cascade noise are not restated");
+      (p->noise && p->photoreceptor == BSX_PHOTORECEPTOR_CASCADE) ||
+      (p->shot_rate_hz > 0 && p->photoreceptor == BSX_PHOTORECEPTOR_CASCADE)) {
+    set_err(err, err_n, "orc_create: glare, high-pass filter, hot pixels and cascade noise (in-path or shot) are not "
+                        "restated");
     return NULL;
   }
   orc_model* m = (orc_model*)calloc(1, sizeof(orc_model));
@@ -548,6 +550,8 @@ typedef struct {
   float z, x1, s2, s3, i_scale, tv, s3_ref, n3_ref, dr_ref;
   int64_t t_ref, blind_until, t_search;
   uint16_t flags;
+  int64_t shot_next;
+  uint32_t shot_index;
   evvec* out;
   orc_stats* stats;
 } pix;
@@ -618,7 +622,12 @@ static uint8_t split_label(float ds, float dl, float D, float cut) {
   return 1;
 }
 
-static void fire(pix* k, int64_t t, int on, float s3, float n3) {
+static void fire_ex(pix* k, int64_t t, int on, float s3, float n3, int shot);
+static void fire(pix* k, int64_t t, int on, float s3, float n3) { fire_ex(k, t, on, s3, n3, 0); }
+
+/* A shot-noise event (this library's extension): an event of the drawn polarity at the arrival,
+ * labelled NOISE and flagged (1 << 13), resetting the pixel as a comparator event does. */
+static void fire_ex(pix* k, int64_t t, int on, float s3, float n3, int shot) {
   const float cut = (float)k->m->p.label_share_cut;
   const float ds = s3 - k->s3_ref;
   const float dn = n3 - k->n3_ref;
@@ -627,7 +636,9 @@ static void fire(pix* k, int64_t t, int on, float s3, float n3) {
   const float D = ds + dn + dl;
   uint8_t label;
   const int first = (k->flags & (1u << 12)) != 0;
-  if (first) {
+  if (shot) {
+    label = 1;
+  } else if (first) {
     const float Dl = ds + dn + l;
     label = (on ? Dl > 0.0f : Dl < 0.0f) ? split_label(ds, l, Dl, cut) : 1;
   } else {
@@ -637,7 +648,7 @@ static void fire(pix* k, int64_t t, int on, float s3, float n3) {
   e.pixel = k->pixel;
   e.polarity = on ? 1 : 0;
   e.label = label;
-  e.flags = k->flags;
+  e.flags = (uint16_t)(k->flags | (shot ? (1u << 13) : 0u));
   e.t_cross = t;
   e.ds = ds;
   e.dn = dn;
@@ -732,16 +743,49 @@ static ihit instant_search(pix* k, int seg, int64_t lo, int64_t hi) {
   return out;
 }
 
+/* The rate-specified shot-noise process (this library's extension, no reference counterpart): per
+ * pixel a Poisson process of rate r = shot_rate_hz. Arrival k follows arrival k - 1 (or the init
+ * time) after ceil(-ln(U) / r * 1e9) ns (at least 1), U the open uniform of word 0 of
+ * Philox(seed, 0x101, camera, pixel, k); word 1's low bit is the event's polarity. */
+static int64_t shot_gap(const orc_model* m, uint32_t pixel, uint64_t index, int* on) {
+  uint32_t w[4];
+  orc_draw(m->seed, 0x101, m->camera, pixel, index, w);
+  const double e = -orc_log(u_open64(w[0]));
+  const double ns = ceil(e / m->p.shot_rate_hz * 1e9);
+  if (on != NULL) *on = (int)(w[1] & 1u);
+  if (!(ns >= 1.0)) return 1;
+  if (ns >= 0x1p62) return (int64_t)1 << 62;
+  return (int64_t)ns;
+}
+
+/* The arrival at T = shot_next: an event unless T falls in the dead time (T < t_search, which
+ * covers the physical blind interval), then the next arrival. */
+static void shot_arrival(pix* k, int seg, int64_t T) {
+  int on = 0;
+  (void)shot_gap(k->m, k->pixel, k->shot_index, &on);
+  if (T >= k->t_search) {
+    seg = seg_of(k, seg, T);
+    fire_ex(k, T, on, log_ratio(k, photocurrent(k, seg, T)), 0.0f, 1);
+  }
+  k->shot_index += 1;
+  k->shot_next = T + shot_gap(k->m, k->pixel, k->shot_index, NULL);
+}
+
 static void advance_instant(pix* k, int64_t t_to) {
   int seg = 0;
   int64_t t = k->t_state;
   const float eps_ci = (float)k->m->p.eps_ci;
   while (t < t_to) {
     seg = seg_of(k, seg, t);
+    if (k->shot_next == t) {
+      shot_arrival(k, seg, t);
+      continue;
+    }
     int64_t e = t_to;
     const int64_t se = seg_end(k, seg), wb = ((t >> 23) + 1) * ((int64_t)1 << 23);
     if (se < e) e = se;
     if (wb < e) e = wb;
+    if (k->shot_next < e) e = k->shot_next;
     ++k->stats->substeps;
     ++k->stats->pieces;
     for (;;) {
@@ -778,6 +822,8 @@ static void load(pix* k, const orc_model* m, size_t p) {
   k->blind_until = s->blind_until;
   k->t_search = s->t_search;
   k->flags = s->flags;
+  k->shot_next = s->shot_next;
+  k->shot_index = s->shot_index;
   for (int g = 0; g < m->G; ++g) k->held[g] = s->held[g];
 }
 
@@ -796,6 +842,8 @@ static void store(const pix* k, orc_model* m, size_t p) {
   s->blind_until = k->blind_until;
   s->t_search = k->t_search;
   s->flags = k->flags;
+  s->shot_next = k->shot_next;
+  s->shot_index = k->shot_index;
   for (int g = 0; g < m->G; ++g) s->held[g] = k->held[g];
 }
 
@@ -843,6 +891,8 @@ int orc_init(orc_model* m, int64_t t0, const float* irr0) {
     k.blind_until = INT64_MIN;
     k.t_search = t0 + 1;
     k.flags = 1u << 12;
+    k.shot_index = 0;
+    k.shot_next = m->p.shot_rate_hz > 0.0 ? t0 + shot_gap(m, (uint32_t)p, 0, NULL) : INT64_MAX;
     store(&k, m, p);
   }
   m->t_state = t0;

@@ -49,6 +49,8 @@ typedef struct orc_state {
   int64_t t_ref, blind_until, t_search;
   uint16_t flags;
   float held[8];
+  int64_t shot_next;   /* the shot-noise process's next arrival (INT64_MAX: off) */
+  uint32_t shot_index; /* its index */
 } orc_state;
 
 typedef struct orc_stats {

@@ -21,7 +21,7 @@ EVENT_DTYPE = np.dtype([("pixel", "<u4"), ("polarity", "u1"), ("label", "u1"), (
 STATE_DTYPE = np.dtype([("z", "<f8"), ("x1", "<f8"), ("s2", "<f8"), ("s3", "<f8"), ("i_scale", "<f8"),
                         ("tv", "<f8"), ("s3_ref", "<f8"), ("n3_ref", "<f8"), ("dr_ref", "<f8"),
                         ("t_ref", "<i8"), ("blind_until", "<i8"), ("t_search", "<i8"), ("flags", "<u2"),
-                        ("held", "<f4", (8,))], align=True)
+                        ("held", "<f4", (8,)), ("shot_next", "<i8"), ("shot_index", "<u4")], align=True)
 # The reference's PixelState (200 B; src/sensor/pixel.h:105-121).
 REF_STATE_DTYPE = np.dtype([("z", "<f8"), ("x1", "<f8"), ("s2", "<f8"), ("s3", "<f8"), ("i_scale", "<f8"),
                             ("tv", "<f8"), ("nk", "<f8", (3,)), ("nk1", "<f8", (3,)), ("ln_a1", "<f8"),

@@ -45,3 +45,38 @@ def first_difference(a, b):
         if a[i] != b[i]:
             return i, a[i], b[i]
     return n, None, None
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 (Salmon et al., SC'11) in plain Python integers: an independent restatement for
+    the tests (the library's and the oracle's are pinned by the Random123 known-answer vectors)."""
+    M = 0xFFFFFFFF
+    c = list(ctr)
+    k0, k1 = key
+    for _ in range(10):
+        pa = 0xD2511F53 * c[0]
+        pb = 0xCD9E8D57 * c[2]
+        c = [((pb >> 32) ^ c[1] ^ k0) & M, pb & M, ((pa >> 32) ^ c[3] ^ k1) & M, pa & M]
+        k0 = (k0 + 0x9E3779B9) & M
+        k1 = (k1 + 0xBB67AE85) & M
+    return c
+
+
+def shot_schedule(seed, camera, pixel, t0, rate_hz, t_end):
+    """Arrival times and polarities of the shot-noise process of one pixel up to t_end, in fp64 with
+    the standard library's log (may differ from the library's by one ns where a ceil is near an
+    integer)."""
+    import math
+    out = []
+    t = t0
+    k = 0
+    while True:
+        w = philox4x32_10([(0x101 << 8) | camera, pixel, k & 0xFFFFFFFF, k >> 32],
+                          [seed & 0xFFFFFFFF, seed >> 32])
+        u = (w[0] + 0.5) * 2.0 ** -32
+        gap = max(1, math.ceil(-math.log(u) / rate_hz * 1e9))
+        t += gap
+        if t >= t_end:
+            return out
+        out.append((t, w[1] & 1))
+        k += 1

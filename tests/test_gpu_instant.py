@@ -18,6 +18,10 @@ CASES = {
                               theta_off=0.12, eps_ci=0.001),
     "symmetric_readout": dict(reset_mode="physical", refractory_s=0.0, initial_phase="symmetric", eps_ci=-0.002,
                               readout_quantum_s=1e-6, readout_delay_s=5e-6, clock_drift_ppm=3.0),
+    # the rate-specified shot-noise process, at rates far above the physical ones so that a few calls
+    # hold thousands of arrivals (inside segments, at segment ends, in dead time)
+    "shot_physical": dict(reset_mode="physical", refractory_s=2e-5, leak_rate_hz=0.5, shot_rate_hz=400.0),
+    "shot_ideal_leak": dict(refractory_s=1e-5, leak_rate_hz=3.0, shot_rate_hz=150.0, sigma_theta_on=0.02),
 }
 
 
@@ -97,6 +101,7 @@ def test_state_roundtrip_and_slicing_invariance():
     W, H = 32, 16
     rng = np.random.default_rng(9)
     p = _params("physical_leak")
+    p.shot_rate_hz = 300.0
     a = bsx.SensorModel(p, W, H, 4, device="cuda")
     b = bsx.SensorModel(p, W, H, 4, device="cuda")
     ref = O.OracleModel(p.c, W, H, 4, 1, 0)
@@ -111,6 +116,9 @@ def test_state_roundtrip_and_slicing_invariance():
     os_ = ref.states()
     for f in ("s3_ref", "dr_ref", "i_scale", "t_ref", "t_search", "blind_until", "flags"):
         assert np.array_equal(st[f], os_[f]), f
+    # the shot-noise process's next arrival and index ride in the cell / nk[0] fields
+    assert np.array_equal(st["cell"], os_["shot_next"])
+    assert np.array_equal(st["nk"][:, 0], os_["shot_index"].astype(np.float64))
     b.load_state(blob)
     ids2, ts2, ir2, offs2 = random_runs(rng, W * H, 2 ** 23, 2 ** 24)
     ea = a.advance(ids2, ts2, ir2, 2 ** 24, offsets=offs2)
