@@ -405,6 +405,19 @@ struct InstantLane {
     return x_lo + base + dl_a - 16.0f * eps * (1.0f + fabs_(x_lo) + fabs_(s3_ref) + fabs_(dl_a)) <= -theta_off ? 2 : 0;
   }
 
+  // Would the walk, reaching the start of the current segment [ta, tb) in the state the lane holds
+  // (the call's input state, with no event, release or arrival before it), finish the segment on its
+  // prepared values without an event or an exact search? Exactly the conditions under which
+  // piece_fast decides the segment's one piece as "no crossing": no release or arrival pending
+  // before tb, no dead time past ta, no 2^23 cut, and fast_search = 0. Such a segment only
+  // advances t_search to tb (and counts one piece, and one root node when tb - ta > 2).
+  BSX_HD BSX_INLINE bool quiet_segment(const Prep& q) const {
+    if (blind_until != kNotBlind || shot_next < tb || t_search > ta) return false;
+    if (((ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs < tb) return false;
+    Hit h{false, false, ta, 0.0f};
+    return fast_search(q, h) == 0;
+  }
+
   // The fast part of one piece [t, e) of the current segment (the body of advance_instant's piece
   // loop up to its exact search). With the segment's prepared values (q non-null), a search that
   // starts at the segment's start and spans all of it is decided on them. Returns true when the
@@ -487,16 +500,28 @@ struct InstantLane {
   bool has_prep, seg_open, in_piece;
   int64_t t_cur, stop, t_end_, s_lo, e_cur;
 
-  BSX_HD BSX_INLINE void begin(int64_t t_end) {
+  // first_loud: the index of the first input sample ending a segment that is not quiet_segment
+  // under the input state (n: none), n_long the quiet segments before it with tb - ta > 2. The walk
+  // starts at that segment, with the quiet ones before it accounted as the walk would leave them.
+  BSX_HD BSX_INLINE void begin(int64_t t_end, int first_loud = 0, uint32_t n_long = 0) {
     t_end_ = t_end;
     next_i = skip;
     ta = t_state;
     for (int g = 0; g < kMaxG; ++g)
       if (g < G) va[g] = held[g];
     if (skip) load_sample(0, ta, va);
+    if (first_loud > skip) {
+      const uint32_t quiet = static_cast<uint32_t>(first_loud - skip);
+      n_substeps += quiet;
+      n_pieces += quiet;
+      n_leaves += n_long;
+      next_i = first_loud;
+      load_sample(first_loud - 1, ta, va);
+      t_search = vmax(t_search, ta);
+    }
     nt = kNoBreakpoint;
     if (next_i < n) load_sample(next_i, nt, nv);
-    t_cur = t_state;
+    t_cur = ta;
     q_next = Prep{};
     if (prep != nullptr && next_i < n) q_next = prep[next_i];
     seg_open = false;

@@ -11,15 +11,26 @@ namespace bsx {
 hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, const float* irr0, int64_t t0,
                        unsigned long long* counters, hipStream_t stream);
 // Phase A: the prepared values of every segment of every run (pos: the exclusive scan of the run
-// lengths, total = pos[n_runs]); phase B: the per-pixel walk (prep null: exact searches only).
+// lengths, total = pos[n_runs]), each run's estimated walk cost (work, added to), its first segment
+// that is not quiet under the input state (first_loud) and the long quiet ones before it (n_long);
+// phase B: the per-pixel walk (prep null: exact searches only; first_loud null: from the start).
 hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
                                const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
-                               int64_t t_state, InstantPrep* out, float* work, hipStream_t stream);
+                               int64_t t_state, InstantPrep* out, float* work, int* first_loud, uint32_t* n_long,
+                               hipStream_t stream);
 // order (null: pixel order) lists the pixels in the order the lanes take them.
 hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
                           const RunsDev& runs, const int64_t* pos, const InstantPrep* prep, int64_t t_state,
                           int64_t t_end, RawEventDev* ev, unsigned long long cap, unsigned long long* counters,
-                          const uint32_t* order, int block, hipStream_t stream);
+                          const uint32_t* order, const int* first_loud, const uint32_t* n_long, int block,
+                          hipStream_t stream);
+// The cascade without noise (the run costs for its order: launch_run_cost, zero-filling work first).
+hipError_t launch_run_cost(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
+                           const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total, float* work,
+                           hipStream_t stream);
+hipError_t launch_cascade(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
+                          const RunsDev& runs, int64_t t_state, int64_t t_end, RawEventDev* ev, unsigned long long cap,
+                          unsigned long long* counters, const uint32_t* order, int block, hipStream_t stream);
 // Heavy-first pixel order from the runs' estimated costs (work[run], zero-filled before phase A
 // adds to it): key (n_pixels bytes) and hist (128 words) are scratch, perm receives the order.
 hipError_t launch_cost_order(const int32_t* row_of, const float* work, uint32_t n_pixels, uint8_t* key,

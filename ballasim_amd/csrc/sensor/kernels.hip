@@ -4,6 +4,7 @@
 // sink (one atomic per wave and push site, consecutive slots for the active lanes).
 #include <hip/hip_runtime.h>
 
+#include "cascade.h"
 #include "kernels.h"
 #include "instant.h"
 
@@ -111,22 +112,52 @@ __device__ __forceinline__ void wave_segmented_add(float* dst, int key, float v)
 // kSearchWeight per threshold's worth of log-intensity change (where the searches and events are).
 constexpr float kSearchWeight = 32.0f;
 
+// Minimum of v over the lanes of the wave that share `key` (keys non-decreasing along the lanes),
+// folded once per key into dst[key] by the key's last lane (key < 0: no contribution).
+__device__ __forceinline__ void wave_segmented_min(int* dst, int key, int v) {
+  const int lane = __lane_id();
+  for (int off = 1; off < 64; off <<= 1) {
+    const int vu = __shfl_up(v, off);
+    const int ku = __shfl_up(key, off);
+    if (lane >= off && ku == key) v = vu < v ? vu : v;
+  }
+  const int kn = __shfl_down(key, 1);
+  if (key >= 0 && (lane == 63 || kn != key) && v != INT32_MAX) atomicMin(dst + key, v);
+}
+
+__device__ __forceinline__ void wave_segmented_count(uint32_t* dst, int key, uint32_t v) {
+  const int lane = __lane_id();
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t vu = __shfl_up(v, off);
+    const int ku = __shfl_up(key, off);
+    if (lane >= off && ku == key) v += vu;
+  }
+  const int kn = __shfl_down(key, 1);
+  if (key >= 0 && (lane == 63 || kn != key) && v != 0) atomicAdd(dst + key, v);
+}
+
+// The run i and segment j of position q (pos[i] <= q < pos[i + 1]).
+__device__ __forceinline__ int64_t run_of_position(const int64_t* pos, int64_t n_runs, int64_t q) {
+  int64_t lo = 0, hi = n_runs;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (pos[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 template <int kMaxG>
 __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp, StateDev in, RunsDev runs,
                                                            const uint32_t* pixel, const int64_t* pos, int64_t n_runs,
                                                            int64_t total, int64_t t_state, InstantPrep* out,
-                                                           float* work) {
+                                                           float* work, int* first_loud) {
   const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int run = -1;
   float cost = 0.0f;
+  int loud = INT32_MAX;
   if (q < total) {
-    int64_t lo = 0, hi = n_runs;  // the run i with pos[i] <= q < pos[i + 1]
-    while (hi - lo > 1) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (pos[mid] <= q) lo = mid;
-      else hi = mid;
-    }
-    const int64_t i = lo, j = q - pos[i];
+    const int64_t i = run_of_position(pos, n_runs, q), j = q - pos[i];
     run = static_cast<int>(i);
     const uint32_t p = pixel[i];
     const int64_t s = runs.row_start[i];
@@ -136,7 +167,16 @@ __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp
     L.dark = mp.dark[p];
     L.scale = mp.scale[p];
     L.leak = mp.leak[p];
+    L.theta_on = mp.theta_on[p];
+    L.theta_off = mp.theta_off[p];
     L.i_scale = in.i_scale[p];
+    L.s3_ref = in.s3_ref[p];
+    L.n3_ref = in.n3_ref[p];
+    L.dr_ref = in.dr_ref[p];
+    L.t_ref = in.t_ref[p];
+    L.blind_until = in.blind_until[p];
+    L.t_search = in.t_search[p];
+    L.shot_next = in.shot_next[p];
     const int64_t* t = runs.t + s;
     const float* v = runs.irr + s * kc.groups;
     bool empty = false;
@@ -160,11 +200,34 @@ __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp
         if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
       const InstantPrep r = L.prep_segment();
       out[q] = r;
-      const float theta = vmin(mp.theta_on[p], mp.theta_off[p]);
-      cost = 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) / theta;
+      const float theta = vmin(L.theta_on, L.theta_off);
+      if (!L.quiet_segment(r)) {
+        loud = static_cast<int>(j);
+        cost = 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) / theta;
+      }
     }
   }
   if (work != nullptr) wave_segmented_add(work, run, cost);
+  if (first_loud != nullptr) wave_segmented_min(first_loud, run, loud);
+}
+
+// The quiet segments before each run's first loud one with tb - ta > 2 (their root nodes).
+__global__ __launch_bounds__(256) void quiet_long_kernel(RunsDev runs, const int64_t* pos, int64_t n_runs,
+                                                         int64_t total, int64_t t_state, const int* first_loud,
+                                                         uint32_t* n_long) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int run = -1;
+  uint32_t c = 0;
+  if (q < total) {
+    const int64_t i = run_of_position(pos, n_runs, q), j = q - pos[i];
+    run = static_cast<int>(i);
+    if (j < first_loud[i]) {
+      const int64_t* t = runs.t + runs.row_start[i];
+      const int64_t ta = j == 0 ? t_state : t[j - 1];
+      c = t[j] - ta > 2 ? 1u : 0u;
+    }
+  }
+  wave_segmented_count(n_long, run, c);
 }
 
 // The pixel order of a walk: pixels by estimated cost, heaviest first (64 logarithmic buckets, bucket
@@ -224,7 +287,8 @@ template <int kMaxG>
 __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsDev mp, StateDev in, StateDev out, RunsDev runs,
                                                       const int64_t* pos, const InstantPrep* prep, int64_t t_state,
                                                       int64_t t_end, RawEventDev* ev, unsigned long long cap,
-                                                      unsigned long long* counters, const uint32_t* order) {
+                                                      unsigned long long* counters, const uint32_t* order,
+                                                      const int* first_loud, const uint32_t* n_long) {
   // The lanes' bisection stacks, one column per lane (conflict-free: a level is a row of the block;
   // blockDim.x <= kInstantBlock by the launch bound).
   __shared__ uint32_t stacks[InstantLane<kMaxG>::kSearchDepth * kInstantBlock];
@@ -248,12 +312,18 @@ __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsD
     L.ideal_reset = kc.ideal_reset != 0;
     const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
     L.prep = nullptr;
+    int loud_start = 0;
+    uint32_t loud_long = 0;
     if (row >= 0) {
       const int64_t start = runs.row_start[row];
       L.ts = runs.t + start;
       L.irr = runs.irr + start * kc.groups;
       L.n = runs.row_count[row];
       if (prep != nullptr) L.prep = prep + pos[row];
+      if (first_loud != nullptr) {
+        loud_start = vmin(first_loud[row], L.n);
+        loud_long = n_long[row];
+      }
     } else {
       L.ts = nullptr;
       L.irr = nullptr;
@@ -276,7 +346,7 @@ __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsD
     L.camera = kc.camera;
     L.shot_next = in.shot_next[p];
     L.shot_index = in.shot_index[p];
-    L.begin(t_end);
+    L.begin(t_end, loud_start, loud_long);
   }
   // The wave's lanes walk their pixels; a lane that reaches an exact search waits until enough lanes
   // wait (or none walks), and then all waiting lanes search together: the rare deep bisections of
@@ -318,6 +388,116 @@ __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsD
   wave_add(counters + 1 + kStSubsteps, L.n_substeps);
   wave_add(counters + 1 + kStPieces, L.n_pieces);
   wave_add(counters + 1 + kStLeaves, L.n_leaves);
+}
+
+// Estimated cost of each run's cascade walk for the heavy-first pixel order: one unit per segment
+// plus kSearchWeight per threshold's worth of log-intensity change (fast logs: a schedule only).
+__global__ __launch_bounds__(256) void run_cost_kernel(KConst kc, MapsDev mp, StateDev in, RunsDev runs,
+                                                       const uint32_t* pixel, const int64_t* pos, int64_t n_runs,
+                                                       int64_t total, float* work) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int run = -1;
+  float cost = 0.0f;
+  if (q < total) {
+    const int64_t i = run_of_position(pos, n_runs, q), j = q - pos[i];
+    run = static_cast<int>(i);
+    const uint32_t p = pixel[i];
+    const float* v = runs.irr + runs.row_start[i] * kc.groups;
+    float Ia = mp.dark[p], Ib = mp.dark[p];
+    for (int g = 0; g < kc.groups; ++g) {
+      Ia += mp.scale[p] * (j == 0 ? in.held[static_cast<int64_t>(g) * kc.n_pixels + p] : v[(j - 1) * kc.groups + g]);
+      Ib += mp.scale[p] * v[j * kc.groups + g];
+    }
+    const float theta = vmin(mp.theta_on[p], mp.theta_off[p]);
+    cost = 1.0f + kSearchWeight * fabs_(__logf(Ib) - __logf(Ia)) / theta;
+  }
+  wave_segmented_add(work, run, cost);
+}
+
+// The cascade without noise: one lane per pixel, in the heavy-first order (null: pixel order).
+template <int kMaxG>
+__global__ __launch_bounds__(256) void cascade_kernel(KConst kc, MapsDev mp, StateDev in, StateDev out, RunsDev runs,
+                                                      int64_t t_state, int64_t t_end, RawEventDev* ev,
+                                                      unsigned long long cap, unsigned long long* counters,
+                                                      const uint32_t* order) {
+  const int lane_index = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = order != nullptr && lane_index < kc.n_pixels ? static_cast<int>(order[lane_index]) : lane_index;
+  CascadeLane<kMaxG> L;
+  L.n_substeps = L.n_pieces = L.n_leaves = 0;
+  L.n_attempts = L.n_settled = L.n_fallbacks = L.n_quarantines = 0;
+  WaveSink sink{ev, cap, counters};
+  if (p < kc.n_pixels) {
+    L.pixel = static_cast<uint32_t>(p);
+    L.theta_on = mp.theta_on[p];
+    L.theta_off = mp.theta_off[p];
+    L.leak = mp.leak[p];
+    L.dark = mp.dark[p];
+    L.scale = mp.scale[p];
+    L.irr_max = kc.irr_max;
+    L.eps_ci = kc.eps_ci;
+    L.label_cut = kc.label_cut;
+    L.refractory_ns = kc.refractory_ns;
+    L.G = kc.groups;
+    L.ideal_reset = kc.ideal_reset != 0;
+    L.cc = cascade::Consts{kc.k, kc.a2, kc.a3, kc.tau_o, kc.tau_sf, kc.eps_accept, kc.eps_drop};
+    const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
+    L.prep = nullptr;
+    if (row >= 0) {
+      const int64_t start = runs.row_start[row];
+      L.ts = runs.t + start;
+      L.irr = runs.irr + start * kc.groups;
+      L.n = runs.row_count[row];
+    } else {
+      L.ts = nullptr;
+      L.irr = nullptr;
+      L.n = 0;
+    }
+    L.t_state = t_state;
+    L.skip = L.n > 0 && L.ts[0] == t_state ? 1 : 0;
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) L.held[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
+    L.i_scale = in.i_scale[p];
+    L.z = in.z[p];
+    L.x1 = in.x1[p];
+    L.s2 = in.s2[p];
+    L.s3 = in.s3[p];
+    L.tv = in.tv[p];
+    L.s3_ref = in.s3_ref[p];
+    L.n3_ref = in.n3_ref[p];
+    L.dr_ref = in.dr_ref[p];
+    L.t_ref = in.t_ref[p];
+    L.blind_until = in.blind_until[p];
+    L.t_search = in.t_search[p];
+    L.flags = in.flags[p];
+    L.shot_rate = 0.0;
+    L.shot_next = INT64_MAX;
+    L.shot_index = 0;
+    L.advance(t_end, sink);
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) out.held[static_cast<int64_t>(g) * kc.n_pixels + p] = L.held[g];
+    out.z[p] = L.z;
+    out.x1[p] = L.x1;
+    out.s2[p] = L.s2;
+    out.s3[p] = L.s3;
+    out.tv[p] = L.tv;
+    out.i_scale[p] = L.i_scale;
+    out.s3_ref[p] = L.s3_ref;
+    out.n3_ref[p] = L.n3_ref;
+    out.dr_ref[p] = L.dr_ref;
+    out.t_ref[p] = L.t_ref;
+    out.blind_until[p] = L.blind_until;
+    out.t_search[p] = L.t_search;
+    out.flags[p] = static_cast<uint16_t>(L.flags);
+    out.shot_next[p] = in.shot_next[p];
+    out.shot_index[p] = in.shot_index[p];
+  }
+  wave_add(counters + 1 + kStSubsteps, L.n_substeps);
+  wave_add(counters + 1 + kStAttempts, L.n_attempts);
+  wave_add(counters + 1 + kStSettled, L.n_settled);
+  wave_add(counters + 1 + kStLeaves, L.n_leaves);
+  wave_add(counters + 1 + kStFallbacks, L.n_fallbacks);
+  wave_add(counters + 1 + kStQuarantines, L.n_quarantines);
+  wave_add(counters + 1 + kStPieces, L.n_pieces);
 }
 
 // row_of[pixel[i]] = i, flagging out-of-range ids (flags[0]), duplicates (flags[1]) and runs outside
@@ -410,32 +590,66 @@ hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, 
 
 hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
                                const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
-                               int64_t t_state, InstantPrep* out, float* work, hipStream_t stream) {
-  if (total <= 0 || n_runs <= 0) return hipSuccess;
+                               int64_t t_state, InstantPrep* out, float* work, int* first_loud, uint32_t* n_long,
+                               hipStream_t stream) {
+  if (n_runs <= 0) return hipSuccess;
+  fill_i32_kernel<<<grid_for(static_cast<size_t>(n_runs), 256), 256, 0, stream>>>(first_loud, n_runs, INT32_MAX);
+  fill_i32_kernel<<<grid_for(static_cast<size_t>(n_runs), 256), 256, 0, stream>>>(reinterpret_cast<int32_t*>(n_long),
+                                                                                  n_runs, 0);
+  if (total <= 0) return hipGetLastError();
   const dim3 grid = grid_for(static_cast<size_t>(total), 256);
-#define BSX_PREP(G) \
-  instant_prep_kernel<G><<<grid, 256, 0, stream>>>(kc, mp, in, runs, pixel, pos, n_runs, total, t_state, out, work)
+#define BSX_PREP(G)                                                                                         \
+  instant_prep_kernel<G><<<grid, 256, 0, stream>>>(kc, mp, in, runs, pixel, pos, n_runs, total, t_state, out, work, \
+                                                   first_loud)
   if (kc.groups <= 1) BSX_PREP(1);
   else if (kc.groups <= 2) BSX_PREP(2);
   else if (kc.groups <= 4) BSX_PREP(4);
   else BSX_PREP(8);
 #undef BSX_PREP
+  quiet_long_kernel<<<grid, 256, 0, stream>>>(runs, pos, n_runs, total, t_state, first_loud, n_long);
   return hipGetLastError();
 }
 
 hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
                           const RunsDev& runs, const int64_t* pos, const InstantPrep* prep, int64_t t_state,
                           int64_t t_end, RawEventDev* ev, unsigned long long cap, unsigned long long* counters,
-                          const uint32_t* order, int block, hipStream_t stream) {
+                          const uint32_t* order, const int* first_loud, const uint32_t* n_long, int block,
+                          hipStream_t stream) {
   const dim3 grid = grid_for(kc.n_pixels, block);
 #define BSX_ADV(G)                                                                                              \
   instant_kernel<G><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, pos, prep, t_state, t_end, ev, cap, counters, \
-                                                order)
+                                                order, first_loud, n_long)
   if (kc.groups <= 1) BSX_ADV(1);
   else if (kc.groups <= 2) BSX_ADV(2);
   else if (kc.groups <= 4) BSX_ADV(4);
   else BSX_ADV(8);
 #undef BSX_ADV
+  return hipGetLastError();
+}
+
+hipError_t launch_run_cost(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
+                           const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total, float* work,
+                           hipStream_t stream) {
+  if (n_runs <= 0) return hipSuccess;
+  fill_i32_kernel<<<grid_for(static_cast<size_t>(n_runs), 256), 256, 0, stream>>>(reinterpret_cast<int32_t*>(work),
+                                                                                  n_runs, 0);
+  if (total > 0)
+    run_cost_kernel<<<grid_for(static_cast<size_t>(total), 256), 256, 0, stream>>>(kc, mp, in, runs, pixel, pos,
+                                                                                   n_runs, total, work);
+  return hipGetLastError();
+}
+
+hipError_t launch_cascade(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
+                          const RunsDev& runs, int64_t t_state, int64_t t_end, RawEventDev* ev, unsigned long long cap,
+                          unsigned long long* counters, const uint32_t* order, int block, hipStream_t stream) {
+  const dim3 grid = grid_for(kc.n_pixels, block);
+#define BSX_CAS(G) \
+  cascade_kernel<G><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters, order)
+  if (kc.groups <= 1) BSX_CAS(1);
+  else if (kc.groups <= 2) BSX_CAS(2);
+  else if (kc.groups <= 4) BSX_CAS(4);
+  else BSX_CAS(8);
+#undef BSX_CAS
   return hipGetLastError();
 }
 

@@ -349,9 +349,7 @@ RunsDev SensorModel::runs_view() const {
 
 void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t t_end,
                                hipStream_t stream) {
-  if (params_.photoreceptor != BSX_PHOTORECEPTOR_INSTANTANEOUS) {
-    throw std::logic_error("SensorModel: the cascade photoreceptor is not built into this backend yet");
-  }
+  const bool instant = params_.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS;
   if (cap_ == 0) cap_ = std::max<unsigned long long>(1ull << 20, 2 * P_);
   d_events_.ensure(cap_ * sizeof(RawEventDev));
   const KConst kc = kconst();
@@ -368,8 +366,10 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
               "read total");
     hip_check(hipStreamSynchronize(stream), "scan runs");
   }
-  d_prep_.ensure(std::max<int64_t>(total, 1) * sizeof(InstantPrep));
+  if (instant) d_prep_.ensure(std::max<int64_t>(total, 1) * sizeof(InstantPrep));
   d_work_.ensure(std::max<size_t>(n_runs, 1) * sizeof(float));
+  d_loud_.ensure(std::max<size_t>(n_runs, 1) * sizeof(int));
+  d_long_.ensure(std::max<size_t>(n_runs, 1) * sizeof(uint32_t));
   d_key_.ensure(P_);
   d_perm_.ensure(P_ * sizeof(uint32_t));
   d_order_hist_.ensure(128 * sizeof(uint32_t));
@@ -380,20 +380,34 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
     hip_check(hipEventRecord(ev0_, stream), "event record");
     if (!prepared) {  // phase A and the pixel order read only the call's input: a re-run reuses them
       hip_check(launch_fill_f32(d_work_.as<float>(), n_runs, 0.0f, stream), "fill work");
-      hip_check(launch_instant_prep(kc, mp, state_view(cur_), runs, pixel, d_pos_.as<const int64_t>(),
-                                    static_cast<int64_t>(n_runs), total, t_state_, d_prep_.as<InstantPrep>(),
-                                    d_work_.as<float>(), stream),
-                "instant prep kernel");
+      if (instant) {
+        hip_check(launch_instant_prep(kc, mp, state_view(cur_), runs, pixel, d_pos_.as<const int64_t>(),
+                                      static_cast<int64_t>(n_runs), total, t_state_, d_prep_.as<InstantPrep>(),
+                                      d_work_.as<float>(), d_loud_.as<int>(), d_long_.as<uint32_t>(), stream),
+                  "instant prep kernel");
+      } else {
+        hip_check(launch_run_cost(kc, mp, state_view(cur_), runs, pixel, d_pos_.as<const int64_t>(),
+                                  static_cast<int64_t>(n_runs), total, d_work_.as<float>(), stream),
+                  "run cost kernel");
+      }
       hip_check(launch_cost_order(runs.row_of, d_work_.as<const float>(), static_cast<uint32_t>(P_),
                                   d_key_.as<uint8_t>(), d_order_hist_.as<uint32_t>(), d_perm_.as<uint32_t>(), stream),
                 "pixel order");
       prepared = true;
     }
-    hip_check(launch_instant(kc, mp, state_view(cur_), state_view(1 - cur_), runs, d_pos_.as<const int64_t>(),
-                             total > 0 ? d_prep_.as<const InstantPrep>() : nullptr, t_state_, t_end,
-                             d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(),
-                             d_perm_.as<const uint32_t>(), block_, stream),
-              "instant kernel");
+    if (instant) {
+      hip_check(launch_instant(kc, mp, state_view(cur_), state_view(1 - cur_), runs, d_pos_.as<const int64_t>(),
+                               total > 0 ? d_prep_.as<const InstantPrep>() : nullptr, t_state_, t_end,
+                               d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(),
+                               d_perm_.as<const uint32_t>(), n_runs > 0 ? d_loud_.as<const int>() : nullptr,
+                               d_long_.as<const uint32_t>(), block_, stream),
+                "instant kernel");
+    } else {
+      hip_check(launch_cascade(kc, mp, state_view(cur_), state_view(1 - cur_), runs, t_state_, t_end,
+                               d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(),
+                               d_perm_.as<const uint32_t>(), block_, stream),
+                "cascade kernel");
+    }
     hip_check(hipEventRecord(ev1_, stream), "event record");
     hip_check(hipMemcpyAsync(c, d_counters_.get(), sizeof(c), hipMemcpyDeviceToHost, stream), "read counters");
     hip_check(hipStreamSynchronize(stream), "advance");

@@ -134,6 +134,7 @@ class SensorModel {
   DeviceBuffer d_t_, d_irr_, d_row_start_, d_row_count_, d_frame_, d_ids_;
   DeviceBuffer d_pos_, d_scan_, d_prep_;  // the runs' segment positions and the prepared segment values
   DeviceBuffer d_work_, d_key_, d_perm_, d_order_hist_;  // estimated run costs and the heavy-first pixel order
+  DeviceBuffer d_loud_, d_long_;  // per run: the first segment that is not quiet, the long quiet ones before it
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 
   int block_ = 256;
