@@ -411,11 +411,13 @@ struct InstantLane {
   // piece_fast decides the segment's one piece as "no crossing": no release or arrival pending
   // before tb, no dead time past ta, no 2^23 cut, and fast_search = 0. Such a segment only
   // advances t_search to tb (and counts one piece, and one root node when tb - ta > 2).
+  // (Evaluated without early exits: every condition is computed and combined at the end.)
   BSX_HD BSX_INLINE bool quiet_segment(const Prep& q) const {
-    if (blind_until != kNotBlind || shot_next < tb || t_search > ta) return false;
-    if (((ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs < tb) return false;
+    const bool eligible = (blind_until == kNotBlind) & (shot_next >= tb) & (t_search <= ta) &
+                          (((ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= tb);
     Hit h{false, false, ta, 0.0f};
-    return fast_search(q, h) == 0;
+    const int r = fast_search(q, h);
+    return eligible & (r == 0);
   }
 
   // The fast part of one piece [t, e) of the current segment (the body of advance_instant's piece

@@ -93,6 +93,36 @@ struct InstantPrep {
   float x_a, x_c, x_hi, x_lo;
 };
 
+struct RawEventDev;
+
+// One camera's view for a walk launch. A rig launches several cameras at once: their pixels form one
+// index space, camera c holding [base, base + kc.n_pixels).
+struct CameraDev {
+  KConst kc;
+  MapsDev mp;
+  StateDev in, out;
+  RunsDev runs;
+  const int64_t* pos;          // the runs' segment positions (phase A)
+  const InstantPrep* prep;     // phase A's values (null: exact searches only)
+  const int* first_loud;       // per run (null: every walk from its start)
+  const uint32_t* n_long;
+  const float* work;           // per run: estimated walk cost (null: no order information)
+  RawEventDev* ev;
+  unsigned long long cap;
+  unsigned long long* counters;  // [0] events, [1 + StatIdx] work counters
+  uint32_t base;
+};
+
+// The walk's launch shape, chosen by the autotuner: threads per block, segments a walking lane does
+// before the wave looks again, waiting lanes that start a joint search, 64-pixel groups a wave takes
+// per grab of the persistent queue (0: one group per wave, no queue).
+struct WalkTuning {
+  int32_t block;
+  int32_t walk_budget;
+  int32_t search_batch;
+  int32_t chunk;
+};
+
 struct RawEventDev {
   uint32_t pixel;
   uint8_t polarity;

@@ -18,12 +18,12 @@ hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateD
                                const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
                                int64_t t_state, InstantPrep* out, float* work, int* first_loud, uint32_t* n_long,
                                hipStream_t stream);
-// order (null: pixel order) lists the pixels in the order the lanes take them.
-hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
-                          const RunsDev& runs, const int64_t* pos, const InstantPrep* prep, int64_t t_state,
-                          int64_t t_end, RawEventDev* ev, unsigned long long cap, unsigned long long* counters,
-                          const uint32_t* order, const int* first_loud, const uint32_t* n_long, int block,
-                          hipStream_t stream);
+// The instantaneous walk over the pixels of n_cams cameras (descriptors in device memory), lane i
+// taking pixel order[i] of the joint pixel space (order null: index order); queue (one word) is the
+// persistent queue's counter when tu.chunk > 0.
+hipError_t launch_instant_walk(const CameraDev* cams, int n_cams, uint32_t n_total, int groups, const uint32_t* order,
+                               unsigned int* queue, const WalkTuning& tu, int64_t t_state, int64_t t_end,
+                               hipStream_t stream);
 // The cascade without noise (the run costs for its order: launch_run_cost, zero-filling work first).
 hipError_t launch_run_cost(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
                            const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total, float* work,
@@ -31,10 +31,11 @@ hipError_t launch_run_cost(const KConst& kc, const MapsDev& mp, const StateDev& 
 hipError_t launch_cascade(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
                           const RunsDev& runs, int64_t t_state, int64_t t_end, RawEventDev* ev, unsigned long long cap,
                           unsigned long long* counters, const uint32_t* order, int block, hipStream_t stream);
-// Heavy-first pixel order from the runs' estimated costs (work[run], zero-filled before phase A
-// adds to it): key (n_pixels bytes) and hist (128 words) are scratch, perm receives the order.
-hipError_t launch_cost_order(const int32_t* row_of, const float* work, uint32_t n_pixels, uint8_t* key,
-                             uint32_t* hist, uint32_t* perm, hipStream_t stream);
+// Heavy-first order of the joint pixel space of n_cams cameras from their runs' estimated costs
+// (CameraDev::work, filled by phase A or launch_run_cost): key (n_total bytes) and hist (128 words)
+// are scratch, perm receives the order.
+hipError_t launch_cost_order(const CameraDev* cams, int n_cams, uint32_t n_total, uint8_t* key, uint32_t* hist,
+                             uint32_t* perm, hipStream_t stream);
 hipError_t launch_fill_f32(float* dst, size_t n, float v, hipStream_t stream);
 // out[0] = pm::log monotonicity violations, out[1] = hash of its bits over every positive float.
 hipError_t launch_math_selfcheck(unsigned long long* out, hipStream_t stream);

@@ -201,10 +201,9 @@ __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp
       const InstantPrep r = L.prep_segment();
       out[q] = r;
       const float theta = vmin(L.theta_on, L.theta_off);
-      if (!L.quiet_segment(r)) {
-        loud = static_cast<int>(j);
-        cost = 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) / theta;
-      }
+      const bool quiet = L.quiet_segment(r);
+      loud = quiet ? INT32_MAX : static_cast<int>(j);
+      cost = quiet ? 0.0f : 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) / theta;
     }
   }
   if (work != nullptr) wave_segmented_add(work, run, cost);
@@ -239,16 +238,25 @@ __device__ __forceinline__ int cost_bucket(float w) {
   return b < 1 ? 1 : (b > 63 ? 63 : b);
 }
 
-__global__ __launch_bounds__(256) void order_count_kernel(const int32_t* row_of, const float* work, uint32_t n,
-                                                          uint8_t* key, uint32_t* hist) {
+// The camera of index g of a launch's pixel space (cameras in base order, a handful at most).
+__device__ __forceinline__ int camera_of(const CameraDev* cams, int n_cams, uint32_t g) {
+  int c = 0;
+  while (c + 1 < n_cams && g >= cams[c + 1].base) ++c;
+  return c;
+}
+
+__global__ __launch_bounds__(256) void order_count_kernel(const CameraDev* cams, int n_cams, uint32_t n, uint8_t* key,
+                                                          uint32_t* hist) {
   __shared__ uint32_t cnt[64];
   if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) {
-    const int32_t r = row_of != nullptr ? row_of[p] : -1;
-    const int b = r >= 0 ? cost_bucket(work[r]) : 0;
-    key[p] = static_cast<uint8_t>(b);
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) {
+    const CameraDev& c = cams[camera_of(cams, n_cams, g)];
+    const uint32_t p = g - c.base;
+    const int32_t r = c.runs.row_of != nullptr ? c.runs.row_of[p] : -1;
+    const int b = r >= 0 && c.work != nullptr ? cost_bucket(c.work[r]) : 0;
+    key[g] = static_cast<uint8_t>(b);
     atomicAdd(&cnt[b], 1u);
   }
   __syncthreads();
@@ -277,27 +285,73 @@ __global__ __launch_bounds__(256) void order_scatter_kernel(const uint8_t* key, 
   if (p < n) perm[base[b] + local] = p;
 }
 
-// Wave scheduling of the instantaneous walk: segments a walking lane does before the wave looks
-// again, and waiting lanes that start a joint search pass.
-constexpr int kWalkBudget = 8;
-constexpr int kSearchBatch = 8;
 constexpr int kInstantBlock = 256;
 
+// Events of lanes of several cameras: per camera, the lanes that push together reserve consecutive
+// slots of that camera's buffer with one atomic add (ballot, popcount rank, shuffle of the base);
+// slots past the capacity are counted, not written (the host re-runs from the untouched input).
+struct RigSink {
+  const CameraDev* cam;
+  int cam_index;
+  __device__ __forceinline__ void push(const RawEventDev& e) {
+    unsigned long long mask = __ballot(1);
+    const int lane = __lane_id();
+    while (mask != 0) {
+      const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+      const int c0 = __shfl(cam_index, leader);
+      const unsigned long long same = __ballot(cam_index == c0) & mask;
+      if (cam_index == c0) {
+        const unsigned rank = __popcll(same & ((1ull << lane) - 1ull));
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(cam->counters, static_cast<unsigned long long>(__popcll(same)));
+        base = __shfl(base, leader);
+        const unsigned long long slot = base + rank;
+        if (slot < cam->cap) cam->ev[slot] = e;
+      }
+      mask &= ~same;
+    }
+  }
+};
+
+// Adds each lane's v to counters[idx] of its camera: one atomic per camera present in the wave
+// (cam_index < 0: no contribution). Called by every lane of the wave.
+__device__ __forceinline__ void camera_add(const CameraDev* cams, int cam_index, int idx, uint32_t v) {
+  unsigned long long mask = __ballot(cam_index >= 0 && v != 0);
+  while (mask != 0) {
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    const int c0 = __shfl(cam_index, leader);
+    unsigned long long s = cam_index == c0 ? v : 0u;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (__lane_id() == leader) atomicAdd(cams[c0].counters + idx, s);
+    mask &= ~__ballot(cam_index == c0);
+  }
+}
+
+// One 64-pixel group of the walk: lane i of the launch's pixel order takes pixel order[i] (of its
+// camera), walks it to t_end and writes its state. The wave's lanes walk their pixels; a lane that
+// reaches an exact search waits until search_batch lanes wait (or none walks), and then all waiting
+// lanes search together, so the searches of different pixels run side by side.
 template <int kMaxG>
-__global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsDev mp, StateDev in, StateDev out, RunsDev runs,
-                                                      const int64_t* pos, const InstantPrep* prep, int64_t t_state,
-                                                      int64_t t_end, RawEventDev* ev, unsigned long long cap,
-                                                      unsigned long long* counters, const uint32_t* order,
-                                                      const int* first_loud, const uint32_t* n_long) {
-  // The lanes' bisection stacks, one column per lane (conflict-free: a level is a row of the block;
-  // blockDim.x <= kInstantBlock by the launch bound).
-  __shared__ uint32_t stacks[InstantLane<kMaxG>::kSearchDepth * kInstantBlock];
-  const int lane_index = blockIdx.x * blockDim.x + threadIdx.x;
-  const int p = order != nullptr && lane_index < kc.n_pixels ? static_cast<int>(order[lane_index]) : lane_index;
+__device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams, uint32_t n_total,
+                                              const uint32_t* order, uint32_t i, const WalkTuning& tu,
+                                              int64_t t_state, int64_t t_end, uint32_t* stacks) {
   InstantLane<kMaxG> L;
   L.n_substeps = L.n_pieces = L.n_leaves = 0;
-  WaveSink sink{ev, cap, counters};
-  if (p < kc.n_pixels) {
+  int ci = -1;
+  const CameraDev* cam = cams;
+  int p = 0;
+  if (i < n_total) {
+    const uint32_t g = order != nullptr ? order[i] : i;
+    ci = camera_of(cams, n_cams, g);
+    cam = cams + ci;
+    p = static_cast<int>(g - cam->base);
+  }
+  RigSink sink{cam, ci};
+  const KConst& kc = cam->kc;
+  if (ci >= 0) {
+    const MapsDev& mp = cam->mp;
+    const StateDev& in = cam->in;
+    const RunsDev& runs = cam->runs;
     L.pixel = static_cast<uint32_t>(p);
     L.theta_on = mp.theta_on[p];
     L.theta_off = mp.theta_off[p];
@@ -319,10 +373,10 @@ __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsD
       L.ts = runs.t + start;
       L.irr = runs.irr + start * kc.groups;
       L.n = runs.row_count[row];
-      if (prep != nullptr) L.prep = prep + pos[row];
-      if (first_loud != nullptr) {
-        loud_start = vmin(first_loud[row], L.n);
-        loud_long = n_long[row];
+      if (cam->prep != nullptr) L.prep = cam->prep + cam->pos[row];
+      if (cam->first_loud != nullptr) {
+        loud_start = vmin(cam->first_loud[row], L.n);
+        loud_long = cam->n_long[row];
       }
     } else {
       L.ts = nullptr;
@@ -348,24 +402,23 @@ __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsD
     L.shot_index = in.shot_index[p];
     L.begin(t_end, loud_start, loud_long);
   }
-  // The wave's lanes walk their pixels; a lane that reaches an exact search waits until enough lanes
-  // wait (or none walks), and then all waiting lanes search together: the rare deep bisections of
-  // different pixels run side by side instead of stalling the wave one at a time.
-  int mode = p < kc.n_pixels ? L.walk(sink, kWalkBudget) : InstantLane<kMaxG>::kDone;
+  int mode = ci >= 0 ? L.walk(sink, tu.walk_budget) : InstantLane<kMaxG>::kDone;
   while (true) {
     const unsigned long long searching = __ballot(mode == InstantLane<kMaxG>::kSearch);
     const unsigned long long walking = __ballot(mode == InstantLane<kMaxG>::kWalk);
     if ((searching | walking) == 0) break;
-    if (searching != 0 && (walking == 0 || __popcll(searching) >= kSearchBatch)) {
+    if (searching != 0 && (walking == 0 || __popcll(searching) >= tu.search_batch)) {
       if (mode == InstantLane<kMaxG>::kSearch) {
-        L.search_step(sink, stacks + threadIdx.x, blockDim.x);
+        L.search_step(sink, stacks, blockDim.x);
         mode = InstantLane<kMaxG>::kWalk;
       }
     } else if (mode == InstantLane<kMaxG>::kWalk) {
-      mode = L.walk(sink, kWalkBudget);
+      mode = L.walk(sink, tu.walk_budget);
     }
   }
-  if (p < kc.n_pixels) {
+  if (ci >= 0) {
+    const StateDev& in = cam->in;
+    const StateDev& out = cam->out;
     L.end();
     for (int g = 0; g < kMaxG; ++g)
       if (g < kc.groups) out.held[static_cast<int64_t>(g) * kc.n_pixels + p] = L.held[g];
@@ -385,9 +438,38 @@ __global__ __launch_bounds__(kInstantBlock) void instant_kernel(KConst kc, MapsD
     out.shot_next[p] = L.shot_next;
     out.shot_index[p] = L.shot_index;
   }
-  wave_add(counters + 1 + kStSubsteps, L.n_substeps);
-  wave_add(counters + 1 + kStPieces, L.n_pieces);
-  wave_add(counters + 1 + kStLeaves, L.n_leaves);
+  camera_add(cams, ci, 1 + kStSubsteps, L.n_substeps);
+  camera_add(cams, ci, 1 + kStPieces, L.n_pieces);
+  camera_add(cams, ci, 1 + kStLeaves, L.n_leaves);
+}
+
+// The instantaneous walk over the pixels of n_cams cameras. Without a queue (tu.chunk = 0) each wave
+// takes one 64-pixel group; with one, waves grab tu.chunk groups at a time from the queue counter
+// until the pixel space is exhausted (a persistent grid: every wave exits once a grab is empty).
+template <int kMaxG>
+__global__ __launch_bounds__(kInstantBlock) void instant_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
+                                                                   const uint32_t* order, unsigned int* queue,
+                                                                   WalkTuning tu, int64_t t_state, int64_t t_end) {
+  // The lanes' bisection stacks, one column per lane (conflict-free: a level is a row of the block;
+  // blockDim.x <= kInstantBlock by the launch bound).
+  __shared__ uint32_t stacks[InstantLane<kMaxG>::kSearchDepth * kInstantBlock];
+  const uint32_t lane = __lane_id();
+  const uint32_t n_groups = (n_total + 63) / 64;
+  if (tu.chunk <= 0 || queue == nullptr) {
+    const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    if (group < n_groups)
+      instant_group<kMaxG>(cams, n_cams, n_total, order, group * 64 + lane, tu, t_state, t_end, stacks + threadIdx.x);
+    return;
+  }
+  while (true) {
+    uint32_t g0 = 0;
+    if (lane == 0) g0 = atomicAdd(queue, static_cast<unsigned int>(tu.chunk));
+    g0 = __shfl(g0, 0);
+    if (g0 >= n_groups) break;
+    const uint32_t g1 = vmin(g0 + static_cast<uint32_t>(tu.chunk), n_groups);
+    for (uint32_t group = g0; group < g1; ++group)
+      instant_group<kMaxG>(cams, n_cams, n_total, order, group * 64 + lane, tu, t_state, t_end, stacks + threadIdx.x);
+  }
 }
 
 // Estimated cost of each run's cascade walk for the heavy-first pixel order: one unit per segment
@@ -610,20 +692,38 @@ hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateD
   return hipGetLastError();
 }
 
-hipError_t launch_instant(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
-                          const RunsDev& runs, const int64_t* pos, const InstantPrep* prep, int64_t t_state,
-                          int64_t t_end, RawEventDev* ev, unsigned long long cap, unsigned long long* counters,
-                          const uint32_t* order, const int* first_loud, const uint32_t* n_long, int block,
-                          hipStream_t stream) {
-  const dim3 grid = grid_for(kc.n_pixels, block);
-#define BSX_ADV(G)                                                                                              \
-  instant_kernel<G><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, pos, prep, t_state, t_end, ev, cap, counters, \
-                                                order, first_loud, n_long)
-  if (kc.groups <= 1) BSX_ADV(1);
-  else if (kc.groups <= 2) BSX_ADV(2);
-  else if (kc.groups <= 4) BSX_ADV(4);
-  else BSX_ADV(8);
-#undef BSX_ADV
+hipError_t launch_instant_walk(const CameraDev* cams, int n_cams, uint32_t n_total, int groups, const uint32_t* order,
+                               unsigned int* queue, const WalkTuning& tu, int64_t t_state, int64_t t_end,
+                               hipStream_t stream) {
+  if (n_total == 0) return hipSuccess;
+  const int block = tu.block;
+  dim3 grid = grid_for(n_total, block);
+  if (tu.chunk > 0 && queue != nullptr) {
+    // persistent: as many blocks as are resident at once (at most one per 64-pixel group set)
+    static int resident[4] = {0, 0, 0, 0};
+    const int gi = groups <= 1 ? 0 : groups <= 2 ? 1 : groups <= 4 ? 2 : 3;
+    if (resident[gi] == 0) {
+      int dev = 0, cus = 0, per_cu = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      hipError_t e = hipSuccess;
+      if (gi == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, instant_walk_kernel<1>, block, 0);
+      else if (gi == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, instant_walk_kernel<2>, block, 0);
+      else if (gi == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, instant_walk_kernel<4>, block, 0);
+      else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, instant_walk_kernel<8>, block, 0);
+      if (e != hipSuccess) return e;
+      resident[gi] = vmax(1, cus * vmax(1, per_cu) * 256 / block);
+    }
+    grid = dim3(vmin(grid.x, static_cast<unsigned>(resident[gi])));
+    fill_i32_kernel<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(queue), 1, 0);
+  }
+#define BSX_WALK(G) \
+  instant_walk_kernel<G><<<grid, block, 0, stream>>>(cams, n_cams, n_total, order, queue, tu, t_state, t_end)
+  if (groups <= 1) BSX_WALK(1);
+  else if (groups <= 2) BSX_WALK(2);
+  else if (groups <= 4) BSX_WALK(4);
+  else BSX_WALK(8);
+#undef BSX_WALK
   return hipGetLastError();
 }
 
@@ -653,14 +753,14 @@ hipError_t launch_cascade(const KConst& kc, const MapsDev& mp, const StateDev& i
   return hipGetLastError();
 }
 
-hipError_t launch_cost_order(const int32_t* row_of, const float* work, uint32_t n_pixels, uint8_t* key,
-                             uint32_t* hist, uint32_t* perm, hipStream_t stream) {
-  if (n_pixels == 0) return hipSuccess;
+hipError_t launch_cost_order(const CameraDev* cams, int n_cams, uint32_t n_total, uint8_t* key, uint32_t* hist,
+                             uint32_t* perm, hipStream_t stream) {
+  if (n_total == 0) return hipSuccess;
   const int block = 256;
   fill_i32_kernel<<<1, 128, 0, stream>>>(reinterpret_cast<int32_t*>(hist), 128, 0);
-  order_count_kernel<<<grid_for(n_pixels, block), block, 0, stream>>>(row_of, work, n_pixels, key, hist);
+  order_count_kernel<<<grid_for(n_total, block), block, 0, stream>>>(cams, n_cams, n_total, key, hist);
   order_offsets_kernel<<<1, 64, 0, stream>>>(hist, hist + 64);
-  order_scatter_kernel<<<grid_for(n_pixels, block), block, 0, stream>>>(key, n_pixels, hist + 64, perm);
+  order_scatter_kernel<<<grid_for(n_total, block), block, 0, stream>>>(key, n_total, hist + 64, perm);
   return hipGetLastError();
 }
 

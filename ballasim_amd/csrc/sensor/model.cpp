@@ -373,9 +373,31 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
   d_key_.ensure(P_);
   d_perm_.ensure(P_ * sizeof(uint32_t));
   d_order_hist_.ensure(128 * sizeof(uint32_t));
+  d_queue_.ensure(64);
+  // The walk's camera descriptor (the event buffer and its capacity are refreshed per attempt).
+  CameraDev cd{};
+  cd.kc = kc;
+  cd.mp = mp;
+  cd.in = state_view(cur_);
+  cd.out = state_view(1 - cur_);
+  cd.runs = runs;
+  cd.pos = d_pos_.as<const int64_t>();
+  cd.prep = instant && total > 0 ? d_prep_.as<const InstantPrep>() : nullptr;
+  cd.first_loud = instant && n_runs > 0 ? d_loud_.as<const int>() : nullptr;
+  cd.n_long = d_long_.as<const uint32_t>();
+  cd.work = n_runs > 0 ? d_work_.as<const float>() : nullptr;
+  cd.counters = d_counters_.as<unsigned long long>();
+  cd.base = 0;
+  d_cams_.ensure(sizeof(CameraDev));
+  const CameraDev* cams = d_cams_.as<const CameraDev>();
   unsigned long long c[1 + kStCount];
   bool prepared = false;
   while (true) {
+    cd.ev = d_events_.as<RawEventDev>();
+    cd.cap = cap_;
+    host_cam_ = cd;
+    hip_check(hipMemcpyAsync(d_cams_.get(), &host_cam_, sizeof(CameraDev), hipMemcpyHostToDevice, stream),
+              "upload camera");
     hip_check(launch_fill_u64(d_counters_.as<unsigned long long>(), 1 + kStCount, 0, stream), "fill counters");
     hip_check(hipEventRecord(ev0_, stream), "event record");
     if (!prepared) {  // phase A and the pixel order read only the call's input: a re-run reuses them
@@ -390,18 +412,17 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
                                   static_cast<int64_t>(n_runs), total, d_work_.as<float>(), stream),
                   "run cost kernel");
       }
-      hip_check(launch_cost_order(runs.row_of, d_work_.as<const float>(), static_cast<uint32_t>(P_),
-                                  d_key_.as<uint8_t>(), d_order_hist_.as<uint32_t>(), d_perm_.as<uint32_t>(), stream),
+      hip_check(launch_cost_order(cams, 1, static_cast<uint32_t>(P_), d_key_.as<uint8_t>(),
+                                  d_order_hist_.as<uint32_t>(), d_perm_.as<uint32_t>(), stream),
                 "pixel order");
       prepared = true;
     }
     if (instant) {
-      hip_check(launch_instant(kc, mp, state_view(cur_), state_view(1 - cur_), runs, d_pos_.as<const int64_t>(),
-                               total > 0 ? d_prep_.as<const InstantPrep>() : nullptr, t_state_, t_end,
-                               d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(),
-                               d_perm_.as<const uint32_t>(), n_runs > 0 ? d_loud_.as<const int>() : nullptr,
-                               d_long_.as<const uint32_t>(), block_, stream),
-                "instant kernel");
+      WalkTuning tu = tuning_;
+      tu.block = block_;
+      hip_check(launch_instant_walk(cams, 1, static_cast<uint32_t>(P_), G_, d_perm_.as<const uint32_t>(),
+                                    d_queue_.as<unsigned int>(), tu, t_state_, t_end, stream),
+                "instant walk");
     } else {
       hip_check(launch_cascade(kc, mp, state_view(cur_), state_view(1 - cur_), runs, t_state_, t_end,
                                d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(),
@@ -623,8 +644,14 @@ void SensorModel::set_policy(const bsx_launch_policy& p) {
   if (p.block_size != 0 && (p.block_size % 64 != 0 || p.block_size < 64 || p.block_size > 256)) {
     throw std::invalid_argument("set_launch_policy: block_size must be a multiple of 64 in [64, 256]");
   }
+  if (p.walk_budget < 0 || p.walk_budget > 1024) throw std::invalid_argument("set_launch_policy: walk_budget in [0, 1024]");
+  if (p.search_batch < 0 || p.search_batch > 64) throw std::invalid_argument("set_launch_policy: search_batch in [0, 64]");
+  if (p.queue_chunk < -1 || p.queue_chunk > 64) throw std::invalid_argument("set_launch_policy: queue_chunk in [-1, 64]");
   if (p.event_capacity != 0) cap_ = p.event_capacity;
   if (p.block_size != 0) block_ = p.block_size;
+  if (p.walk_budget != 0) tuning_.walk_budget = p.walk_budget;
+  if (p.search_batch != 0) tuning_.search_batch = p.search_batch;
+  if (p.queue_chunk != 0) tuning_.chunk = p.queue_chunk < 0 ? 0 : p.queue_chunk;
 }
 
 bsx_sensor_info_t SensorModel::info() const {

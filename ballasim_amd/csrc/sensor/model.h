@@ -135,6 +135,9 @@ class SensorModel {
   DeviceBuffer d_pos_, d_scan_, d_prep_;  // the runs' segment positions and the prepared segment values
   DeviceBuffer d_work_, d_key_, d_perm_, d_order_hist_;  // estimated run costs and the heavy-first pixel order
   DeviceBuffer d_loud_, d_long_;  // per run: the first segment that is not quiet, the long quiet ones before it
+  DeviceBuffer d_cams_, d_queue_;  // the walk's camera descriptor and the persistent queue's counter
+  CameraDev host_cam_{};           // the descriptor's host copy (source of its upload)
+  WalkTuning tuning_{256, 8, 8, 0};
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 
   int block_ = 256;

@@ -261,8 +261,8 @@ class SensorModel:
     def reset_stats(self):
         _abi.check(self._lib.bsx_sensor_reset_stats(self._h))
 
-    def set_launch_policy(self, event_capacity=0, block_size=0):
-        pol = _abi.LaunchPolicy(event_capacity, block_size, 0)
+    def set_launch_policy(self, event_capacity=0, block_size=0, walk_budget=0, search_batch=0, queue_chunk=0):
+        pol = _abi.LaunchPolicy(event_capacity, block_size, walk_budget, search_batch, queue_chunk)
         _abi.check(self._lib.bsx_sensor_set_launch_policy(self._h, C.byref(pol)))
 
     def _info(self):

@@ -81,7 +81,20 @@ def main():
     ap.add_argument("--photoreceptor", default="instantaneous", choices=["instantaneous", "cascade"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work (0: skip)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device-to-device copy (the reference kernel for rocprofv3's "
+                         "FETCH_SIZE / WRITE_SIZE scale) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28  # 2^28 fp32 = 1 GiB read + 1 GiB written per copy
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py runs on one GPU: the sensor path has no multi-GPU form here "
                                    "(pixels shard by tile or camera; see DESIGN.md)", "n_gpus": args.gpus}))

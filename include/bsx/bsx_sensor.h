@@ -143,11 +143,15 @@ typedef struct bsx_stats {
 } bsx_stats;
 
 /* Launch policy (reference: GpuLaunchPolicy, src/sensor/launch.h:16-27): the initial event
- * capacity (0 = automatic), which the model grows on overflow. */
+ * capacity (0 = automatic), which the model grows on overflow, and the walk's launch shape (0 in a
+ * field = keep the current value; the autotuner of a rig sets them). */
 typedef struct bsx_launch_policy {
   uint64_t event_capacity;
-  int32_t block_size; /* threads per workgroup, a multiple of 64 in [64, 256]; 0 = automatic (256) */
-  int32_t reserved;
+  int32_t block_size;   /* threads per workgroup, a multiple of 64 in [64, 256] (default 256) */
+  int32_t walk_budget;  /* segments a walking lane does before its wave re-plans, [1, 1024] (default 8) */
+  int32_t search_batch; /* waiting lanes that start a joint exact-search pass, [1, 64] (default 8) */
+  int32_t queue_chunk;  /* 64-pixel groups per grab of a persistent work queue, [0, 64]; 0 = no queue;
+                           -1 sets 0 (default 0) */
 } bsx_launch_policy;
 
 typedef struct bsx_sensor_info_t {
