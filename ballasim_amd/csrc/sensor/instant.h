@@ -503,9 +503,10 @@ struct InstantLane {
   int64_t t_cur, stop, t_end_, s_lo, e_cur;
 
   // first_loud: the index of the first input sample ending a segment that is not quiet_segment
-  // under the input state (n: none), n_long the quiet segments before it with tb - ta > 2. The walk
-  // starts at that segment, with the quiet ones before it accounted as the walk would leave them.
-  BSX_HD BSX_INLINE void begin(int64_t t_end, int first_loud = 0, uint32_t n_long = 0) {
+  // under the input state (n: none). The walk starts at that segment, with the quiet ones before it
+  // left as the walk would leave them (t_search, the piece counts; phase A's tests are not counted as
+  // leaf intervals).
+  BSX_HD BSX_INLINE void begin(int64_t t_end, int first_loud = 0) {
     t_end_ = t_end;
     next_i = skip;
     ta = t_state;
@@ -516,7 +517,6 @@ struct InstantLane {
       const uint32_t quiet = static_cast<uint32_t>(first_loud - skip);
       n_substeps += quiet;
       n_pieces += quiet;
-      n_leaves += n_long;
       next_i = first_loud;
       load_sample(first_loud - 1, ta, va);
       t_search = vmax(t_search, ta);

@@ -105,7 +105,6 @@ struct CameraDev {
   const int64_t* pos;          // the runs' segment positions (phase A)
   const InstantPrep* prep;     // phase A's values (null: exact searches only)
   const int* first_loud;       // per run (null: every walk from its start)
-  const uint32_t* n_long;
   const float* work;           // per run: estimated walk cost (null: no order information)
   RawEventDev* ev;
   unsigned long long cap;

@@ -11,13 +11,12 @@ namespace bsx {
 hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, const float* irr0, int64_t t0,
                        unsigned long long* counters, hipStream_t stream);
 // Phase A: the prepared values of every segment of every run (pos: the exclusive scan of the run
-// lengths, total = pos[n_runs]), each run's estimated walk cost (work, added to), its first segment
-// that is not quiet under the input state (first_loud) and the long quiet ones before it (n_long);
+// lengths, total = pos[n_runs]), each run's estimated walk cost (work, added to) and its first segment
+// that is not quiet under the input state (first_loud);
 // phase B: the per-pixel walk (prep null: exact searches only; first_loud null: from the start).
 hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
                                const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
-                               int64_t t_state, InstantPrep* out, float* work, int* first_loud, uint32_t* n_long,
-                               hipStream_t stream);
+                               int64_t t_state, InstantPrep* out, float* work, int* first_loud, hipStream_t stream);
 // The instantaneous walk over the pixels of n_cams cameras (descriptors in device memory), lane i
 // taking pixel order[i] of the joint pixel space (order null: index order); queue (one word) is the
 // persistent queue's counter when tu.chunk > 0.

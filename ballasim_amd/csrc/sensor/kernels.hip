@@ -136,7 +136,7 @@ __device__ __forceinline__ void wave_segmented_count(uint32_t* dst, int key, uin
   if (key >= 0 && (lane == 63 || kn != key) && v != 0) atomicAdd(dst + key, v);
 }
 
-// The run i and segment j of position q (pos[i] <= q < pos[i + 1]).
+// The run i of position q (the last i with pos[i] <= q; pos non-decreasing), by bisection.
 __device__ __forceinline__ int64_t run_of_position(const int64_t* pos, int64_t n_runs, int64_t q) {
   int64_t lo = 0, hi = n_runs;
   while (hi - lo > 1) {
@@ -147,17 +147,32 @@ __device__ __forceinline__ int64_t run_of_position(const int64_t* pos, int64_t n
   return lo;
 }
 
+// The same for the block's threads at once (q = the block's first position + threadIdx.x): one
+// bisection for the block's first position, then a short forward walk per thread (a block's 256
+// consecutive positions span few runs). Called by every thread of the block.
+__device__ __forceinline__ int64_t block_run_of_position(const int64_t* pos, int64_t n_runs, int64_t q0,
+                                                         int64_t q, int64_t total) {
+  __shared__ int64_t first;
+  if (threadIdx.x == 0) first = q0 < total ? run_of_position(pos, n_runs, q0) : 0;
+  __syncthreads();
+  int64_t i = first;
+  if (q < total)
+    while (i + 1 < n_runs && pos[i + 1] <= q) ++i;
+  return i;
+}
+
 template <int kMaxG>
 __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp, StateDev in, RunsDev runs,
                                                            const uint32_t* pixel, const int64_t* pos, int64_t n_runs,
                                                            int64_t total, int64_t t_state, InstantPrep* out,
                                                            float* work, int* first_loud) {
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * blockDim.x, q = q0 + threadIdx.x;
+  const int64_t i_q = block_run_of_position(pos, n_runs, q0, q, total);
   int run = -1;
   float cost = 0.0f;
   int loud = INT32_MAX;
   if (q < total) {
-    const int64_t i = run_of_position(pos, n_runs, q), j = q - pos[i];
+    const int64_t i = i_q, j = q - pos[i];
     run = static_cast<int>(i);
     const uint32_t p = pixel[i];
     const int64_t s = runs.row_start[i];
@@ -208,25 +223,6 @@ __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp
   }
   if (work != nullptr) wave_segmented_add(work, run, cost);
   if (first_loud != nullptr) wave_segmented_min(first_loud, run, loud);
-}
-
-// The quiet segments before each run's first loud one with tb - ta > 2 (their root nodes).
-__global__ __launch_bounds__(256) void quiet_long_kernel(RunsDev runs, const int64_t* pos, int64_t n_runs,
-                                                         int64_t total, int64_t t_state, const int* first_loud,
-                                                         uint32_t* n_long) {
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  int run = -1;
-  uint32_t c = 0;
-  if (q < total) {
-    const int64_t i = run_of_position(pos, n_runs, q), j = q - pos[i];
-    run = static_cast<int>(i);
-    if (j < first_loud[i]) {
-      const int64_t* t = runs.t + runs.row_start[i];
-      const int64_t ta = j == 0 ? t_state : t[j - 1];
-      c = t[j] - ta > 2 ? 1u : 0u;
-    }
-  }
-  wave_segmented_count(n_long, run, c);
 }
 
 // The pixel order of a walk: pixels by estimated cost, heaviest first (64 logarithmic buckets, bucket
@@ -367,17 +363,13 @@ __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams,
     const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
     L.prep = nullptr;
     int loud_start = 0;
-    uint32_t loud_long = 0;
     if (row >= 0) {
       const int64_t start = runs.row_start[row];
       L.ts = runs.t + start;
       L.irr = runs.irr + start * kc.groups;
       L.n = runs.row_count[row];
       if (cam->prep != nullptr) L.prep = cam->prep + cam->pos[row];
-      if (cam->first_loud != nullptr) {
-        loud_start = vmin(cam->first_loud[row], L.n);
-        loud_long = cam->n_long[row];
-      }
+      if (cam->first_loud != nullptr) loud_start = vmin(cam->first_loud[row], L.n);
     } else {
       L.ts = nullptr;
       L.irr = nullptr;
@@ -400,7 +392,7 @@ __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams,
     L.camera = kc.camera;
     L.shot_next = in.shot_next[p];
     L.shot_index = in.shot_index[p];
-    L.begin(t_end, loud_start, loud_long);
+    L.begin(t_end, loud_start);
   }
   int mode = ci >= 0 ? L.walk(sink, tu.walk_budget) : InstantLane<kMaxG>::kDone;
   while (true) {
@@ -477,11 +469,12 @@ __global__ __launch_bounds__(kInstantBlock) void instant_walk_kernel(const Camer
 __global__ __launch_bounds__(256) void run_cost_kernel(KConst kc, MapsDev mp, StateDev in, RunsDev runs,
                                                        const uint32_t* pixel, const int64_t* pos, int64_t n_runs,
                                                        int64_t total, float* work) {
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * blockDim.x, q = q0 + threadIdx.x;
+  const int64_t i_q = block_run_of_position(pos, n_runs, q0, q, total);
   int run = -1;
   float cost = 0.0f;
   if (q < total) {
-    const int64_t i = run_of_position(pos, n_runs, q), j = q - pos[i];
+    const int64_t i = i_q, j = q - pos[i];
     run = static_cast<int>(i);
     const uint32_t p = pixel[i];
     const float* v = runs.irr + runs.row_start[i] * kc.groups;
@@ -672,12 +665,9 @@ hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, 
 
 hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
                                const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
-                               int64_t t_state, InstantPrep* out, float* work, int* first_loud, uint32_t* n_long,
-                               hipStream_t stream) {
+                               int64_t t_state, InstantPrep* out, float* work, int* first_loud, hipStream_t stream) {
   if (n_runs <= 0) return hipSuccess;
   fill_i32_kernel<<<grid_for(static_cast<size_t>(n_runs), 256), 256, 0, stream>>>(first_loud, n_runs, INT32_MAX);
-  fill_i32_kernel<<<grid_for(static_cast<size_t>(n_runs), 256), 256, 0, stream>>>(reinterpret_cast<int32_t*>(n_long),
-                                                                                  n_runs, 0);
   if (total <= 0) return hipGetLastError();
   const dim3 grid = grid_for(static_cast<size_t>(total), 256);
 #define BSX_PREP(G)                                                                                         \
@@ -688,7 +678,6 @@ hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateD
   else if (kc.groups <= 4) BSX_PREP(4);
   else BSX_PREP(8);
 #undef BSX_PREP
-  quiet_long_kernel<<<grid, 256, 0, stream>>>(runs, pos, n_runs, total, t_state, first_loud, n_long);
   return hipGetLastError();
 }
 

@@ -369,7 +369,6 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
   if (instant) d_prep_.ensure(std::max<int64_t>(total, 1) * sizeof(InstantPrep));
   d_work_.ensure(std::max<size_t>(n_runs, 1) * sizeof(float));
   d_loud_.ensure(std::max<size_t>(n_runs, 1) * sizeof(int));
-  d_long_.ensure(std::max<size_t>(n_runs, 1) * sizeof(uint32_t));
   d_key_.ensure(P_);
   d_perm_.ensure(P_ * sizeof(uint32_t));
   d_order_hist_.ensure(128 * sizeof(uint32_t));
@@ -384,7 +383,6 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
   cd.pos = d_pos_.as<const int64_t>();
   cd.prep = instant && total > 0 ? d_prep_.as<const InstantPrep>() : nullptr;
   cd.first_loud = instant && n_runs > 0 ? d_loud_.as<const int>() : nullptr;
-  cd.n_long = d_long_.as<const uint32_t>();
   cd.work = n_runs > 0 ? d_work_.as<const float>() : nullptr;
   cd.counters = d_counters_.as<unsigned long long>();
   cd.base = 0;
@@ -405,7 +403,7 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
       if (instant) {
         hip_check(launch_instant_prep(kc, mp, state_view(cur_), runs, pixel, d_pos_.as<const int64_t>(),
                                       static_cast<int64_t>(n_runs), total, t_state_, d_prep_.as<InstantPrep>(),
-                                      d_work_.as<float>(), d_loud_.as<int>(), d_long_.as<uint32_t>(), stream),
+                                      d_work_.as<float>(), d_loud_.as<int>(), stream),
                   "instant prep kernel");
       } else {
         hip_check(launch_run_cost(kc, mp, state_view(cur_), runs, pixel, d_pos_.as<const int64_t>(),

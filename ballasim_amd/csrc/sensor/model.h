@@ -134,7 +134,7 @@ class SensorModel {
   DeviceBuffer d_t_, d_irr_, d_row_start_, d_row_count_, d_frame_, d_ids_;
   DeviceBuffer d_pos_, d_scan_, d_prep_;  // the runs' segment positions and the prepared segment values
   DeviceBuffer d_work_, d_key_, d_perm_, d_order_hist_;  // estimated run costs and the heavy-first pixel order
-  DeviceBuffer d_loud_, d_long_;  // per run: the first segment that is not quiet, the long quiet ones before it
+  DeviceBuffer d_loud_;  // per run: the first segment that is not quiet under the input state
   DeviceBuffer d_cams_, d_queue_;  // the walk's camera descriptor and the persistent queue's counter
   CameraDev host_cam_{};           // the descriptor's host copy (source of its upload)
   WalkTuning tuning_{256, 8, 8, 0};
