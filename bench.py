@@ -25,6 +25,16 @@ if ROOT not in sys.path:
 WINDOW = 1 << 23
 HBM_PEAK_GBS = 8000.0
 S_STATE, S_MAP, S_EVENT = 48, 16, 16  # the fixed accounting's per-pixel state, map and per-event bytes
+# The kernels of one advance call (launched back to back on the call's stream and timed together
+# with HIP events around them: the roofline's "launch").
+ADVANCE_CHAIN = {
+    "instantaneous": "advance chain: instant_prep_kernel + order_count/offsets/scatter + instant_walk_kernel",
+    "cascade": "advance chain: run_cost_kernel + order_count/offsets/scatter + cascade_kernel",
+}
+# Measured HBM traffic of the instantaneous chain on this workload, bytes per sample: rocprofv3
+# FETCH_SIZE x 2 + WRITE_SIZE summed over the chain's kernels, averaged over the timed launches (the
+# factor 2 and 1 from a 1 GiB device copy read by the same counters: profiles/advance_chain_1280x720.md).
+TRAFFIC_BYTES_PER_SAMPLE = 68.8
 
 
 def sensor_params(bsx, photoreceptor):
@@ -47,9 +57,10 @@ def algorithmic_bytes(samples, n_listed, n_pixels, events, groups=1):
     return samples * (8 + 4 * groups) + n_listed * 12 + n_pixels * 2 * S_STATE + n_pixels * S_MAP + events * S_EVENT
 
 
-def cpu_baseline(p, runs_host, frame, width, height, budget_s):
+def cpu_baseline(p, window_runs, frame, width, height, budget_s):
     """The reference's own CPU path (compiled from its sources: oracle/_ref) when present, else the
-    oracle's C restatement, on the first windows of the same runs until budget_s of CPU work."""
+    oracle's C restatement, on the first windows of the same runs until budget_s of CPU work
+    (window_runs yields each window's runs copied to the host)."""
     import numpy as np
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)
@@ -58,7 +69,7 @@ def cpu_baseline(p, runs_host, frame, width, height, budget_s):
          else O.OracleModel(p.c, width, height, 1, 1, 0, threads))
     m.init(0, frame.reshape(-1, 1))
     samples, events, elapsed, windows = 0, 0, 0.0, 0
-    for i, (ids, t, irr, offs) in enumerate(runs_host):
+    for i, (ids, t, irr, offs) in enumerate(window_runs):
         t0 = time.perf_counter()
         ev, _ = m.advance_runs(ids, t, irr, offs, (i + 1) * WINDOW)
         elapsed += time.perf_counter() - t0
@@ -79,7 +90,7 @@ def main():
     ap.add_argument("--steps", type=int, default=58)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--photoreceptor", default="instantaneous", choices=["instantaneous", "cascade"])
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline work (0: skip)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work (0: skip)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (the reference kernel for rocprofv3's "
@@ -161,16 +172,16 @@ def main():
         "events_per_s": events / elapsed,
         "samples_per_event": samples / max(events, 1),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "bsx::instant_kernel" if args.photoreceptor == "instantaneous" else "bsx::cascade_kernel",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (TRAFFIC_BYTES_PER_SAMPLE * samples / args.steps
+                                 if args.photoreceptor == "instantaneous" and TRAFFIC_BYTES_PER_SAMPLE else None),
+                     "kernel": ADVANCE_CHAIN[args.photoreceptor],
                      "kernel_ms_per_launch": st["kernel_ms"] / args.steps,
                      "bytes_per_launch": bytes_total / args.steps},
     }
     if args.cpu_budget > 0:
-        runs_host = []
-        for i in range(min(n_win, 6)):
-            runs_host.append(S.runs_to_host(runs[i]))
-        line["cpu_baseline"] = cpu_baseline(p, runs_host, frame, W, H, args.cpu_budget)
+        window_runs = (S.runs_to_host(runs[i]) for i in range(n_win))
+        line["cpu_baseline"] = cpu_baseline(p, window_runs, frame, W, H, args.cpu_budget)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
