@@ -93,6 +93,11 @@ class LaunchPolicy(C.Structure):
                 ("search_batch", C.c_int32), ("queue_chunk", C.c_int32)]
 
 
+class RigStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("graph_launches", C.c_uint64), ("captures", C.c_uint64),
+                ("walk_reruns", C.c_uint64), ("kernel_ms", C.c_double)]
+
+
 class Info(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("groups", C.c_int32), ("initialized", C.c_int32),
                 ("time_ns", C.c_int64), ("readout_phase_ns", C.c_int64), ("irr_max", C.c_float),
@@ -144,6 +149,15 @@ SIGNATURES = {
     "bsx_sensor_set_launch_policy": (C.c_int, [P, C.POINTER(LaunchPolicy)]),
     "bsx_sensor_info": (C.c_int, [P, C.POINTER(Info)]),
     "bsx_sensor_maps": (C.c_int, [P, P, P, P, P, P, P]),
+    "bsx_rig_create": (C.c_int, [C.POINTER(P), C.c_int32, C.POINTER(P)]),
+    "bsx_rig_destroy": (C.c_int, [P]),
+    "bsx_rig_advance_device": (C.c_int, [P, C.POINTER(DeviceRuns), C.c_int64, P, C.POINTER(DeviceEvents)]),
+    "bsx_rig_set_tuning": (C.c_int, [P, C.POINTER(LaunchPolicy)]),
+    "bsx_rig_get_tuning": (C.c_int, [P, C.POINTER(LaunchPolicy)]),
+    "bsx_rig_set_graph": (C.c_int, [P, C.c_int32]),
+    "bsx_rig_stats": (C.c_int, [P, C.POINTER(RigStats)]),
+    "bsx_rig_autotune": (C.c_int, [P, C.POINTER(DeviceRuns), C.c_int64, P, C.c_int32, C.POINTER(LaunchPolicy),
+                                   C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32)]),
     "bsx_draw_maps": (C.c_int, [C.POINTER(Params), C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P]),
     "bsx_scene_create": (C.c_int, [C.POINTER(Shape), C.c_int32, C.c_float, C.POINTER(P)]),
     "bsx_scene_destroy": (C.c_int, [P]),

@@ -21,6 +21,8 @@ SOURCES = [
     "sensor/params.cpp",
     "sensor/maps.cpp",
     "sensor/model.cpp",
+    "sensor/chain.cpp",
+    "sensor/rig.cpp",
     "sensor/kernels.hip",
     "scene/scene_host.cpp",
     "scene/scene.hip",

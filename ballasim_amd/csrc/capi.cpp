@@ -4,18 +4,26 @@
 // RuntimeError and so on (src/sensor/sensor_module.cpp).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
+#include "../../include/bsx/bsx_rig.h"
 #include "../../include/bsx/bsx_sensor.h"
 #include "sensor/kernels.h"
 #include "sensor/model.h"
 #include "sensor/params.h"
+#include "sensor/rig.h"
 
 struct bsx_sensor {
   bsx::SensorModel* model;
+};
+
+struct bsx_rig {
+  bsx::Rig* rig;
 };
 
 namespace {
@@ -45,6 +53,20 @@ int guarded(F&& f) {
 bsx::SensorModel& model_of(bsx_sensor* s) {
   if (s == nullptr || s->model == nullptr) throw std::invalid_argument("bsx: null sensor handle");
   return *s->model;
+}
+
+bsx::Rig& rig_of(bsx_rig* r) {
+  if (r == nullptr || r->rig == nullptr) throw std::invalid_argument("bsx: null rig handle");
+  return *r->rig;
+}
+
+bsx_launch_policy policy_of(const bsx::WalkTuning& t) {
+  bsx_launch_policy p{};
+  p.block_size = t.block;
+  p.walk_budget = t.walk_budget;
+  p.search_batch = t.search_batch;
+  p.queue_chunk = t.chunk;
+  return p;
 }
 
 void need_ptr(const void* p, const char* what) {
@@ -293,6 +315,84 @@ int bsx_sensor_maps(bsx_sensor* s, double* theta_on, double* theta_off, double* 
     if (dark_fA != nullptr) std::memcpy(dark_fA, m.dark_fA.data(), n * 8);
     if (phi0 != nullptr) std::memcpy(phi0, m.phi0.data(), n * 8);
     if (hot != nullptr) std::memcpy(hot, m.hot.data(), n);
+  });
+}
+
+int bsx_rig_create(bsx_sensor* const* cameras, int32_t n_cameras, bsx_rig** out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    *out = nullptr;
+    if (n_cameras < 1 || n_cameras > 64) throw std::invalid_argument("bsx_rig_create: need 1 to 64 cameras");
+    need_ptr(cameras, "cameras");
+    std::vector<bsx::SensorModel*> ms;
+    for (int32_t c = 0; c < n_cameras; ++c) ms.push_back(&model_of(cameras[c]));
+    auto* r = new bsx::Rig(std::move(ms));
+    *out = new bsx_rig{r};
+  });
+}
+
+int bsx_rig_destroy(bsx_rig* rig) {
+  return guarded([&] {
+    if (rig == nullptr) return;
+    delete rig->rig;
+    delete rig;
+  });
+}
+
+int bsx_rig_advance_device(bsx_rig* rig, const bsx_device_runs* runs, int64_t t_end_ns, void* stream,
+                           bsx_device_events* out) {
+  return guarded([&] {
+    need_ptr(runs, "runs");
+    need_ptr(out, "out");
+    rig_of(rig).advance_device(runs, t_end_ns, static_cast<hipStream_t>(stream), out);
+  });
+}
+
+int bsx_rig_set_tuning(bsx_rig* rig, const bsx_launch_policy* tuning) {
+  return guarded([&] {
+    need_ptr(tuning, "tuning");
+    bsx::LaunchChain& ch = rig_of(rig).chain();
+    bsx::WalkTuning t = ch.tuning();
+    if (tuning->block_size != 0) t.block = tuning->block_size;
+    if (tuning->walk_budget != 0) t.walk_budget = tuning->walk_budget;
+    if (tuning->search_batch != 0) t.search_batch = tuning->search_batch;
+    if (tuning->queue_chunk != 0) t.chunk = tuning->queue_chunk < 0 ? 0 : tuning->queue_chunk;
+    ch.set_tuning(t);
+  });
+}
+
+int bsx_rig_get_tuning(bsx_rig* rig, bsx_launch_policy* out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    *out = policy_of(rig_of(rig).chain().tuning());
+  });
+}
+
+int bsx_rig_set_graph(bsx_rig* rig, int32_t on) {
+  return guarded([&] { rig_of(rig).chain().set_graph(on != 0); });
+}
+
+int bsx_rig_stats(bsx_rig* rig, bsx_rig_stats_t* out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    bsx::Rig& r = rig_of(rig);
+    const bsx::ChainStats& s = r.chain().stats();
+    *out = bsx_rig_stats_t{s.launches, s.graph_launches, s.captures, s.walk_reruns, r.kernel_ms()};
+  });
+}
+
+int bsx_rig_autotune(bsx_rig* rig, const bsx_device_runs* runs, int64_t t_end_ns, void* stream, int32_t reps,
+                     bsx_launch_policy* tunings, double* walk_ms, int32_t max_trials, int32_t* n_trials) {
+  return guarded([&] {
+    need_ptr(runs, "runs");
+    if (max_trials < 0) throw std::invalid_argument("bsx_rig_autotune: max_trials must be >= 0");
+    const std::vector<bsx::RigTrial> tr = rig_of(rig).autotune(runs, t_end_ns, static_cast<hipStream_t>(stream), reps);
+    const int32_t k = std::min<int32_t>(max_trials, static_cast<int32_t>(tr.size()));
+    for (int32_t i = 0; i < k; ++i) {
+      if (tunings != nullptr) tunings[i] = policy_of(tr[i].tuning);
+      if (walk_ms != nullptr) walk_ms[i] = tr[i].walk_ms;
+    }
+    if (n_trials != nullptr) *n_trials = static_cast<int32_t>(tr.size());
   });
 }
 

@@ -95,21 +95,37 @@ struct InstantPrep {
 
 struct RawEventDev;
 
-// One camera's view for a walk launch. A rig launches several cameras at once: their pixels form one
-// index space, camera c holding [base, base + kc.n_pixels).
+// One camera's view for a launch. A launch covers one camera or a rig of several: their pixels form
+// one index space (camera c holds [base, base + kc.n_pixels)), their runs one run space ([run_base,
+// run_base + n_runs)) and their segments one segment space ([seg_base, seg_base + total)); run_base
+// and seg_base are multiples of 256, so that no 256-thread block of a run- or segment-indexed kernel
+// spans two cameras.
 struct CameraDev {
   KConst kc;
   MapsDev mp;
   StateDev in, out;
   RunsDev runs;
-  const int64_t* pos;          // the runs' segment positions (phase A)
-  const InstantPrep* prep;     // phase A's values (null: exact searches only)
-  const int* first_loud;       // per run (null: every walk from its start)
-  const float* work;           // per run: estimated walk cost (null: no order information)
+  const uint32_t* pixel;       // the runs' pixel ids
+  const int64_t* pos;          // the runs' segment positions (exclusive scan of the run lengths)
+  InstantPrep* prep;           // phase A's values (null: exact searches only)
+  int* first_loud;             // per run (null: every walk from its start)
+  float* work;                 // per run: estimated walk cost (null: no order information)
   RawEventDev* ev;
   unsigned long long cap;
   unsigned long long* counters;  // [0] events, [1 + StatIdx] work counters
+  int64_t t_state;             // the camera's model time at the start of the call
   uint32_t base;
+  int64_t n_runs, run_base;
+  int64_t total, seg_base;
+};
+
+// What a launch's kernels read besides the cameras: the call's end time and the extents of the joint
+// run and segment spaces (padding included). Kept in device memory, so that a captured graph of the
+// launch replays any call whose extents fit the grids it was captured with.
+struct CallDev {
+  int64_t t_end;
+  int64_t n_segments;
+  int64_t n_runs;
 };
 
 // The walk's launch shape, chosen by the autotuner: threads per block, segments a walking lane does

@@ -10,31 +10,35 @@ namespace bsx {
 
 hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, const float* irr0, int64_t t0,
                        unsigned long long* counters, hipStream_t stream);
-// Phase A: the prepared values of every segment of every run (pos: the exclusive scan of the run
-// lengths, total = pos[n_runs]), each run's estimated walk cost (work, added to) and its first segment
-// that is not quiet under the input state (first_loud);
-// phase B: the per-pixel walk (prep null: exact searches only; first_loud null: from the start).
-hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
-                               const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
-                               int64_t t_state, InstantPrep* out, float* work, int* first_loud, hipStream_t stream);
-// The instantaneous walk over the pixels of n_cams cameras (descriptors in device memory), lane i
-// taking pixel order[i] of the joint pixel space (order null: index order); queue (one word) is the
-// persistent queue's counter when tu.chunk > 0.
-hipError_t launch_instant_walk(const CameraDev* cams, int n_cams, uint32_t n_total, int groups, const uint32_t* order,
-                               unsigned int* queue, const WalkTuning& tu, int64_t t_state, int64_t t_end,
-                               hipStream_t stream);
-// The cascade without noise (the run costs for its order: launch_run_cost, zero-filling work first).
-hipError_t launch_run_cost(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
-                           const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total, float* work,
-                           hipStream_t stream);
-hipError_t launch_cascade(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
-                          const RunsDev& runs, int64_t t_state, int64_t t_end, RawEventDev* ev, unsigned long long cap,
-                          unsigned long long* counters, const uint32_t* order, int block, hipStream_t stream);
-// Heavy-first order of the joint pixel space of n_cams cameras from their runs' estimated costs
-// (CameraDev::work, filled by phase A or launch_run_cost): key (n_total bytes) and hist (128 words)
-// are scratch, perm receives the order.
-hipError_t launch_cost_order(const CameraDev* cams, int n_cams, uint32_t n_total, uint8_t* key, uint32_t* hist,
-                             uint32_t* perm, hipStream_t stream);
+// One launch of the advance over one camera or a rig of cameras: descriptors and the call header in
+// device memory (so that a captured graph replays any call whose extents fit its grids), the joint
+// pixel order's scratch (key: n_total bytes, hist: 128 words), the order (perm: n_total words) and
+// the persistent queue's counter.
+struct ChainArgs {
+  const CameraDev* cams;
+  int n_cams;
+  uint32_t n_total;      // pixels of all cameras
+  int groups;            // the largest light-group count of the cameras (the kernels' register tier)
+  bool instant;          // instantaneous response (else the cascade)
+  const CallDev* call;
+  int64_t seg_capacity;  // grid extents, at least the call's n_segments and n_runs
+  int64_t run_capacity;
+  uint8_t* key;
+  uint32_t* hist;
+  uint32_t* perm;
+  unsigned int* queue;
+  WalkTuning tu;
+};
+// Zeroes the counters and per-run arrays, runs phase A (instantaneous: every segment's prepared values,
+// each run's estimated cost and first loud segment; cascade: the run costs) and the heavy-first pixel
+// order of the joint pixel space.
+hipError_t launch_chain_prepare(const ChainArgs& a, hipStream_t stream);
+// The walk over every camera's pixels (persistent when tu.chunk > 0: chain_walk_blocks blocks, as many
+// as are resident at once).
+hipError_t launch_chain_walk(const ChainArgs& a, hipStream_t stream);
+int chain_walk_blocks(const ChainArgs& a);
+// Zeroes the counters and the queue only (before a re-run of the walk after an event-buffer overflow).
+hipError_t launch_chain_reset(const ChainArgs& a, hipStream_t stream);
 hipError_t launch_fill_f32(float* dst, size_t n, float v, hipStream_t stream);
 // out[0] = pm::log monotonicity violations, out[1] = hash of its bits over every positive float.
 hipError_t launch_math_selfcheck(unsigned long long* out, hipStream_t stream);

@@ -12,27 +12,6 @@ namespace bsx {
 
 namespace {
 
-// Wave-aggregated event sink: the lanes that reach a push together reserve consecutive slots with
-// one atomic add (ballot over the active lanes, rank by a popcount of the lower lanes), then each
-// writes its 32-byte event; slots past the capacity are counted, not written (the host re-runs the
-// call from the untouched input state with a larger buffer).
-struct WaveSink {
-  RawEventDev* ev;
-  unsigned long long cap;
-  unsigned long long* count;
-  __device__ __forceinline__ void push(const RawEventDev& e) {
-    const unsigned long long mask = __ballot(1);
-    const int lane = __lane_id();
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    const unsigned rank = __popcll(mask & ((1ull << lane) - 1ull));
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(count, static_cast<unsigned long long>(__popcll(mask)));
-    base = __shfl(base, leader);
-    const unsigned long long slot = base + rank;
-    if (slot < cap) ev[slot] = e;
-  }
-};
-
 __device__ __forceinline__ float clamp_irr(float v, float vmax_) { return v >= 0.0f ? (v <= vmax_ ? v : vmax_) : 0.0f; }
 
 // Sum of a per-lane counter over the wave, added once to the global counter.
@@ -161,20 +140,37 @@ __device__ __forceinline__ int64_t block_run_of_position(const int64_t* pos, int
   return i;
 }
 
+// The camera of a launch that holds offset q of the joint run (kRuns) or segment space (cameras in
+// offset order, a handful at most; an empty camera shares its offset with the next and is skipped).
+enum class Space { kRuns, kSegments };
+template <Space kSpace>
+__device__ __forceinline__ int camera_at(const CameraDev* cams, int n_cams, int64_t q) {
+  int c = 0;
+  while (c + 1 < n_cams && q >= (kSpace == Space::kRuns ? cams[c + 1].run_base : cams[c + 1].seg_base)) ++c;
+  return c;
+}
+
+// Phase A of the instantaneous response over every segment of every camera's runs: one block of 256
+// consecutive positions of one camera (the segment bases are multiples of 256).
 template <int kMaxG>
-__global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp, StateDev in, RunsDev runs,
-                                                           const uint32_t* pixel, const int64_t* pos, int64_t n_runs,
-                                                           int64_t total, int64_t t_state, InstantPrep* out,
-                                                           float* work, int* first_loud) {
-  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * blockDim.x, q = q0 + threadIdx.x;
-  const int64_t i_q = block_run_of_position(pos, n_runs, q0, q, total);
+__global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, int n_cams, const CallDev* call) {
+  const int64_t g0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
+  if (g0 >= call->n_segments) return;  // beyond this call's segments (a grid sized for a larger one)
+  const CameraDev& c = cams[camera_at<Space::kSegments>(cams, n_cams, g0)];
+  const int64_t q0 = g0 - c.seg_base, q = q0 + threadIdx.x;
+  const int64_t total = c.total;
+  const int64_t i_q = block_run_of_position(c.pos, c.n_runs, q0, q, total);
+  const KConst& kc = c.kc;
+  const MapsDev& mp = c.mp;
+  const StateDev& in = c.in;
+  const RunsDev& runs = c.runs;
   int run = -1;
   float cost = 0.0f;
   int loud = INT32_MAX;
   if (q < total) {
-    const int64_t i = i_q, j = q - pos[i];
+    const int64_t i = i_q, j = q - c.pos[i];
     run = static_cast<int>(i);
-    const uint32_t p = pixel[i];
+    const uint32_t p = c.pixel[i];
     const int64_t s = runs.row_start[i];
     InstantLane<kMaxG> L;
     L.G = kc.groups;
@@ -196,11 +192,11 @@ __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp
     const float* v = runs.irr + s * kc.groups;
     bool empty = false;
     if (j == 0) {
-      if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
-        out[q] = InstantPrep{0.0f, 0.0f, 0.0f, 0.0f};
+      if (t[0] == c.t_state) {  // the first sample is the model time's: no segment ends at it
+        c.prep[q] = InstantPrep{0.0f, 0.0f, 0.0f, 0.0f};
         empty = true;
       } else {
-        L.ta = t_state;
+        L.ta = c.t_state;
         for (int g = 0; g < kMaxG; ++g)
           if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
       }
@@ -214,15 +210,45 @@ __global__ __launch_bounds__(256) void instant_prep_kernel(KConst kc, MapsDev mp
       for (int g = 0; g < kMaxG; ++g)
         if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
       const InstantPrep r = L.prep_segment();
-      out[q] = r;
+      c.prep[q] = r;
       const float theta = vmin(L.theta_on, L.theta_off);
       const bool quiet = L.quiet_segment(r);
       loud = quiet ? INT32_MAX : static_cast<int>(j);
       cost = quiet ? 0.0f : 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) / theta;
     }
   }
-  if (work != nullptr) wave_segmented_add(work, run, cost);
-  if (first_loud != nullptr) wave_segmented_min(first_loud, run, loud);
+  if (c.work != nullptr) wave_segmented_add(c.work, run, cost);
+  if (c.first_loud != nullptr) wave_segmented_min(c.first_loud, run, loud);
+}
+
+// Before phase A: zero each camera's counters, the order histogram and the queue counter (block 0),
+// and set every run's cost to 0 and first loud segment to none (one thread per run, cameras' run
+// bases multiples of 256).
+__device__ __forceinline__ void reset_counters(const CameraDev* cams, int n_cams, unsigned int* queue) {
+  for (int k = threadIdx.x; k < n_cams * (1 + kStCount); k += blockDim.x)
+    cams[k / (1 + kStCount)].counters[k % (1 + kStCount)] = 0ull;
+  if (threadIdx.x == 0 && queue != nullptr) *queue = 0u;
+}
+
+__global__ __launch_bounds__(256) void chain_init_kernel(const CameraDev* cams, int n_cams, const CallDev* call,
+                                                         uint32_t* hist, unsigned int* queue) {
+  if (blockIdx.x == 0) {
+    reset_counters(cams, n_cams, queue);
+    if (threadIdx.x < 128) hist[threadIdx.x] = 0u;
+  }
+  const int64_t g0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
+  if (g0 >= call->n_runs) return;
+  const CameraDev& c = cams[camera_at<Space::kRuns>(cams, n_cams, g0)];
+  const int64_t r = g0 - c.run_base + threadIdx.x;
+  if (r < c.n_runs) {
+    if (c.work != nullptr) c.work[r] = 0.0f;
+    if (c.first_loud != nullptr) c.first_loud[r] = INT32_MAX;
+  }
+}
+
+// Before a re-run of the walk: the counters and the queue counter only.
+__global__ void chain_reset_kernel(const CameraDev* cams, int n_cams, unsigned int* queue) {
+  reset_counters(cams, n_cams, queue);
 }
 
 // The pixel order of a walk: pixels by estimated cost, heaviest first (64 logarithmic buckets, bucket
@@ -330,7 +356,7 @@ __device__ __forceinline__ void camera_add(const CameraDev* cams, int cam_index,
 template <int kMaxG>
 __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams, uint32_t n_total,
                                               const uint32_t* order, uint32_t i, const WalkTuning& tu,
-                                              int64_t t_state, int64_t t_end, uint32_t* stacks) {
+                                              int64_t t_end, uint32_t* stacks) {
   InstantLane<kMaxG> L;
   L.n_substeps = L.n_pieces = L.n_leaves = 0;
   int ci = -1;
@@ -375,8 +401,8 @@ __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams,
       L.irr = nullptr;
       L.n = 0;
     }
-    L.t_state = t_state;
-    L.skip = L.n > 0 && L.ts[0] == t_state ? 1 : 0;
+    L.t_state = cam->t_state;
+    L.skip = L.n > 0 && L.ts[0] == L.t_state ? 1 : 0;
     for (int g = 0; g < kMaxG; ++g)
       if (g < kc.groups) L.held[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
     L.i_scale = in.i_scale[p];
@@ -435,22 +461,18 @@ __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams,
   camera_add(cams, ci, 1 + kStLeaves, L.n_leaves);
 }
 
-// The instantaneous walk over the pixels of n_cams cameras. Without a queue (tu.chunk = 0) each wave
-// takes one 64-pixel group; with one, waves grab tu.chunk groups at a time from the queue counter
-// until the pixel space is exhausted (a persistent grid: every wave exits once a grab is empty).
-template <int kMaxG>
-__global__ __launch_bounds__(kInstantBlock) void instant_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
-                                                                   const uint32_t* order, unsigned int* queue,
-                                                                   WalkTuning tu, int64_t t_state, int64_t t_end) {
-  // The lanes' bisection stacks, one column per lane (conflict-free: a level is a row of the block;
-  // blockDim.x <= kInstantBlock by the launch bound).
-  __shared__ uint32_t stacks[InstantLane<kMaxG>::kSearchDepth * kInstantBlock];
+// The walk over the pixels of n_cams cameras, in a launch's pixel order. Without a queue (tu.chunk =
+// 0) each wave takes one 64-pixel group; with one, waves grab tu.chunk groups at a time from the
+// queue counter until the pixel space is exhausted (a persistent grid: every wave exits once a grab
+// is empty).
+template <class Group>
+__device__ __forceinline__ void walk_groups(uint32_t n_total, unsigned int* queue, const WalkTuning& tu,
+                                            const Group& group_at) {
   const uint32_t lane = __lane_id();
   const uint32_t n_groups = (n_total + 63) / 64;
   if (tu.chunk <= 0 || queue == nullptr) {
     const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    if (group < n_groups)
-      instant_group<kMaxG>(cams, n_cams, n_total, order, group * 64 + lane, tu, t_state, t_end, stacks + threadIdx.x);
+    if (group < n_groups) group_at(group * 64 + lane);
     return;
   }
   while (true) {
@@ -459,49 +481,76 @@ __global__ __launch_bounds__(kInstantBlock) void instant_walk_kernel(const Camer
     g0 = __shfl(g0, 0);
     if (g0 >= n_groups) break;
     const uint32_t g1 = vmin(g0 + static_cast<uint32_t>(tu.chunk), n_groups);
-    for (uint32_t group = g0; group < g1; ++group)
-      instant_group<kMaxG>(cams, n_cams, n_total, order, group * 64 + lane, tu, t_state, t_end, stacks + threadIdx.x);
+    for (uint32_t group = g0; group < g1; ++group) group_at(group * 64 + lane);
   }
 }
 
-// Estimated cost of each run's cascade walk for the heavy-first pixel order: one unit per segment
-// plus kSearchWeight per threshold's worth of log-intensity change (fast logs: a schedule only).
-__global__ __launch_bounds__(256) void run_cost_kernel(KConst kc, MapsDev mp, StateDev in, RunsDev runs,
-                                                       const uint32_t* pixel, const int64_t* pos, int64_t n_runs,
-                                                       int64_t total, float* work) {
-  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * blockDim.x, q = q0 + threadIdx.x;
-  const int64_t i_q = block_run_of_position(pos, n_runs, q0, q, total);
+template <int kMaxG>
+__global__ __launch_bounds__(kInstantBlock) void instant_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
+                                                                   const uint32_t* order, unsigned int* queue,
+                                                                   WalkTuning tu, const CallDev* call) {
+  // The lanes' bisection stacks, one column per lane (conflict-free: a level is a row of the block;
+  // blockDim.x <= kInstantBlock by the launch bound).
+  __shared__ uint32_t stacks[InstantLane<kMaxG>::kSearchDepth * kInstantBlock];
+  const int64_t t_end = call->t_end;
+  walk_groups(n_total, queue, tu, [&](uint32_t i) {
+    instant_group<kMaxG>(cams, n_cams, n_total, order, i, tu, t_end, stacks + threadIdx.x);
+  });
+}
+
+// Estimated cost of each run's cascade walk for the heavy-first pixel order, over every segment of
+// every camera's runs: one unit per segment plus kSearchWeight per threshold's worth of
+// log-intensity change (fast logs: a schedule only).
+__global__ __launch_bounds__(256) void chain_cost_kernel(const CameraDev* cams, int n_cams, const CallDev* call) {
+  const int64_t g0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
+  if (g0 >= call->n_segments) return;
+  const CameraDev& c = cams[camera_at<Space::kSegments>(cams, n_cams, g0)];
+  const int64_t q0 = g0 - c.seg_base, q = q0 + threadIdx.x;
+  const int64_t i_q = block_run_of_position(c.pos, c.n_runs, q0, q, c.total);
+  const KConst& kc = c.kc;
+  const MapsDev& mp = c.mp;
   int run = -1;
   float cost = 0.0f;
-  if (q < total) {
-    const int64_t i = i_q, j = q - pos[i];
+  if (q < c.total) {
+    const int64_t i = i_q, j = q - c.pos[i];
     run = static_cast<int>(i);
-    const uint32_t p = pixel[i];
-    const float* v = runs.irr + runs.row_start[i] * kc.groups;
+    const uint32_t p = c.pixel[i];
+    const float* v = c.runs.irr + c.runs.row_start[i] * kc.groups;
     float Ia = mp.dark[p], Ib = mp.dark[p];
     for (int g = 0; g < kc.groups; ++g) {
-      Ia += mp.scale[p] * (j == 0 ? in.held[static_cast<int64_t>(g) * kc.n_pixels + p] : v[(j - 1) * kc.groups + g]);
+      Ia += mp.scale[p] * (j == 0 ? c.in.held[static_cast<int64_t>(g) * kc.n_pixels + p] : v[(j - 1) * kc.groups + g]);
       Ib += mp.scale[p] * v[j * kc.groups + g];
     }
     const float theta = vmin(mp.theta_on[p], mp.theta_off[p]);
     cost = 1.0f + kSearchWeight * fabs_(__logf(Ib) - __logf(Ia)) / theta;
   }
-  wave_segmented_add(work, run, cost);
+  if (c.work != nullptr) wave_segmented_add(c.work, run, cost);
 }
 
-// The cascade without noise: one lane per pixel, in the heavy-first order (null: pixel order).
+// The cascade without noise: lane i of the launch's pixel order takes pixel order[i] of its camera
+// and advances it to t_end.
 template <int kMaxG>
-__global__ __launch_bounds__(256) void cascade_kernel(KConst kc, MapsDev mp, StateDev in, StateDev out, RunsDev runs,
-                                                      int64_t t_state, int64_t t_end, RawEventDev* ev,
-                                                      unsigned long long cap, unsigned long long* counters,
-                                                      const uint32_t* order) {
-  const int lane_index = blockIdx.x * blockDim.x + threadIdx.x;
-  const int p = order != nullptr && lane_index < kc.n_pixels ? static_cast<int>(order[lane_index]) : lane_index;
+__device__ __forceinline__ void cascade_lane(const CameraDev* cams, int n_cams, uint32_t n_total, const uint32_t* order,
+                                             uint32_t i, int64_t t_end) {
   CascadeLane<kMaxG> L;
   L.n_substeps = L.n_pieces = L.n_leaves = 0;
   L.n_attempts = L.n_settled = L.n_fallbacks = L.n_quarantines = 0;
-  WaveSink sink{ev, cap, counters};
-  if (p < kc.n_pixels) {
+  int ci = -1;
+  const CameraDev* cam = cams;
+  int p = 0;
+  if (i < n_total) {
+    const uint32_t g = order != nullptr ? order[i] : i;
+    ci = camera_of(cams, n_cams, g);
+    cam = cams + ci;
+    p = static_cast<int>(g - cam->base);
+  }
+  RigSink sink{cam, ci};
+  if (ci >= 0) {
+    const KConst& kc = cam->kc;
+    const MapsDev& mp = cam->mp;
+    const StateDev& in = cam->in;
+    const StateDev& out = cam->out;
+    const RunsDev& runs = cam->runs;
     L.pixel = static_cast<uint32_t>(p);
     L.theta_on = mp.theta_on[p];
     L.theta_off = mp.theta_off[p];
@@ -527,8 +576,8 @@ __global__ __launch_bounds__(256) void cascade_kernel(KConst kc, MapsDev mp, Sta
       L.irr = nullptr;
       L.n = 0;
     }
-    L.t_state = t_state;
-    L.skip = L.n > 0 && L.ts[0] == t_state ? 1 : 0;
+    L.t_state = cam->t_state;
+    L.skip = L.n > 0 && L.ts[0] == L.t_state ? 1 : 0;
     for (int g = 0; g < kMaxG; ++g)
       if (g < kc.groups) L.held[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
     L.i_scale = in.i_scale[p];
@@ -566,13 +615,21 @@ __global__ __launch_bounds__(256) void cascade_kernel(KConst kc, MapsDev mp, Sta
     out.shot_next[p] = in.shot_next[p];
     out.shot_index[p] = in.shot_index[p];
   }
-  wave_add(counters + 1 + kStSubsteps, L.n_substeps);
-  wave_add(counters + 1 + kStAttempts, L.n_attempts);
-  wave_add(counters + 1 + kStSettled, L.n_settled);
-  wave_add(counters + 1 + kStLeaves, L.n_leaves);
-  wave_add(counters + 1 + kStFallbacks, L.n_fallbacks);
-  wave_add(counters + 1 + kStQuarantines, L.n_quarantines);
-  wave_add(counters + 1 + kStPieces, L.n_pieces);
+  camera_add(cams, ci, 1 + kStSubsteps, L.n_substeps);
+  camera_add(cams, ci, 1 + kStAttempts, L.n_attempts);
+  camera_add(cams, ci, 1 + kStSettled, L.n_settled);
+  camera_add(cams, ci, 1 + kStLeaves, L.n_leaves);
+  camera_add(cams, ci, 1 + kStFallbacks, L.n_fallbacks);
+  camera_add(cams, ci, 1 + kStQuarantines, L.n_quarantines);
+  camera_add(cams, ci, 1 + kStPieces, L.n_pieces);
+}
+
+template <int kMaxG>
+__global__ __launch_bounds__(256) void cascade_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
+                                                           const uint32_t* order, unsigned int* queue, WalkTuning tu,
+                                                           const CallDev* call) {
+  const int64_t t_end = call->t_end;
+  walk_groups(n_total, queue, tu, [&](uint32_t i) { cascade_lane<kMaxG>(cams, n_cams, n_total, order, i, t_end); });
 }
 
 // row_of[pixel[i]] = i, flagging out-of-range ids (flags[0]), duplicates (flags[1]) and runs outside
@@ -663,93 +720,94 @@ hipError_t launch_init(const KConst& kc, const MapsDev& mp, const StateDev& st, 
   return hipGetLastError();
 }
 
-hipError_t launch_instant_prep(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
-                               const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total,
-                               int64_t t_state, InstantPrep* out, float* work, int* first_loud, hipStream_t stream) {
-  if (n_runs <= 0) return hipSuccess;
-  fill_i32_kernel<<<grid_for(static_cast<size_t>(n_runs), 256), 256, 0, stream>>>(first_loud, n_runs, INT32_MAX);
-  if (total <= 0) return hipGetLastError();
-  const dim3 grid = grid_for(static_cast<size_t>(total), 256);
-#define BSX_PREP(G)                                                                                         \
-  instant_prep_kernel<G><<<grid, 256, 0, stream>>>(kc, mp, in, runs, pixel, pos, n_runs, total, t_state, out, work, \
-                                                   first_loud)
-  if (kc.groups <= 1) BSX_PREP(1);
-  else if (kc.groups <= 2) BSX_PREP(2);
-  else if (kc.groups <= 4) BSX_PREP(4);
-  else BSX_PREP(8);
-#undef BSX_PREP
+namespace {
+
+// Blocks of the walk kernel resident on the device at once (a persistent grid's size), per kernel,
+// light-group tier and block size.
+int resident_blocks(bool instant, int tier, int block) {
+  static int cache[2][4][4] = {};
+  int& r = cache[instant ? 1 : 0][tier][block / 64 - 1];
+  if (r == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    static const void* const kernels[2][4] = {
+        {reinterpret_cast<const void*>(cascade_walk_kernel<1>), reinterpret_cast<const void*>(cascade_walk_kernel<2>),
+         reinterpret_cast<const void*>(cascade_walk_kernel<4>), reinterpret_cast<const void*>(cascade_walk_kernel<8>)},
+        {reinterpret_cast<const void*>(instant_walk_kernel<1>), reinterpret_cast<const void*>(instant_walk_kernel<2>),
+         reinterpret_cast<const void*>(instant_walk_kernel<4>), reinterpret_cast<const void*>(instant_walk_kernel<8>)}};
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[instant ? 1 : 0][tier], block, 0) != hipSuccess)
+      per_cu = 1;
+    r = vmax(1, cus) * vmax(1, per_cu);
+  }
+  return r;
+}
+
+int group_tier(int groups) { return groups <= 1 ? 0 : groups <= 2 ? 1 : groups <= 4 ? 2 : 3; }
+
+hipError_t chain_check(const ChainArgs& a) {
+  if (a.n_cams < 1 || a.cams == nullptr || a.call == nullptr) return hipErrorInvalidValue;
+  if (a.tu.block < 64 || a.tu.block > kInstantBlock || a.tu.block % 64 != 0) return hipErrorInvalidValue;
+  if (a.seg_capacity < 0 || a.run_capacity < 0 || a.groups < 1 || a.groups > kMaxGroups) return hipErrorInvalidValue;
+  return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_chain_prepare(const ChainArgs& a, hipStream_t stream) {
+  const hipError_t e = chain_check(a);
+  if (e != hipSuccess) return e;
+  const dim3 run_grid = grid_for(static_cast<size_t>(vmax<int64_t>(a.run_capacity, 1)), 256);
+  chain_init_kernel<<<run_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call, a.hist, a.queue);
+  if (a.seg_capacity > 0) {
+    const dim3 grid = grid_for(static_cast<size_t>(a.seg_capacity), 256);
+    if (!a.instant) chain_cost_kernel<<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else if (a.groups <= 1) chain_prep_kernel<1><<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else if (a.groups <= 2) chain_prep_kernel<2><<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else if (a.groups <= 4) chain_prep_kernel<4><<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else chain_prep_kernel<8><<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+  }
+  if (a.n_total > 0) {
+    order_count_kernel<<<grid_for(a.n_total, 256), 256, 0, stream>>>(a.cams, a.n_cams, a.n_total, a.key, a.hist);
+    order_offsets_kernel<<<1, 64, 0, stream>>>(a.hist, a.hist + 64);
+    order_scatter_kernel<<<grid_for(a.n_total, 256), 256, 0, stream>>>(a.key, a.n_total, a.hist + 64, a.perm);
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_instant_walk(const CameraDev* cams, int n_cams, uint32_t n_total, int groups, const uint32_t* order,
-                               unsigned int* queue, const WalkTuning& tu, int64_t t_state, int64_t t_end,
-                               hipStream_t stream) {
-  if (n_total == 0) return hipSuccess;
-  const int block = tu.block;
-  dim3 grid = grid_for(n_total, block);
-  if (tu.chunk > 0 && queue != nullptr) {
-    // persistent: as many blocks as are resident at once (at most one per 64-pixel group set)
-    static int resident[4] = {0, 0, 0, 0};
-    const int gi = groups <= 1 ? 0 : groups <= 2 ? 1 : groups <= 4 ? 2 : 3;
-    if (resident[gi] == 0) {
-      int dev = 0, cus = 0, per_cu = 0;
-      (void)hipGetDevice(&dev);
-      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      hipError_t e = hipSuccess;
-      if (gi == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, instant_walk_kernel<1>, block, 0);
-      else if (gi == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, instant_walk_kernel<2>, block, 0);
-      else if (gi == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, instant_walk_kernel<4>, block, 0);
-      else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, instant_walk_kernel<8>, block, 0);
-      if (e != hipSuccess) return e;
-      resident[gi] = vmax(1, cus * vmax(1, per_cu) * 256 / block);
-    }
-    grid = dim3(vmin(grid.x, static_cast<unsigned>(resident[gi])));
-    fill_i32_kernel<<<1, 64, 0, stream>>>(reinterpret_cast<int32_t*>(queue), 1, 0);
+int chain_walk_blocks(const ChainArgs& a) {
+  const int block = a.tu.block;
+  const unsigned full = static_cast<unsigned>((static_cast<uint64_t>(a.n_total) + block - 1) / block);
+  if (a.tu.chunk <= 0 || a.queue == nullptr) return static_cast<int>(full);
+  return static_cast<int>(vmin(full, static_cast<unsigned>(resident_blocks(a.instant, group_tier(a.groups), block))));
+}
+
+hipError_t launch_chain_walk(const ChainArgs& a, hipStream_t stream) {
+  const hipError_t e = chain_check(a);
+  if (e != hipSuccess) return e;
+  if (a.n_total == 0) return hipSuccess;
+  const int block = a.tu.block, tier = group_tier(a.groups);
+  const dim3 grid(static_cast<unsigned>(chain_walk_blocks(a)));
+#define BSX_WALK(K, G) K<G><<<grid, block, 0, stream>>>(a.cams, a.n_cams, a.n_total, a.perm, a.queue, a.tu, a.call)
+  if (a.instant) {
+    if (tier == 0) BSX_WALK(instant_walk_kernel, 1);
+    else if (tier == 1) BSX_WALK(instant_walk_kernel, 2);
+    else if (tier == 2) BSX_WALK(instant_walk_kernel, 4);
+    else BSX_WALK(instant_walk_kernel, 8);
+  } else {
+    if (tier == 0) BSX_WALK(cascade_walk_kernel, 1);
+    else if (tier == 1) BSX_WALK(cascade_walk_kernel, 2);
+    else if (tier == 2) BSX_WALK(cascade_walk_kernel, 4);
+    else BSX_WALK(cascade_walk_kernel, 8);
   }
-#define BSX_WALK(G) \
-  instant_walk_kernel<G><<<grid, block, 0, stream>>>(cams, n_cams, n_total, order, queue, tu, t_state, t_end)
-  if (groups <= 1) BSX_WALK(1);
-  else if (groups <= 2) BSX_WALK(2);
-  else if (groups <= 4) BSX_WALK(4);
-  else BSX_WALK(8);
 #undef BSX_WALK
   return hipGetLastError();
 }
 
-hipError_t launch_run_cost(const KConst& kc, const MapsDev& mp, const StateDev& in, const RunsDev& runs,
-                           const uint32_t* pixel, const int64_t* pos, int64_t n_runs, int64_t total, float* work,
-                           hipStream_t stream) {
-  if (n_runs <= 0) return hipSuccess;
-  fill_i32_kernel<<<grid_for(static_cast<size_t>(n_runs), 256), 256, 0, stream>>>(reinterpret_cast<int32_t*>(work),
-                                                                                  n_runs, 0);
-  if (total > 0)
-    run_cost_kernel<<<grid_for(static_cast<size_t>(total), 256), 256, 0, stream>>>(kc, mp, in, runs, pixel, pos,
-                                                                                   n_runs, total, work);
-  return hipGetLastError();
-}
-
-hipError_t launch_cascade(const KConst& kc, const MapsDev& mp, const StateDev& in, const StateDev& out,
-                          const RunsDev& runs, int64_t t_state, int64_t t_end, RawEventDev* ev, unsigned long long cap,
-                          unsigned long long* counters, const uint32_t* order, int block, hipStream_t stream) {
-  const dim3 grid = grid_for(kc.n_pixels, block);
-#define BSX_CAS(G) \
-  cascade_kernel<G><<<grid, block, 0, stream>>>(kc, mp, in, out, runs, t_state, t_end, ev, cap, counters, order)
-  if (kc.groups <= 1) BSX_CAS(1);
-  else if (kc.groups <= 2) BSX_CAS(2);
-  else if (kc.groups <= 4) BSX_CAS(4);
-  else BSX_CAS(8);
-#undef BSX_CAS
-  return hipGetLastError();
-}
-
-hipError_t launch_cost_order(const CameraDev* cams, int n_cams, uint32_t n_total, uint8_t* key, uint32_t* hist,
-                             uint32_t* perm, hipStream_t stream) {
-  if (n_total == 0) return hipSuccess;
-  const int block = 256;
-  fill_i32_kernel<<<1, 128, 0, stream>>>(reinterpret_cast<int32_t*>(hist), 128, 0);
-  order_count_kernel<<<grid_for(n_total, block), block, 0, stream>>>(cams, n_cams, n_total, key, hist);
-  order_offsets_kernel<<<1, 64, 0, stream>>>(hist, hist + 64);
-  order_scatter_kernel<<<grid_for(n_total, block), block, 0, stream>>>(key, n_total, hist + 64, perm);
+hipError_t launch_chain_reset(const ChainArgs& a, hipStream_t stream) {
+  const hipError_t e = chain_check(a);
+  if (e != hipSuccess) return e;
+  chain_reset_kernel<<<1, 256, 0, stream>>>(a.cams, a.n_cams, a.queue);
   return hipGetLastError();
 }
 

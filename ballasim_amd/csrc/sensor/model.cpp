@@ -7,6 +7,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "chain.h"
 #include "kernels.h"
 
 namespace bsx {
@@ -123,6 +124,7 @@ SensorModel::SensorModel(const SensorParams& params, uint32_t width, uint32_t he
   d_flags_.ensure(64);
   hip_check(hipEventCreate(&ev0_), "hipEventCreate");
   hip_check(hipEventCreate(&ev1_), "hipEventCreate");
+  chain_ = std::make_unique<LaunchChain>();
 }
 
 SensorModel::~SensorModel() {
@@ -324,11 +326,9 @@ void SensorModel::advance_runs(const uint32_t* ids, size_t n, const int64_t* t, 
   hip_check(hipMemcpy(d_row_of_.get(), row_of.data(), P_ * 4, hipMemcpyHostToDevice), "upload row_of");
   d_ids_.ensure(std::max<size_t>(n, 1) * 4);
   if (n > 0) hip_check(hipMemcpy(d_ids_.get(), ids, n * 4, hipMemcpyHostToDevice), "upload ids");
-  run_advance(runs_view(), d_ids_.as<const uint32_t>(), n, t_end_ns, nullptr);
-  t_state_ = t_end_ns;
+  run_advance(runs_view(), d_ids_.as<const uint32_t>(), n, t_end_ns, nullptr, false);
   initialized_ = true;
-  unsigned long long count = 0;
-  hip_check(hipMemcpy(&count, d_counters_.get(), 8, hipMemcpyDeviceToHost), "read count");
+  const unsigned long long count = counters_host_[0];
   raw_host_.resize(count);
   if (count > 0) {
     hip_check(hipMemcpy(raw_host_.data(), d_events_.get(), count * sizeof(RawEventDev), hipMemcpyDeviceToHost),
@@ -347,99 +347,56 @@ RunsDev SensorModel::runs_view() const {
   return r;
 }
 
-void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t t_end,
-                               hipStream_t stream) {
-  const bool instant = params_.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS;
+void SensorModel::enqueue_scan(const RunsDev& runs, size_t n_runs, hipStream_t stream) {
+  d_pos_.ensure(std::max<size_t>(n_runs, 1) * 8);
+  scan_total_ = 0;
+  if (n_runs == 0) return;
+  d_scan_.ensure(scan_scratch_count(n_runs) * 8);
+  hip_check(launch_exclusive_scan(runs.row_count, n_runs, d_pos_.as<int64_t>(), d_scan_.as<int64_t>(), stream),
+            "scan runs");
+  hip_check(hipMemcpyAsync(&scan_total_, d_scan_.as<int64_t>() + scan_scratch_count(n_runs) - 1, 8,
+                           hipMemcpyDeviceToHost, stream),
+            "read total");
+}
+
+CameraDev SensorModel::stage(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t total) {
+  const bool inst = instant();
   if (cap_ == 0) cap_ = std::max<unsigned long long>(1ull << 20, 2 * P_);
   d_events_.ensure(cap_ * sizeof(RawEventDev));
-  const KConst kc = kconst();
-  const MapsDev mp = maps_view(d_maps_, P_);
-  // The runs' segments laid end to end: pos = exclusive scan of the run lengths.
-  int64_t total = 0;
-  d_pos_.ensure(std::max<size_t>(n_runs, 1) * 8);
-  if (n_runs > 0) {
-    d_scan_.ensure(scan_scratch_count(n_runs) * 8);
-    hip_check(launch_exclusive_scan(runs.row_count, n_runs, d_pos_.as<int64_t>(), d_scan_.as<int64_t>(), stream),
-              "scan runs");
-    hip_check(hipMemcpyAsync(&total, d_scan_.as<int64_t>() + scan_scratch_count(n_runs) - 1, 8,
-                             hipMemcpyDeviceToHost, stream),
-              "read total");
-    hip_check(hipStreamSynchronize(stream), "scan runs");
-  }
-  if (instant) d_prep_.ensure(std::max<int64_t>(total, 1) * sizeof(InstantPrep));
+  if (inst) d_prep_.ensure(std::max<int64_t>(total, 1) * sizeof(InstantPrep));
   d_work_.ensure(std::max<size_t>(n_runs, 1) * sizeof(float));
   d_loud_.ensure(std::max<size_t>(n_runs, 1) * sizeof(int));
-  d_key_.ensure(P_);
-  d_perm_.ensure(P_ * sizeof(uint32_t));
-  d_order_hist_.ensure(128 * sizeof(uint32_t));
-  d_queue_.ensure(64);
-  // The walk's camera descriptor (the event buffer and its capacity are refreshed per attempt).
   CameraDev cd{};
-  cd.kc = kc;
-  cd.mp = mp;
+  cd.kc = kconst();
+  cd.mp = maps_view(d_maps_, P_);
   cd.in = state_view(cur_);
   cd.out = state_view(1 - cur_);
   cd.runs = runs;
+  cd.pixel = pixel;
   cd.pos = d_pos_.as<const int64_t>();
-  cd.prep = instant && total > 0 ? d_prep_.as<const InstantPrep>() : nullptr;
-  cd.first_loud = instant && n_runs > 0 ? d_loud_.as<const int>() : nullptr;
-  cd.work = n_runs > 0 ? d_work_.as<const float>() : nullptr;
+  cd.prep = inst ? d_prep_.as<InstantPrep>() : nullptr;
+  cd.first_loud = inst ? d_loud_.as<int>() : nullptr;
+  cd.work = d_work_.as<float>();
+  cd.ev = d_events_.as<RawEventDev>();
+  cd.cap = cap_;
   cd.counters = d_counters_.as<unsigned long long>();
-  cd.base = 0;
-  d_cams_.ensure(sizeof(CameraDev));
-  const CameraDev* cams = d_cams_.as<const CameraDev>();
-  unsigned long long c[1 + kStCount];
-  bool prepared = false;
-  while (true) {
-    cd.ev = d_events_.as<RawEventDev>();
-    cd.cap = cap_;
-    host_cam_ = cd;
-    hip_check(hipMemcpyAsync(d_cams_.get(), &host_cam_, sizeof(CameraDev), hipMemcpyHostToDevice, stream),
-              "upload camera");
-    hip_check(launch_fill_u64(d_counters_.as<unsigned long long>(), 1 + kStCount, 0, stream), "fill counters");
-    hip_check(hipEventRecord(ev0_, stream), "event record");
-    if (!prepared) {  // phase A and the pixel order read only the call's input: a re-run reuses them
-      hip_check(launch_fill_f32(d_work_.as<float>(), n_runs, 0.0f, stream), "fill work");
-      if (instant) {
-        hip_check(launch_instant_prep(kc, mp, state_view(cur_), runs, pixel, d_pos_.as<const int64_t>(),
-                                      static_cast<int64_t>(n_runs), total, t_state_, d_prep_.as<InstantPrep>(),
-                                      d_work_.as<float>(), d_loud_.as<int>(), stream),
-                  "instant prep kernel");
-      } else {
-        hip_check(launch_run_cost(kc, mp, state_view(cur_), runs, pixel, d_pos_.as<const int64_t>(),
-                                  static_cast<int64_t>(n_runs), total, d_work_.as<float>(), stream),
-                  "run cost kernel");
-      }
-      hip_check(launch_cost_order(cams, 1, static_cast<uint32_t>(P_), d_key_.as<uint8_t>(),
-                                  d_order_hist_.as<uint32_t>(), d_perm_.as<uint32_t>(), stream),
-                "pixel order");
-      prepared = true;
-    }
-    if (instant) {
-      WalkTuning tu = tuning_;
-      tu.block = block_;
-      hip_check(launch_instant_walk(cams, 1, static_cast<uint32_t>(P_), G_, d_perm_.as<const uint32_t>(),
-                                    d_queue_.as<unsigned int>(), tu, t_state_, t_end, stream),
-                "instant walk");
-    } else {
-      hip_check(launch_cascade(kc, mp, state_view(cur_), state_view(1 - cur_), runs, t_state_, t_end,
-                               d_events_.as<RawEventDev>(), cap_, d_counters_.as<unsigned long long>(),
-                               d_perm_.as<const uint32_t>(), block_, stream),
-                "cascade kernel");
-    }
-    hip_check(hipEventRecord(ev1_, stream), "event record");
-    hip_check(hipMemcpyAsync(c, d_counters_.get(), sizeof(c), hipMemcpyDeviceToHost, stream), "read counters");
-    hip_check(hipStreamSynchronize(stream), "advance");
-    float ms = 0.0f;
-    hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event time");
-    stats_.kernel_ms += ms;
-    ++stats_.launches;
-    if (c[0] <= cap_) break;
-    cap_ = std::max<unsigned long long>(cap_ * 2, c[0] + c[0] / 4);
-    d_events_.ensure(cap_ * sizeof(RawEventDev));
-    ++stats_.overflow_reruns;
-  }
+  cd.t_state = t_state_;
+  cd.n_runs = static_cast<int64_t>(n_runs);
+  cd.total = total;
+  return cd;
+}
+
+void SensorModel::grow_events(unsigned long long need, CameraDev& cd) {
+  cap_ = std::max<unsigned long long>(cap_ * 2, need + need / 4);
+  d_events_.ensure(cap_ * sizeof(RawEventDev));
+  cd.ev = d_events_.as<RawEventDev>();
+  cd.cap = cap_;
+  ++stats_.overflow_reruns;
+}
+
+void SensorModel::commit(const unsigned long long* c, int64_t t_end) {
   cur_ = 1 - cur_;
+  t_state_ = t_end;
   stats_.events += c[0];
   stats_.substeps += c[1 + kStSubsteps];
   stats_.substep_attempts += c[1 + kStAttempts];
@@ -452,7 +409,34 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
   stats_.pieces += c[1 + kStPieces];
 }
 
-bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream) {
+void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t t_end,
+                               hipStream_t stream, bool scanned) {
+  if (!scanned) {
+    enqueue_scan(runs, n_runs, stream);
+    hip_check(hipStreamSynchronize(stream), "scan runs");
+  }
+  std::vector<CameraDev> cams{stage(runs, pixel, n_runs, scan_total_)};
+  unsigned long long* c = counters_host_;
+  hip_check(hipEventRecord(ev0_, stream), "event record");
+  chain_->launch(cams, instant(), t_end, stream);
+  while (true) {
+    hip_check(hipEventRecord(ev1_, stream), "event record");
+    hip_check(hipMemcpyAsync(c, d_counters_.get(), sizeof(counters_host_), hipMemcpyDeviceToHost, stream),
+              "read counters");
+    hip_check(hipStreamSynchronize(stream), "advance");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event time");
+    stats_.kernel_ms += ms;
+    ++stats_.launches;
+    if (c[0] <= cap_) break;
+    grow_events(c[0], cams[0]);
+    hip_check(hipEventRecord(ev0_, stream), "event record");
+    chain_->rerun_walk(cams, stream);
+  }
+  commit(c, t_end);
+}
+
+void SensorModel::enqueue_device_checks(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream) {
   if (!initialized_) {
     throw std::logic_error(
         "SensorModel.advance: call init() or load_state() first (a call that failed part-way leaves the model "
@@ -467,16 +451,29 @@ bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64
   unsigned long long* chk = reinterpret_cast<unsigned long long*>(d_flags_.as<char>() + 16);
   hip_check(launch_fill_u64(chk, 2, 0, stream), "fill");
   hip_check(launch_check_samples(runs.irr, runs.n_samples * G_, irr_max_, chk, stream), "check samples");
-  unsigned int f[4];
-  hip_check(hipMemcpyAsync(f, d_flags_.get(), 16, hipMemcpyDeviceToHost, stream), "read flags");
-  unsigned long long cc[2];
-  hip_check(hipMemcpyAsync(cc, chk, 16, hipMemcpyDeviceToHost, stream), "read checks");
+  hip_check(hipMemcpyAsync(check_flags_, d_flags_.get(), 16, hipMemcpyDeviceToHost, stream), "read flags");
+  hip_check(hipMemcpyAsync(check_samples_, chk, 16, hipMemcpyDeviceToHost, stream), "read checks");
+  // The scan reads row_count of runs the scatter may have refused; its result is used only if every
+  // check passes.
+  RunsDev r{};
+  r.row_count = runs.row_count;
+  enqueue_scan(r, runs.n, stream);
+}
+
+void SensorModel::finish_device_checks() {
+  if (check_flags_[0] != 0) throw std::invalid_argument("SensorModel.advance: pixel id out of range");
+  if (check_flags_[1] != 0) throw std::invalid_argument("SensorModel.advance: duplicate pixel id");
+  if (check_flags_[2] != 0) throw std::invalid_argument("SensorModel.advance: sample run out of range");
+  if (check_samples_[0] != 0) {
+    throw std::invalid_argument("SensorModel.advance_device: non-finite irradiance in the runs");
+  }
+}
+
+bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream) {
+  enqueue_device_checks(runs, t_end_ns, stream);
   hip_check(hipStreamSynchronize(stream), "checks");
-  if (f[0] != 0) throw std::invalid_argument("SensorModel.advance: pixel id out of range");
-  if (f[1] != 0) throw std::invalid_argument("SensorModel.advance: duplicate pixel id");
-  if (f[2] != 0) throw std::invalid_argument("SensorModel.advance: sample run out of range");
-  if (cc[0] != 0) throw std::invalid_argument("SensorModel.advance_device: non-finite irradiance in the runs");
-  stats_.input_clamps += cc[1];
+  finish_device_checks();
+  stats_.input_clamps += check_samples_[1];
   RunsDev r{};
   r.row_of = d_row_of_.as<const int32_t>();
   r.row_start = runs.row_start;
@@ -484,12 +481,10 @@ bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64
   r.t = runs.t;
   r.irr = runs.irr;
   initialized_ = false;
-  run_advance(r, runs.pixel, runs.n, t_end_ns, stream);
-  t_state_ = t_end_ns;
+  run_advance(r, runs.pixel, runs.n, t_end_ns, stream, true);
   initialized_ = true;
-  unsigned long long count = 0;
-  hip_check(hipMemcpy(&count, d_counters_.get(), 8, hipMemcpyDeviceToHost), "read count");
-  return bsx_device_events{reinterpret_cast<const bsx_raw_event*>(d_events_.get()), static_cast<size_t>(count)};
+  return bsx_device_events{reinterpret_cast<const bsx_raw_event*>(d_events_.get()),
+                           static_cast<size_t>(counters_host_[0])};
 }
 
 void SensorModel::download_events(const bsx_device_events& ev) {
@@ -639,17 +634,19 @@ void SensorModel::load_state(const uint8_t* data, size_t n) {
 void SensorModel::reset_stats() { stats_ = bsx_stats{}; }
 
 void SensorModel::set_policy(const bsx_launch_policy& p) {
+  WalkTuning tu = chain_->tuning();
   if (p.block_size != 0 && (p.block_size % 64 != 0 || p.block_size < 64 || p.block_size > 256)) {
     throw std::invalid_argument("set_launch_policy: block_size must be a multiple of 64 in [64, 256]");
   }
   if (p.walk_budget < 0 || p.walk_budget > 1024) throw std::invalid_argument("set_launch_policy: walk_budget in [0, 1024]");
   if (p.search_batch < 0 || p.search_batch > 64) throw std::invalid_argument("set_launch_policy: search_batch in [0, 64]");
   if (p.queue_chunk < -1 || p.queue_chunk > 64) throw std::invalid_argument("set_launch_policy: queue_chunk in [-1, 64]");
+  if (p.block_size != 0) tu.block = p.block_size;
+  if (p.walk_budget != 0) tu.walk_budget = p.walk_budget;
+  if (p.search_batch != 0) tu.search_batch = p.search_batch;
+  if (p.queue_chunk != 0) tu.chunk = p.queue_chunk < 0 ? 0 : p.queue_chunk;
+  chain_->set_tuning(tu);
   if (p.event_capacity != 0) cap_ = p.event_capacity;
-  if (p.block_size != 0) block_ = p.block_size;
-  if (p.walk_budget != 0) tuning_.walk_budget = p.walk_budget;
-  if (p.search_batch != 0) tuning_.search_batch = p.search_batch;
-  if (p.queue_chunk != 0) tuning_.chunk = p.queue_chunk < 0 ? 0 : p.queue_chunk;
 }
 
 bsx_sensor_info_t SensorModel::info() const {

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,9 @@
 #include "params.h"
 
 namespace bsx {
+
+class LaunchChain;
+class Rig;
 
 // Grow-only device allocation.
 class DeviceBuffer {
@@ -97,19 +101,37 @@ class SensorModel {
   const bsx_stats& stats() const { return stats_; }
   void reset_stats();
   void set_policy(const bsx_launch_policy& p);
+  LaunchChain& chain() { return *chain_; }
   bsx_sensor_info_t info() const;
   const SensorMaps& maps() const { return maps_; }
   const SensorParams& params() const { return params_; }
 
  private:
+  friend class Rig;
   void upload_maps();
   StateDev state_view(int which) const;
   RunsDev runs_view() const;
-  // The launch loop of one call on device runs (row_of built), with overflow re-runs.
-  // pixel: the runs' pixel ids in device memory (run i is pixel[i]), n_runs of them.
-  void run_advance(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t t_end, hipStream_t stream);
+  // The checks of device runs (ids, run bounds, finite samples) and the scan of the run lengths,
+  // queued on stream; their results are read by finish_device_checks after the stream is synchronized.
+  void enqueue_device_checks(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream);
+  // Throws std::invalid_argument for the first failed check (before any state change).
+  void finish_device_checks();
+  // This camera's descriptor for a launch of the chain over the runs (row_of built, pos scanned:
+  // total segments), its buffers sized for them.
+  CameraDev stage(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t total);
+  // Queues the scan of the run lengths into d_pos_ and the total's read into scan_total_.
+  void enqueue_scan(const RunsDev& runs, size_t n_runs, hipStream_t stream);
+  // Grows the event buffer after an overflow to hold `need` events (the descriptor's ev and cap).
+  void grow_events(unsigned long long need, CameraDev& cd);
+  // The end of a call whose chain completed: the written state becomes current, the counters are
+  // added to the statistics.
+  void commit(const unsigned long long* counters, int64_t t_end);
+  // One call on device runs (row_of built, checks passed), with overflow re-runs.
+  void run_advance(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t t_end, hipStream_t stream,
+                   bool scanned);
   void finish_host_events(const RawEventDev* raw, size_t n);
   KConst kconst() const;
+  bool instant() const { return params_.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS; }
 
   SensorParams params_;
   uint32_t width_, height_;
@@ -133,14 +155,16 @@ class SensorModel {
   DeviceBuffer d_row_of_;
   DeviceBuffer d_t_, d_irr_, d_row_start_, d_row_count_, d_frame_, d_ids_;
   DeviceBuffer d_pos_, d_scan_, d_prep_;  // the runs' segment positions and the prepared segment values
-  DeviceBuffer d_work_, d_key_, d_perm_, d_order_hist_;  // estimated run costs and the heavy-first pixel order
+  DeviceBuffer d_work_;  // per run: estimated walk cost (the heavy-first order's key)
   DeviceBuffer d_loud_;  // per run: the first segment that is not quiet under the input state
-  DeviceBuffer d_cams_, d_queue_;  // the walk's camera descriptor and the persistent queue's counter
-  CameraDev host_cam_{};           // the descriptor's host copy (source of its upload)
-  WalkTuning tuning_{256, 8, 8, 0};
+  std::unique_ptr<LaunchChain> chain_;
+  // Host copies of a device call's check words and scan total (written by queued copies).
+  unsigned int check_flags_[4] = {0, 0, 0, 0};
+  unsigned long long check_samples_[2] = {0, 0};
+  int64_t scan_total_ = 0;
+  unsigned long long counters_host_[1 + kStCount] = {};
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 
-  int block_ = 256;
   int64_t t_state_ = 0;
   bool initialized_ = false;
   bsx_stats stats_{};

@@ -293,3 +293,100 @@ class SensorModel:
     irr_max = property(lambda self: self._info().irr_max)
     precision = property(lambda self: "fp32")
     device = property(lambda self: "cuda")
+
+
+def _device_runs(r):
+    """A bsx_device_runs from a DeviceRuns struct or a (pixel, row_start, row_count, t, irr) tuple of
+    device tensors / raw addresses."""
+    if isinstance(r, _abi.DeviceRuns):
+        return r
+    pixel, row_start, row_count, t, irr = r
+
+    def addr(x):
+        return x.data_ptr() if hasattr(x, "data_ptr") else int(x)
+
+    def numel(x):
+        return x.numel() if hasattr(x, "numel") else None
+
+    return _abi.DeviceRuns(addr(pixel), addr(row_start), addr(row_count), numel(pixel), addr(t), addr(irr), numel(t))
+
+
+class Rig:
+    """Several cameras (SensorModel, device 'cuda', one photoreceptor kind) advanced by one launch
+    chain: phase A over every camera's segments, one heavy-first order of the joint pixel space and
+    one walk launch over all pixels, captured into a hipGraph and replayed. Each camera's events,
+    state and statistics equal those of advancing it alone.
+
+    It replaces the reference engine's loop of per-camera advance_device calls on one stream
+    (src/engine/engine.h:1-8, src/engine/engine_impl.h:1343-1348). The cameras are borrowed: keep
+    them alive while the rig is used."""
+
+    def __init__(self, cameras):
+        cameras = list(cameras)
+        self._lib = _abi.load_library()
+        self._cams = cameras
+        hs = (C.c_void_p * len(cameras))(*[c._h.value if c._h else None for c in cameras])
+        self._h = C.c_void_p()
+        _abi.check(self._lib.bsx_rig_create(hs, len(cameras), C.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.bsx_rig_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self._cams)
+
+    def _runs(self, runs):
+        if len(runs) != len(self._cams):
+            raise ValueError(f"Rig: need runs for each of the {len(self._cams)} cameras")
+        arr = (_abi.DeviceRuns * len(runs))()
+        for i, r in enumerate(runs):
+            arr[i] = _device_runs(r)
+        return arr
+
+    def advance_device(self, runs, t_end_ns, stream=0):
+        """Advances camera c on runs[c] to t_end_ns; returns [(data_ptr, count)] of each camera's
+        unordered raw events in device memory (download with that camera's download_events)."""
+        arr = self._runs(runs)
+        out = (_abi.DeviceEvents * len(runs))()
+        st = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
+        _abi.check(self._lib.bsx_rig_advance_device(self._h, arr, int(t_end_ns), C.c_void_p(st), out))
+        return [(o.data, o.count) for o in out]
+
+    def set_tuning(self, block_size=0, walk_budget=0, search_batch=0, queue_chunk=0):
+        pol = _abi.LaunchPolicy(0, block_size, walk_budget, search_batch, queue_chunk)
+        _abi.check(self._lib.bsx_rig_set_tuning(self._h, C.byref(pol)))
+
+    def tuning(self):
+        pol = _abi.LaunchPolicy()
+        _abi.check(self._lib.bsx_rig_get_tuning(self._h, C.byref(pol)))
+        return {k: getattr(pol, k) for k in ("block_size", "walk_budget", "search_batch", "queue_chunk")}
+
+    def set_graph(self, on):
+        _abi.check(self._lib.bsx_rig_set_graph(self._h, 1 if on else 0))
+
+    def stats(self):
+        s = _abi.RigStats()
+        _abi.check(self._lib.bsx_rig_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in _abi.RigStats._fields_}
+
+    def autotune(self, runs, t_end_ns, stream=0, reps=3):
+        """Times the walk of this call under candidate launch shapes and keeps the fastest (no camera
+        changes). Returns the trials as [(tuning dict, walk_ms)]."""
+        arr = self._runs(runs)
+        cap = 256
+        tun = (_abi.LaunchPolicy * cap)()
+        ms = (C.c_double * cap)()
+        n = C.c_int32()
+        st = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
+        _abi.check(self._lib.bsx_rig_autotune(self._h, arr, int(t_end_ns), C.c_void_p(st), int(reps), tun, ms, cap,
+                                              C.byref(n)))
+        keys = ("block_size", "walk_budget", "search_batch", "queue_chunk")
+        return [({k: getattr(tun[i], k) for k in keys}, ms[i]) for i in range(min(n.value, cap))]

@@ -28,8 +28,8 @@ S_STATE, S_MAP, S_EVENT = 48, 16, 16  # the fixed accounting's per-pixel state, 
 # The kernels of one advance call (launched back to back on the call's stream and timed together
 # with HIP events around them: the roofline's "launch").
 ADVANCE_CHAIN = {
-    "instantaneous": "advance chain: instant_prep_kernel + order_count/offsets/scatter + instant_walk_kernel",
-    "cascade": "advance chain: run_cost_kernel + order_count/offsets/scatter + cascade_kernel",
+    "instantaneous": "advance chain: chain_init_kernel + chain_prep_kernel + order_count/offsets/scatter + instant_walk_kernel",
+    "cascade": "advance chain: chain_init_kernel + chain_cost_kernel + order_count/offsets/scatter + cascade_walk_kernel",
 }
 # Measured HBM traffic of the instantaneous chain on this workload, bytes per sample: rocprofv3
 # FETCH_SIZE x 2 + WRITE_SIZE summed over the chain's kernels, averaged over the timed launches (the
