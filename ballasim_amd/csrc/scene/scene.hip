@@ -121,6 +121,32 @@ __global__ __launch_bounds__(256) void write_kernel(SceneDev sc, ViewDev view, R
   sample_pixel<true>(sc, view, r, t_w, p, t + s, irr + s);
 }
 
+// Samples at imposed per-pixel counts: pixel p's n = counts[p] samples at t_w + floor((k + 1) W / n),
+// k = 0 .. n - 1 (evenly spread, the last at the window's end), one thread per sample (its pixel by
+// bisection of the scanned offsets), every shape of the scene evaluated.
+__global__ __launch_bounds__(256) void counts_write_kernel(SceneDev sc, ViewDev view, int64_t t_w, const int32_t* counts,
+                                                           const int64_t* row_start, uint32_t n_pixels, int64_t total,
+                                                           int64_t* t, float* irr) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= total) return;
+  uint32_t lo = 0, hi = n_pixels;  // the last pixel whose run starts at or before s
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (row_start[mid] <= s) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t p = lo;
+  const int64_t k = s - row_start[p], n = counts[p];
+  const int64_t dt = (k + 1) * kWindowNs / n;
+  const double x = view.ox + (static_cast<double>(p % view.width) + 0.5) * view.scale;
+  const double y = view.oy + (static_cast<double>(p / view.width) + 0.5) * view.scale;
+  uint64_t mask[kMaskWords];
+  for (int w = 0; w < kMaskWords; ++w) mask[w] = 0;
+  for (int i = 0; i < sc.n; ++i) mask[i >> 6] |= 1ull << (i & 63);
+  t[s] = t_w + dt;
+  irr[s] = eval_scene(sc, mask, x, y, static_cast<double>(t_w) * 1e-9 + static_cast<double>(dt) * 1e-9).e;
+}
+
 // Exclusive scan of int32 counts into int64 offsets: per-block scans (wave shuffles, then the
 // block's wave totals through LDS), the block totals scanned by one block, then added back.
 constexpr int kScanBlock = 1024;
@@ -219,6 +245,14 @@ hipError_t launch_write(const SceneDev& sc, const ViewDev& view, const bsx_sampl
   const size_t P = static_cast<size_t>(view.width) * view.height;
   write_kernel<<<static_cast<unsigned>((P + 255) / 256), 256, 0, stream>>>(sc, view, rules_dev(r), t_w, row_start, t,
                                                                           irr);
+  return hipGetLastError();
+}
+
+hipError_t launch_counts_write(const SceneDev& sc, const ViewDev& view, int64_t t_w, const int32_t* counts,
+                               const int64_t* row_start, int64_t total, int64_t* t, float* irr, hipStream_t stream) {
+  if (total <= 0) return hipSuccess;
+  counts_write_kernel<<<static_cast<unsigned>((total + 255) / 256), 256, 0, stream>>>(
+      sc, view, t_w, counts, row_start, view.width * view.height, total, t, irr);
   return hipGetLastError();
 }
 

@@ -74,6 +74,67 @@ class Scene {
     hip_check(hipMemcpy(out, frame_.get(), P * 4, hipMemcpyDeviceToHost), "download frame");
   }
 
+  WindowSlot& slot_at(int32_t slot) {
+    if (slot < 0 || slot > 100000) throw std::invalid_argument("sample_window: slot out of range");
+    if (static_cast<size_t>(slot) >= slots_.size()) slots_.resize(static_cast<size_t>(slot) + 1);
+    if (!slots_[slot]) slots_[slot] = std::make_unique<WindowSlot>();
+    return *slots_[slot];
+  }
+
+  // The pixel ids 0 .. P - 1 of a view's runs. Runs handed out earlier keep pointing at the buffer they
+  // were given, so a larger view gets a new buffer and the old ones stay alive with the scene.
+  void ensure_ids(size_t P) {
+    if (ids_.empty() || ids_.back()->bytes() < P * 4) {
+      std::vector<uint32_t> iota(P);
+      for (size_t i = 0; i < P; ++i) iota[i] = static_cast<uint32_t>(i);
+      ids_.push_back(std::make_unique<DeviceBuffer>());
+      ids_.back()->ensure(P * 4);
+      hip_check(hipMemcpy(ids_.back()->get(), iota.data(), P * 4, hipMemcpyHostToDevice), "ids");
+    }
+  }
+
+  bsx_device_runs runs_of(WindowSlot& w, size_t P, int64_t total) const {
+    bsx_device_runs out{};
+    out.pixel = ids_.back()->as<const uint32_t>();
+    out.row_start = w.row_start.as<const int64_t>();
+    out.row_count = w.counts.as<const int32_t>();
+    out.n = P;
+    out.t = w.t.as<const int64_t>();
+    out.irr = w.irr.as<const float>();
+    out.n_samples = static_cast<size_t>(total);
+    return out;
+  }
+
+  bsx_device_runs sample_counts(const bsx_view& view, const int32_t* counts, int64_t t_w, int32_t slot,
+                                hipStream_t stream, uint64_t* total_out) {
+    const ViewDev v = view_of(view);
+    if ((t_w & (kWindowNs - 1)) != 0) throw std::invalid_argument("sample_counts: t_w must be a multiple of 2^23 ns");
+    const size_t P = static_cast<size_t>(v.width) * v.height;
+    int64_t total = 0;
+    for (size_t p = 0; p < P; ++p) {
+      if (counts[p] < 0 || counts[p] > kWindowNs) {
+        throw std::invalid_argument("sample_counts: a pixel's count must lie in [0, 2^23]");
+      }
+      total += counts[p];
+    }
+    WindowSlot& w = slot_at(slot);
+    w.counts.ensure(P * 4);
+    w.row_start.ensure(P * 8);
+    scan_.ensure(scan_scratch_count(P) * 8);
+    ensure_ids(P);
+    w.t.ensure(std::max<int64_t>(total, 1) * 8);
+    w.irr.ensure(std::max<int64_t>(total, 1) * 4);
+    hip_check(hipMemcpyAsync(w.counts.get(), counts, P * 4, hipMemcpyHostToDevice, stream), "counts");
+    hip_check(launch_exclusive_scan(w.counts.as<int32_t>(), P, w.row_start.as<int64_t>(), scan_.as<int64_t>(), stream),
+              "scan");
+    hip_check(launch_counts_write(dev(), v, t_w, w.counts.as<const int32_t>(), w.row_start.as<const int64_t>(), total,
+                                  w.t.as<int64_t>(), w.irr.as<float>(), stream),
+              "write samples");
+    hip_check(hipStreamSynchronize(stream), "sample counts");
+    if (total_out != nullptr) *total_out = static_cast<uint64_t>(total);
+    return runs_of(w, P, total);
+  }
+
   bsx_device_runs sample(const bsx_view& view, const bsx_sampler_rules& r, int64_t t_w, int32_t slot,
                          hipStream_t stream, uint64_t* total_out) {
     const ViewDev v = view_of(view);
@@ -86,20 +147,12 @@ class Scene {
     if (r.fixed_period_ns > 0 && r.fixed_period_ns < 64) {
       throw std::invalid_argument("sample_window: fixed_period_ns must be >= 64");
     }
-    if (slot < 0 || slot > 100000) throw std::invalid_argument("sample_window: slot out of range");
-    if (static_cast<size_t>(slot) >= slots_.size()) slots_.resize(static_cast<size_t>(slot) + 1);
-    if (!slots_[slot]) slots_[slot] = std::make_unique<WindowSlot>();
-    WindowSlot& w = *slots_[slot];
+    WindowSlot& w = slot_at(slot);
     const size_t P = static_cast<size_t>(v.width) * v.height;
     w.counts.ensure(P * 4);
     w.row_start.ensure(P * 8);
     scan_.ensure(scan_scratch_count(P) * 8);
-    if (ids_.bytes() < P * 4) {
-      std::vector<uint32_t> iota(P);
-      for (size_t i = 0; i < P; ++i) iota[i] = static_cast<uint32_t>(i);
-      ids_.ensure(P * 4);
-      hip_check(hipMemcpy(ids_.get(), iota.data(), P * 4, hipMemcpyHostToDevice), "ids");
-    }
+    ensure_ids(P);
     hip_check(launch_count(dev(), v, r, t_w, w.counts.as<int32_t>(), stream), "count samples");
     hip_check(launch_exclusive_scan(w.counts.as<int32_t>(), P, w.row_start.as<int64_t>(), scan_.as<int64_t>(), stream),
               "scan");
@@ -113,21 +166,14 @@ class Scene {
               "write samples");
     hip_check(hipStreamSynchronize(stream), "sample window");
     if (total_out != nullptr) *total_out = static_cast<uint64_t>(total);
-    bsx_device_runs out{};
-    out.pixel = ids_.as<const uint32_t>();
-    out.row_start = w.row_start.as<const int64_t>();
-    out.row_count = w.counts.as<const int32_t>();
-    out.n = P;
-    out.t = w.t.as<const int64_t>();
-    out.irr = w.irr.as<const float>();
-    out.n_samples = static_cast<size_t>(total);
-    return out;
+    return runs_of(w, P, total);
   }
 
  private:
   int32_t n_;
   float background_;
-  DeviceBuffer shapes_, frame_, ids_, scan_;
+  DeviceBuffer shapes_, frame_, scan_;
+  std::vector<std::unique_ptr<DeviceBuffer>> ids_;
   std::vector<std::unique_ptr<WindowSlot>> slots_;
 };
 
@@ -195,6 +241,16 @@ int bsx_scene_sample_window(bsx_scene* s, const bsx_view* view, const bsx_sample
       throw std::invalid_argument("bsx_scene_sample_window: null");
     }
     *out = s->s->sample(*view, *rules, t_w, slot, static_cast<hipStream_t>(stream), total_samples);
+  });
+}
+
+int bsx_scene_sample_counts(bsx_scene* s, const bsx_view* view, const int32_t* counts, int64_t t_w, int32_t slot,
+                            void* stream, bsx_device_runs* out, uint64_t* total_samples) {
+  return scene_guarded([&] {
+    if (s == nullptr || view == nullptr || counts == nullptr || out == nullptr) {
+      throw std::invalid_argument("bsx_scene_sample_counts: null");
+    }
+    *out = s->s->sample_counts(*view, counts, t_w, slot, static_cast<hipStream_t>(stream), total_samples);
   });
 }
 

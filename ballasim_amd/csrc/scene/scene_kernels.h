@@ -12,6 +12,10 @@ hipError_t launch_count(const SceneDev& sc, const ViewDev& view, const bsx_sampl
                         int32_t* counts, hipStream_t stream);
 hipError_t launch_write(const SceneDev& sc, const ViewDev& view, const bsx_sampler_rules& r, int64_t t_w,
                         const int64_t* row_start, int64_t* t, float* irr, hipStream_t stream);
+// Samples at imposed per-pixel counts (scanned into row_start, total of them), evenly spread over
+// (t_w, t_w + 2^23]; the scene's irradiance at each.
+hipError_t launch_counts_write(const SceneDev& sc, const ViewDev& view, int64_t t_w, const int32_t* counts,
+                               const int64_t* row_start, int64_t total, int64_t* t, float* irr, hipStream_t stream);
 // Scratch of scan_scratch_count(n) int64 values; out[i] = sum of counts[0..i), scratch[blocks] = the total.
 size_t scan_scratch_count(size_t n);
 hipError_t launch_exclusive_scan(const int32_t* counts, size_t n, int64_t* out, int64_t* scratch, hipStream_t stream);

@@ -48,6 +48,20 @@ class Scene:
                                                      C.c_void_p(st), C.byref(runs), C.byref(total)))
         return runs, total.value
 
+    def sample_counts(self, view, counts, t_w, slot=0, stream=0):
+        """Runs over (t_w, t_w + 2^23] at imposed per-pixel counts (int32, width*height), samples
+        evenly spread in the window: (DeviceRuns, samples)."""
+        counts = np.ascontiguousarray(counts, np.int32)
+        if counts.size != view.width * view.height:
+            raise ValueError("sample_counts: one count per pixel")
+        runs = _abi.DeviceRuns()
+        total = C.c_uint64()
+        st = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
+        _abi.check(self._lib.bsx_scene_sample_counts(self._h, C.byref(view), counts.ctypes.data_as(C.c_void_p),
+                                                     int(t_w), int(slot), C.c_void_p(st), C.byref(runs),
+                                                     C.byref(total)))
+        return runs, total.value
+
 
 def runs_to_host(runs, groups=1):
     """A copy of device runs on the host: (ids, t, irr (S, groups), offsets (n + 1))."""
@@ -127,8 +141,16 @@ def config2_scene(seed=0):
 
 
 RIG = [(1280, 720), (640, 480), (346, 260), (1920, 1080)]
+# Scene-plane origins of the rig's cameras (1 scene px per camera px): each sees the config-1 disk's
+# path (around scene point (173, 130)) at another place of its frame.
+RIG_OFFSETS = ((-467.0, -230.0), (-100.0, -50.0), (0.0, 0.0), (-900.0, -400.0))
 
 
-def rig_views(offsets=((0.0, 0.0), (320.0, 120.0), (467.0, 230.0), (-320.0, -180.0))):
+def rig_views(offsets=RIG_OFFSETS):
     """The 4-camera rig of mixed resolutions viewing one scene, each with its own extrinsic offset."""
     return [view(w, h, ox, oy) for (w, h), (ox, oy) in zip(RIG, offsets)]
+
+
+def lognormal_counts(rng, n, mu=np.log(20.0), sigma=1.5, lo=1, hi=4000):
+    """Per-pixel sample counts of the ragged-batch workload: round(exp(N(mu, sigma))) clipped to [lo, hi]."""
+    return np.clip(np.rint(np.exp(rng.normal(mu, sigma, size=n))), lo, hi).astype(np.int32)

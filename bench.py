@@ -11,6 +11,15 @@ device and resident in HBM before the timed region. value = pixel-samples advanc
 The JSON line also carries the roofline of the advance kernel (the fixed algorithmic bytes per
 launch over its HIP-event time, against 8 TB/s) and the reference's CPU path timed on this host's
 cores on a bounded sample of the same runs.
+
+--workload rig: the ragged batch. A 4-camera rig (1280x720, 640x480, 346x260, 1920x1080, each with
+its own offset in the scene plane) watching the 346x260 scene of one textured disk (100 lux
+background, radius 40 px, 8 octaves, contrast 0.5, 800 px/s); per-pixel sample counts log-normal
+(mu ln 20, sigma 1.5, clipped to [1, 4000]), evenly spread over each window; thresholds 0.2 +- 0.02,
+refractory 100 us, noise off. A step advances all four cameras over one window by one graph-captured
+launch chain, after the autotuner has picked the walk's launch shape on the warm-up window; the
+same windows advanced camera by camera (four calls per window, the reference engine's loop) are
+timed beside it.
 """
 import argparse
 import json
@@ -84,12 +93,168 @@ def cpu_baseline(p, window_runs, frame, width, height, budget_s):
             "events_per_s": events / elapsed}
 
 
+def rig_params(bsx, photoreceptor):
+    p = bsx.preset("prophesee_gen3")
+    p.photoreceptor = photoreceptor
+    p.noise = False
+    p.theta_on = p.theta_off = 0.2
+    p.sigma_theta_on = p.sigma_theta_off = 0.02
+    p.refractory_s = 100e-6
+    p.leak_rate_hz = 0.0
+    p.shot_rate_hz = 0.0
+    from ballasim_amd.scene import LUX_TO_FA
+    p.illuminance_scale = LUX_TO_FA
+    return p
+
+
+def bench_rig(args):
+    import numpy as np
+
+    import ballasim_amd as bsx
+    from ballasim_amd import scene as S
+    from ballasim_amd.sensor import Rig, SensorModel
+
+    scene, _ = S.config1_scene(args.seed)
+    views = S.rig_views()
+    p = rig_params(bsx, args.photoreceptor)
+    rng = np.random.default_rng(args.seed + 1)
+    counts = [S.lognormal_counts(rng, v.width * v.height) for v in views]
+    allc = np.sort(np.concatenate(counts))[::-1]
+    top1 = float(allc[: max(1, len(allc) // 100)].sum() / allc.sum())
+    n_win = args.warmup + args.steps
+    runs, totals = [], []
+    for i in range(n_win):
+        rw, tw = [], 0
+        for c, v in enumerate(views):
+            r, tot = scene.sample_counts(v, counts[c], i * WINDOW, slot=i * len(views) + c)
+            rw.append(r)
+            tw += tot
+        runs.append(rw)
+        totals.append(tw)
+    print(f"# generated {n_win} windows of the rig, {sum(totals)} samples", file=sys.stderr, flush=True)
+
+    def cameras():
+        cams = []
+        for c, v in enumerate(views):
+            m = SensorModel(p, v.width, v.height, 1 + c, ["constant"], c, device="cuda")
+            m.init(0, scene.render(v, 0).reshape(-1, 1))
+            cams.append(m)
+        return cams
+
+    cams = cameras()
+    rig = Rig(cams)
+    trials = rig.autotune(runs[0], WINDOW, reps=2)
+    tuning = rig.tuning()
+    for i in range(args.warmup):
+        rig.advance_device(runs[i], (i + 1) * WINDOW)
+    for m in cams:
+        m.reset_stats()
+    k0 = rig.stats()["kernel_ms"]
+    ev_counts = []
+    t0 = time.perf_counter()
+    for i in range(args.warmup, n_win):
+        out = rig.advance_device(runs[i], (i + 1) * WINDOW)
+        ev_counts.append(sum(n for _, n in out))
+    elapsed = time.perf_counter() - t0
+    kernel_ms = rig.stats()["kernel_ms"] - k0
+    # The same windows camera by camera (a fresh set of cameras with the same tuning).
+    solo = cameras()
+    for m in solo:
+        m.set_launch_policy(**{k: (v if v != 0 or k != "queue_chunk" else -1) for k, v in tuning.items()})
+    for i in range(args.warmup):
+        for c, m in enumerate(solo):
+            m.advance_device_runs(runs[i][c], (i + 1) * WINDOW)
+    t0 = time.perf_counter()
+    solo_events = 0
+    for i in range(args.warmup, n_win):
+        for c, m in enumerate(solo):
+            solo_events += m.advance_device_runs(runs[i][c], (i + 1) * WINDOW)[1]
+    solo_elapsed = time.perf_counter() - t0
+    samples = sum(totals[args.warmup:])
+    events = sum(ev_counts)
+    assert events == solo_events, (events, solo_events)
+    P = sum(v.width * v.height for v in views)
+    bytes_total = sum(algorithmic_bytes(totals[i], P, P, ev_counts[i - args.warmup]) for i in range(args.warmup, n_win))
+    achieved = bytes_total / (kernel_ms * 1e-3) / 1e9
+    line = {
+        "metric": "pixel-samples/s + events/s on 1 MI355X; irradiance samples/event at fixed error",
+        "value": samples / elapsed,
+        "unit": "pixel-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (procedural scene and log-normal sample counts, seed %d)" % args.seed,
+        "config": {"workload": "4-camera rig 1280x720 + 640x480 + 346x260 + 1920x1080 (%d px) with scene-plane "
+                               "offsets, one textured disk at 800 px/s over 100 lux, per-pixel sample counts "
+                               "log-normal(ln 20, 1.5) clipped to [1, 4000], %d windows of 2^23 ns, %s response, "
+                               "noise off, one graph-captured launch chain per window"
+                               % (P, args.steps, args.photoreceptor),
+                   "pixels": P, "samples": samples, "events": events,
+                   "top_1pct_pixels_sample_share": top1},
+        "events_per_s": events / elapsed,
+        "per_camera_calls_value": samples / solo_elapsed,
+        "rig_speedup_vs_per_camera_calls": solo_elapsed / elapsed,
+        "tuning": tuning,
+        "autotune_trials": len(trials),
+        "rig_stats": rig.stats(),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": ADVANCE_CHAIN[args.photoreceptor] + " (4 cameras, one graph)",
+                     "kernel_ms_per_launch": kernel_ms / args.steps,
+                     "bytes_per_launch": bytes_total / args.steps},
+    }
+    if args.cpu_budget > 0:
+        line["cpu_baseline"] = cpu_baseline_rig(p, runs, views, scene, args.cpu_budget)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_rig(p, runs, views, scene, budget_s):
+    """The CPU path on the rig's cameras in turn (window by window) until budget_s of CPU work."""
+    from ballasim_amd import scene as S
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    kind = "reference" if O.ref_available() else "port"
+    models = []
+    for c, v in enumerate(views):
+        m = (O.RefModel(p.c, v.width, v.height, 1 + c, 1, c, threads, True) if kind == "reference"
+             else O.OracleModel(p.c, v.width, v.height, 1 + c, 1, c, threads))
+        m.init(0, scene.render(v, 0).reshape(-1, 1))
+        models.append(m)
+    samples, events, elapsed, done = 0, 0, 0.0, []
+    for i in range(len(runs)):
+        for c, m in enumerate(models):
+            ids, t, irr, offs = S.runs_to_host(runs[i][c])
+            t0 = time.perf_counter()
+            ev, _ = m.advance_runs(ids, t, irr, offs, (i + 1) * WINDOW)
+            elapsed += time.perf_counter() - t0
+            samples += len(t)
+            events += len(ev)
+            done.append((i, c))
+            if elapsed >= budget_s:
+                break
+        if elapsed >= budget_s:
+            break
+    return {"value": samples / elapsed, "unit": "pixel-samples/s", "cores": threads, "kind": kind,
+            "sample": f"{len(done)} camera-windows of the same runs (window, camera) {done[0]}..{done[-1]}, "
+                      f"{samples} samples, {events} events, {elapsed:.2f} s",
+            "events_per_s": events / elapsed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=58)
+    ap.add_argument("--steps", type=int, default=None, help="timed windows (default 58; rig: 8)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--photoreceptor", default="instantaneous", choices=["instantaneous", "cascade"])
+    ap.add_argument("--workload", default="plain", choices=["plain", "rig"],
+                    help="plain: the 1280x720 sensor (default); rig: the 4-camera ragged batch")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work (0: skip)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -110,6 +275,10 @@ def main():
         print(json.dumps({"error": "bench.py runs on one GPU: the sensor path has no multi-GPU form here "
                                    "(pixels shard by tile or camera; see DESIGN.md)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.steps is None:
+        args.steps = 8 if args.workload == "rig" else 58
+    if args.workload == "rig":
+        return bench_rig(args)
     import numpy as np
 
     import ballasim_amd as bsx

@@ -79,6 +79,14 @@ int bsx_scene_render(bsx_scene* s, const bsx_view* view, int64_t t_ns, float* ou
 int bsx_scene_sample_window(bsx_scene* s, const bsx_view* view, const bsx_sampler_rules* rules, int64_t t_w,
                             int32_t slot, void* stream, bsx_device_runs* out, uint64_t* total_samples);
 
+/* Samples of window (t_w, t_w + 2^23 ns] at imposed per-pixel counts (a host array of width*height
+ * values in [0, 2^23]): pixel p's n samples at t_w + floor((k + 1) 2^23 / n), k = 0 .. n - 1, with the
+ * scene's irradiance at each; runs in pixel order in buffer set `slot`, as bsx_scene_sample_window.
+ * No reference counterpart: the workload generator of ragged batches whose sample counts follow a
+ * given distribution rather than the controller. */
+int bsx_scene_sample_counts(bsx_scene* s, const bsx_view* view, const int32_t* counts, int64_t t_w, int32_t slot,
+                            void* stream, bsx_device_runs* out, uint64_t* total_samples);
+
 /* Copies `bytes` from device memory to host memory (synchronous): a caller without its own HIP
  * bindings reads runs or raw events with it. */
 int bsx_copy_to_host(void* dst, const void* src, size_t bytes);

@@ -148,3 +148,40 @@ def test_sensor_on_scene_runs_matches_oracle():
         assert np.array_equal(ev["t_reported_ns"], rep)
         total_events += len(want)
     assert total_events > 100
+
+
+def test_sample_counts_times_and_values():
+    """Imposed per-pixel counts: n samples of pixel p at t_w + floor((k + 1) 2^23 / n), each carrying
+    the scene's irradiance at that time (a frame render at the same instant), runs in pixel order."""
+    from ballasim_amd import scene as S
+    scene, view = S.config1_scene(2)
+    P = view.width * view.height
+    rng = np.random.default_rng(5)
+    counts = S.lognormal_counts(rng, P, mu=np.log(3.0), sigma=1.0, hi=40)
+    counts[::97] = 0
+    t_w = 3 * S.WINDOW_NS
+    runs, total = scene.sample_counts(view, counts, t_w, slot=7)
+    ids, t, irr, offs = S.runs_to_host(runs)
+    assert total == counts.sum() == len(t)
+    assert np.array_equal(ids, np.arange(P)) and np.array_equal(np.diff(offs), counts)
+    for p in rng.choice(np.nonzero(counts)[0], size=40, replace=False):
+        n = int(counts[p])
+        k = np.arange(n, dtype=np.int64)
+        assert np.array_equal(t[offs[p]:offs[p + 1]], t_w + (k + 1) * S.WINDOW_NS // n)
+    # values: the frame at one of the times, at the pixels sampled then
+    tt = int(t_w + S.WINDOW_NS)  # every nonempty run's last sample
+    frame = scene.render(view, tt)
+    last = offs[1:][counts > 0] - 1
+    np.testing.assert_allclose(irr[last, 0], frame[counts > 0], rtol=1e-6)
+
+
+def test_runs_of_smaller_view_survive_a_larger_view():
+    """Runs handed out for one view stay valid when a larger view is sampled into another slot."""
+    from ballasim_amd import scene as S
+    scene, small = S.config1_scene(3)
+    big = S.view(small.width * 2, small.height * 2)
+    rng = np.random.default_rng(6)
+    r0, _ = scene.sample_counts(small, S.lognormal_counts(rng, small.width * small.height, hi=8), 0, slot=0)
+    scene.sample_counts(big, S.lognormal_counts(rng, big.width * big.height, hi=8), 0, slot=1)
+    ids, _, _, _ = S.runs_to_host(r0)
+    assert np.array_equal(ids, np.arange(small.width * small.height))
