@@ -52,7 +52,7 @@ def _compile(src):
     if src.endswith(".hip"):
         cmd += DEVICE + ["-x", "hip"]
     else:
-        cmd += DEVICE
+        cmd += DEVICE + ["--offload-host-only"]  # host code only: the kernels live in the .hip files
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

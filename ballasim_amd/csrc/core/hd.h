@@ -24,6 +24,15 @@
 #define BSX_NOINLINE
 #endif
 
+// Pointers into device memory that the kernels read from descriptors (not kernel arguments) carry
+// the global address space in the device pass, so that their accesses compile to global (not flat)
+// memory instructions; the host pass sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define BSX_GLOBAL __attribute__((address_space(1)))
+#else
+#define BSX_GLOBAL
+#endif
+
 namespace bsx {
 
 template <class T>

@@ -36,9 +36,9 @@ struct InstantLane {
   bool ideal_reset;
   // The call's input run, and its segments' prepared values (prep[j]: the segment ending at input
   // sample j; null: every search is exact).
-  const int64_t* ts;
-  const float* irr;
-  const InstantPrep* prep;
+  const BSX_GLOBAL int64_t* ts;
+  const BSX_GLOBAL float* irr;
+  const BSX_GLOBAL InstantPrep* prep;
   int n, skip;
   int64_t t_state;
   // State.
@@ -405,15 +405,16 @@ struct InstantLane {
     return x_lo + base + dl_a - 16.0f * eps * (1.0f + fabs_(x_lo) + fabs_(s3_ref) + fabs_(dl_a)) <= -theta_off ? 2 : 0;
   }
 
-  // Would the walk, reaching the start of the current segment [ta, tb) in the state the lane holds
-  // (the call's input state, with no event, release or arrival before it), finish the segment on its
-  // prepared values without an event or an exact search? Exactly the conditions under which
-  // piece_fast decides the segment's one piece as "no crossing": no release or arrival pending
-  // before tb, no dead time past ta, no 2^23 cut, and fast_search = 0. Such a segment only
-  // advances t_search to tb (and counts one piece, and one root node when tb - ta > 2).
+  // Does the walk, reaching the start of the current segment [ta, tb) in the state the lane holds
+  // (the call's input state, with no event, release or arrival before it), finish the segment
+  // without an event? True when no release or arrival is pending before tb, the segment is one piece
+  // (no 2^23 cut) and fast_search = 0: then no integer of [ta, tb) fires under that state, so none
+  // of [max(t_search, ta), tb) does either, and the walk's one piece of the segment (decided by
+  // piece_fast, by an exact search after dead time, or skipped inside it) only advances t_search to
+  // max(t_search, tb) and counts one piece.
   // (Evaluated without early exits: every condition is computed and combined at the end.)
   BSX_HD BSX_INLINE bool quiet_segment(const Prep& q) const {
-    const bool eligible = (blind_until == kNotBlind) & (shot_next >= tb) & (t_search <= ta) &
+    const bool eligible = (blind_until == kNotBlind) & (shot_next >= tb) &
                           (((ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= tb);
     Hit h{false, false, ta, 0.0f};
     const int r = fast_search(q, h);

@@ -44,43 +44,43 @@ struct KConst {
 // Per-pixel constants in the kernel's precision (the reference casts its fp64 maps to T once per
 // call: src/sensor/pixel.h:771-775).
 struct MapsDev {
-  const float* theta_on;
-  const float* theta_off;
-  const float* leak;
-  const float* dark;
-  const float* scale;  // illuminance_scale x gain, formed in fp32 as the reference's scale_
-  const float* phi0;
+  const BSX_GLOBAL float* theta_on;
+  const BSX_GLOBAL float* theta_off;
+  const BSX_GLOBAL float* leak;
+  const BSX_GLOBAL float* dark;
+  const BSX_GLOBAL float* scale;  // illuminance_scale x gain, formed in fp32 as the reference's scale_
+  const BSX_GLOBAL float* phi0;
 };
 
 // Working state (the fp32 kernel's view of the reference's PixelState, src/sensor/pixel.h:105-121;
 // the fp32 values round-trip exactly through the reference's fp64 storage, so the saved-state format
 // is produced on the host by widening).
 struct StateDev {
-  float* z;
-  float* x1;
-  float* s2;
-  float* s3;
-  float* i_scale;
-  float* tv;
-  float* s3_ref;
-  float* n3_ref;
-  float* dr_ref;
-  int64_t* t_ref;
-  int64_t* blind_until;
-  int64_t* t_search;
-  int64_t* shot_next;  // next shot-noise arrival (ns), this library's extension
-  uint32_t* shot_index;
-  uint16_t* flags;
-  float* held;  // [g * n_pixels + p]
+  BSX_GLOBAL float* z;
+  BSX_GLOBAL float* x1;
+  BSX_GLOBAL float* s2;
+  BSX_GLOBAL float* s3;
+  BSX_GLOBAL float* i_scale;
+  BSX_GLOBAL float* tv;
+  BSX_GLOBAL float* s3_ref;
+  BSX_GLOBAL float* n3_ref;
+  BSX_GLOBAL float* dr_ref;
+  BSX_GLOBAL int64_t* t_ref;
+  BSX_GLOBAL int64_t* blind_until;
+  BSX_GLOBAL int64_t* t_search;
+  BSX_GLOBAL int64_t* shot_next;  // next shot-noise arrival (ns), this library's extension
+  BSX_GLOBAL uint32_t* shot_index;
+  BSX_GLOBAL uint16_t* flags;
+  BSX_GLOBAL float* held;  // [g * n_pixels + p]
 };
 
 // Sample runs of a call as the kernels read them: row_of[p] is pixel p's run or -1.
 struct RunsDev {
-  const int32_t* row_of;
-  const int64_t* row_start;
-  const int32_t* row_count;
-  const int64_t* t;
-  const float* irr;
+  const BSX_GLOBAL int32_t* row_of;
+  const BSX_GLOBAL int64_t* row_start;
+  const BSX_GLOBAL int32_t* row_count;
+  const BSX_GLOBAL int64_t* t;
+  const BSX_GLOBAL float* irr;
 };
 
 // The reference-independent values of a whole-segment search of the instantaneous response,
@@ -105,12 +105,12 @@ struct CameraDev {
   MapsDev mp;
   StateDev in, out;
   RunsDev runs;
-  const uint32_t* pixel;       // the runs' pixel ids
-  const int64_t* pos;          // the runs' segment positions (exclusive scan of the run lengths)
-  InstantPrep* prep;           // phase A's values (null: exact searches only)
-  int* first_loud;             // per run (null: every walk from its start)
-  float* work;                 // per run: estimated walk cost (null: no order information)
-  RawEventDev* ev;
+  const BSX_GLOBAL uint32_t* pixel;       // the runs' pixel ids
+  const BSX_GLOBAL int64_t* pos;          // the runs' segment positions (exclusive scan of the run lengths)
+  BSX_GLOBAL InstantPrep* prep;           // phase A's values (null: exact searches only)
+  BSX_GLOBAL int* first_loud;             // per run (null: every walk from its start)
+  BSX_GLOBAL float* work;                 // per run: estimated walk cost (null: no order information)
+  BSX_GLOBAL RawEventDev* ev;
   unsigned long long cap;
   unsigned long long* counters;  // [0] events, [1 + StatIdx] work counters
   int64_t t_state;             // the camera's model time at the start of the call

@@ -156,21 +156,27 @@ template <int kMaxG>
 __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, int n_cams, const CallDev* call) {
   const int64_t g0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
   if (g0 >= call->n_segments) return;  // beyond this call's segments (a grid sized for a larger one)
-  const CameraDev& c = cams[camera_at<Space::kSegments>(cams, n_cams, g0)];
-  const int64_t q0 = g0 - c.seg_base, q = q0 + threadIdx.x;
-  const int64_t total = c.total;
-  const int64_t i_q = block_run_of_position(c.pos, c.n_runs, q0, q, total);
-  const KConst& kc = c.kc;
-  const MapsDev& mp = c.mp;
-  const StateDev& in = c.in;
-  const RunsDev& runs = c.runs;
+  // The block's camera, copied to registers before any store (so that no store forces a reload).
+  const CameraDev& cd = cams[camera_at<Space::kSegments>(cams, n_cams, g0)];
+  const KConst kc = cd.kc;
+  const MapsDev mp = cd.mp;
+  const StateDev in = cd.in;
+  const RunsDev runs = cd.runs;
+  const uint32_t* const pixel = cd.pixel;
+  const int64_t* const pos = cd.pos;
+  InstantPrep* const prep = cd.prep;
+  float* const work = cd.work;
+  int* const first_loud = cd.first_loud;
+  const int64_t t_state = cd.t_state, total = cd.total, n_runs = cd.n_runs;
+  const int64_t q0 = g0 - cd.seg_base, q = q0 + threadIdx.x;
+  const int64_t i_q = block_run_of_position(pos, n_runs, q0, q, total);
   int run = -1;
   float cost = 0.0f;
   int loud = INT32_MAX;
   if (q < total) {
-    const int64_t i = i_q, j = q - c.pos[i];
+    const int64_t i = i_q, j = q - pos[i];
     run = static_cast<int>(i);
-    const uint32_t p = c.pixel[i];
+    const uint32_t p = pixel[i];
     const int64_t s = runs.row_start[i];
     InstantLane<kMaxG> L;
     L.G = kc.groups;
@@ -192,11 +198,11 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
     const float* v = runs.irr + s * kc.groups;
     bool empty = false;
     if (j == 0) {
-      if (t[0] == c.t_state) {  // the first sample is the model time's: no segment ends at it
-        c.prep[q] = InstantPrep{0.0f, 0.0f, 0.0f, 0.0f};
+      if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
+        prep[q] = InstantPrep{0.0f, 0.0f, 0.0f, 0.0f};
         empty = true;
       } else {
-        L.ta = c.t_state;
+        L.ta = t_state;
         for (int g = 0; g < kMaxG; ++g)
           if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
       }
@@ -210,15 +216,15 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
       for (int g = 0; g < kMaxG; ++g)
         if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
       const InstantPrep r = L.prep_segment();
-      c.prep[q] = r;
+      prep[q] = r;
       const float theta = vmin(L.theta_on, L.theta_off);
       const bool quiet = L.quiet_segment(r);
       loud = quiet ? INT32_MAX : static_cast<int>(j);
       cost = quiet ? 0.0f : 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) / theta;
     }
   }
-  if (c.work != nullptr) wave_segmented_add(c.work, run, cost);
-  if (c.first_loud != nullptr) wave_segmented_min(c.first_loud, run, loud);
+  if (work != nullptr) wave_segmented_add(work, run, cost);
+  if (first_loud != nullptr) wave_segmented_min(first_loud, run, loud);
 }
 
 // Before phase A: zero each camera's counters, the order histogram and the queue counter (block 0),

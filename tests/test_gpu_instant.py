@@ -64,6 +64,10 @@ def test_instant_matches_oracle(case):
         t = t1
     assert total > 1000
     assert gpu.time_ns == t
+    # the work counters the reference defines for this path (the leaf intervals are this backend's own)
+    ks, ko = gpu.stats(), ref.stats()
+    for f in ("substeps", "pieces"):
+        assert ks[f] == ko[f], (f, ks[f], ko[f])
 
 
 def test_padded_counts_and_unlisted_pixels():
