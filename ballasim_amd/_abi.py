@@ -158,6 +158,20 @@ SIGNATURES = {
     "bsx_rig_stats": (C.c_int, [P, C.POINTER(RigStats)]),
     "bsx_rig_autotune": (C.c_int, [P, C.POINTER(DeviceRuns), C.c_int64, P, C.c_int32, C.POINTER(LaunchPolicy),
                                    C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32)]),
+    "bsx_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(P)]),
+    "bsx_mlp_destroy": (C.c_int, [P]),
+    "bsx_mlp_set_params": (C.c_int, [P, P]),
+    "bsx_mlp_get_params": (C.c_int, [P, P]),
+    "bsx_mlp_gradients": (C.c_int, [P, P, P, C.c_int64, P, P, C.POINTER(C.c_double)]),
+    "bsx_mlp_train": (C.c_int, [P, P, P, C.c_int64, C.c_int32, C.c_int32, C.c_float, P, C.POINTER(C.c_double)]),
+    "bsx_mlp_predict": (C.c_int, [P, P, C.c_int64, P, P]),
+    "bsx_mlp_tile_check": (C.c_int, [P, P, P]),
+    "bsx_scene_learned_examples": (C.c_int, [P, C.POINTER(View), P, P, P, P, C.POINTER(C.c_uint64)]),
+    "bsx_learned_sampler_create": (C.c_int, [P, C.POINTER(View), P, P, C.POINTER(P)]),
+    "bsx_learned_sampler_destroy": (C.c_int, [P]),
+    "bsx_learned_sampler_init": (C.c_int, [P, C.c_int64, P]),
+    "bsx_learned_sampler_window": (C.c_int, [P, C.c_int64, C.c_int32, P, C.POINTER(DeviceRuns), C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "bsx_draw_maps": (C.c_int, [C.POINTER(Params), C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P]),
     "bsx_scene_create": (C.c_int, [C.POINTER(Shape), C.c_int32, C.c_float, C.POINTER(P)]),
     "bsx_scene_destroy": (C.c_int, [P]),

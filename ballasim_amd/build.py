@@ -23,9 +23,12 @@ SOURCES = [
     "sensor/model.cpp",
     "sensor/chain.cpp",
     "sensor/rig.cpp",
+    "mlp/proposer.cpp",
+    "mlp/mlp.hip",
     "sensor/kernels.hip",
     "scene/scene_host.cpp",
     "scene/scene.hip",
+    "scene/learned.hip",
 ]
 
 COMMON = [

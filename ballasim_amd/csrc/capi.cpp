@@ -5,26 +5,23 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "../../include/bsx/bsx_mlp.h"
 #include "../../include/bsx/bsx_rig.h"
 #include "../../include/bsx/bsx_sensor.h"
+#include "mlp/proposer.h"
 #include "sensor/kernels.h"
 #include "sensor/model.h"
 #include "sensor/params.h"
 #include "sensor/rig.h"
 
-struct bsx_sensor {
-  bsx::SensorModel* model;
-};
-
-struct bsx_rig {
-  bsx::Rig* rig;
-};
+#include "handles.h"
 
 namespace {
 
@@ -58,6 +55,11 @@ bsx::SensorModel& model_of(bsx_sensor* s) {
 bsx::Rig& rig_of(bsx_rig* r) {
   if (r == nullptr || r->rig == nullptr) throw std::invalid_argument("bsx: null rig handle");
   return *r->rig;
+}
+
+bsx::Proposer& mlp_of(bsx_mlp* m) {
+  if (m == nullptr || m->p == nullptr) throw std::invalid_argument("bsx: null proposer handle");
+  return *m->p;
 }
 
 bsx_launch_policy policy_of(const bsx::WalkTuning& t) {
@@ -393,6 +395,90 @@ int bsx_rig_autotune(bsx_rig* rig, const bsx_device_runs* runs, int64_t t_end_ns
       if (walk_ms != nullptr) walk_ms[i] = tr[i].walk_ms;
     }
     if (n_trials != nullptr) *n_trials = static_cast<int32_t>(tr.size());
+  });
+}
+
+int bsx_mlp_create(uint64_t seed, bsx_mlp** out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw std::runtime_error("bsx_mlp_create: no HIP device");
+    auto* p = new bsx::Proposer(seed);
+    *out = new bsx_mlp{p};
+  });
+}
+
+int bsx_mlp_destroy(bsx_mlp* m) {
+  return guarded([&] {
+    if (m == nullptr) return;
+    delete m->p;
+    delete m;
+  });
+}
+
+int bsx_mlp_set_params(bsx_mlp* m, const float* params) {
+  return guarded([&] {
+    need_ptr(params, "params");
+    for (int i = 0; i < BSX_MLP_PARAMS; ++i)
+      if (!std::isfinite(params[i])) throw std::invalid_argument("bsx_mlp_set_params: non-finite parameter");
+    mlp_of(m).set_params(params);
+  });
+}
+
+int bsx_mlp_get_params(bsx_mlp* m, float* params) {
+  return guarded([&] {
+    need_ptr(params, "params");
+    mlp_of(m).get_params(params);
+  });
+}
+
+int bsx_mlp_gradients(bsx_mlp* m, const float* x, const float* target, int64_t n, void* stream, float* grad,
+                      double* loss) {
+  return guarded([&] {
+    need_ptr(x, "x");
+    need_ptr(target, "target");
+    need_ptr(grad, "grad");
+    const double l = mlp_of(m).gradients(x, target, n, grad, static_cast<hipStream_t>(stream));
+    if (loss != nullptr) *loss = l;
+  });
+}
+
+int bsx_mlp_train(bsx_mlp* m, const float* x, const float* target, int64_t n, int32_t batch, int32_t epochs, float lr,
+                  void* stream, double* loss) {
+  return guarded([&] {
+    need_ptr(x, "x");
+    need_ptr(target, "target");
+    const double l = mlp_of(m).train(x, target, n, batch, epochs, lr, static_cast<hipStream_t>(stream));
+    if (loss != nullptr) *loss = l;
+  });
+}
+
+int bsx_mlp_predict(bsx_mlp* m, const float* x, int64_t n, float* y, void* stream) {
+  return guarded([&] {
+    if (n < 0) throw std::invalid_argument("bsx_mlp_predict: n must be >= 0");
+    if (n == 0) return;
+    need_ptr(x, "x");
+    need_ptr(y, "y");
+    mlp_of(m).predict(x, n, y, static_cast<hipStream_t>(stream));
+  });
+}
+
+int bsx_mlp_tile_check(const uint16_t* a, const uint16_t* b, float* c) {
+  return guarded([&] {
+    need_ptr(a, "a");
+    need_ptr(b, "b");
+    need_ptr(c, "c");
+    bsx::DeviceBuffer da, db, dc;
+    da.ensure(512);
+    db.ensure(512);
+    dc.ensure(4096);
+    bsx::hip_check(hipMemcpy(da.get(), a, 512, hipMemcpyHostToDevice), "upload");
+    bsx::hip_check(hipMemcpy(db.get(), b, 512, hipMemcpyHostToDevice), "upload");
+    bsx::hip_check(bsx::mlp::launch_tile_check(da.as<const uint16_t>(), db.as<const uint16_t>(), dc.as<float>(),
+                                               nullptr),
+                   "tile check");
+    bsx::hip_check(hipMemcpy(c, dc.get(), 4096, hipMemcpyDeviceToHost), "download");
   });
 }
 

@@ -1,0 +1,190 @@
+// Kernels of the learned proposer: bf16 copies, batched forward, per-wave gradients (forward and
+// backward on the MFMA units, one wave per 64 rows), and the reduction of the partial gradients with
+// an Adam step.
+#include <hip/hip_runtime.h>
+
+#include "mlp_dev.h"
+
+namespace bsx {
+namespace mlp {
+
+namespace {
+
+__global__ void to_bf16_kernel(const float* w, uint16_t* wb, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) wb[i] = to_bf16(w[i]);
+}
+
+__device__ __forceinline__ void load_row(const float* x, int64_t row, bool valid, float* f) {
+  for (int c = 0; c < kIn; ++c) f[c] = valid ? x[row * kIn + c] : 0.0f;
+}
+
+// Rows in groups of 64, one group per wave per iteration (grid-stride).
+__global__ __launch_bounds__(64) void forward_kernel(const uint16_t* wb, const float* w, const float* x, int64_t n,
+                                                     float* y) {
+  __shared__ WeightsLds W;
+  __shared__ WaveLds S;
+  load_weights(W, wb, w);
+  const int l = threadIdx.x;
+  const int64_t groups = (n + 63) / 64;
+  for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+    const int64_t row = g * 64 + l;
+    const bool valid = row < n;
+    float f[kIn];
+    load_row(x, row, valid, f);
+    put_features(S, f);
+    const float out = wave_forward(W, S);
+    if (valid) y[row] = out;
+    wave_sync();
+  }
+}
+
+// Gradients of (1 / batch) sum (y - t)^2 over the rows of group g = blockIdx.x (64 rows), written as
+// one partial gradient vector per group.
+__global__ __launch_bounds__(64) void gradients_kernel(const uint16_t* wb, const float* w, const float* x,
+                                                       const float* target, int64_t n, float two_over_batch,
+                                                       float* partial, float* loss_partial) {
+  __shared__ WeightsLds W;
+  __shared__ WaveLds S;
+  __shared__ uint16_t dz2[kRowsPerWave * kS16];
+  __shared__ uint16_t dz1[kRowsPerWave * kS16];
+  __shared__ float dys[kRowsPerWave];
+  load_weights(W, wb, w);
+  const int l = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  const int64_t row = g * 64 + l;
+  const bool valid = row < n;
+  float f[kIn];
+  load_row(x, row, valid, f);
+  put_features(S, f);
+  const float y = wave_forward(W, S);
+  const float e = valid ? y - target[row] : 0.0f;
+  const float dy = e * two_over_batch;
+  dys[l] = dy;
+  float se = e * e;
+  for (int off = 32; off > 0; off >>= 1) se += __shfl_xor(se, off);
+  if (l == 0) loss_partial[g] = se;
+  // dz2 = dy w3 silu'(z2) (this lane's row)
+  for (int h = 0; h < kHid; ++h) dz2[l * kS16 + h] = to_bf16(dy * W.w3[h] * silu_grad(S.z2[l * kS32 + h]));
+  wave_sync();
+  float* P = partial + g * kParams;
+  // dw3[h] = sum_r dy_r h2[r][h]; db2[h] = sum_r dz2[r][h] (lane h); db3 = sum_r dy_r
+  {
+    float s3 = 0.0f, s2 = 0.0f;
+    for (int r = 0; r < kRowsPerWave; ++r) {
+      s3 += dys[r] * from_bf16(to_bf16(silu(S.z2[r * kS32 + l])));
+      s2 += from_bf16(dz2[r * kS16 + l]);
+    }
+    P[kOffW3 + l] = s3;
+    P[kOffB2 + l] = s2;
+    if (l == 0) {
+      float s = 0.0f;
+      for (int r = 0; r < kRowsPerWave; ++r) s += dys[r];
+      P[kOffB3] = s;
+    }
+  }
+  // dW2 = h1^T dz2 (rows of W2: layer-1 units)
+  for (int mt = 0; mt < 2; ++mt)
+    for (int nt = 0; nt < 2; ++nt) {
+      f32x16 acc = {};
+      mma_tile(acc, S.h1 + mt * 32, 1, kS16, dz2 + nt * 32, kS16, 1, kRowsPerWave);
+      for (int j = 0; j < 16; ++j) P[kOffW2 + (mt * 32 + acc_row(j)) * kHid + nt * 32 + acc_col()] = acc[j];
+    }
+  // dh1 = dz2 W2^T, dz1 = dh1 silu'(z1)
+  for (int mt = 0; mt < 2; ++mt)
+    for (int nt = 0; nt < 2; ++nt) {
+      f32x16 acc = {};
+      mma_tile(acc, dz2 + mt * 32 * kS16, kS16, 1, W.w2 + nt * 32 * kS16, 1, kS16, kHid);
+      for (int j = 0; j < 16; ++j) {
+        const int r = mt * 32 + acc_row(j), c = nt * 32 + acc_col();
+        dz1[r * kS16 + c] = to_bf16(acc[j] * silu_grad(S.z1[r * kS32 + c]));
+      }
+    }
+  wave_sync();
+  {
+    float s1 = 0.0f;
+    for (int r = 0; r < kRowsPerWave; ++r) s1 += from_bf16(dz1[r * kS16 + l]);
+    P[kOffB1 + l] = s1;
+  }
+  // dW1 = x^T dz1 (features padded to a 32-row tile; rows 12 .. 31 are zero and not stored)
+  for (int nt = 0; nt < 2; ++nt) {
+    f32x16 acc = {};
+    mma_tile(acc, S.xs, 1, kS16, dz1 + nt * 32, kS16, 1, kRowsPerWave);
+    for (int j = 0; j < 16; ++j) {
+      const int fr = acc_row(j);
+      if (fr < kIn) P[kOffW1 + fr * kHid + nt * 32 + acc_col()] = acc[j];
+    }
+  }
+}
+
+__global__ void adam_kernel(const float* partial, int waves, float* grad, float* w, float* m, float* v, uint16_t* wb,
+                            float lr, float beta1, float beta2, float eps, float bc1, float bc2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kParams) return;
+  float g = 0.0f;
+  for (int k = 0; k < waves; ++k) g += partial[static_cast<int64_t>(k) * kParams + i];
+  grad[i] = g;
+  if (lr == 0.0f) return;  // gradients only
+  const float mi = beta1 * m[i] + (1.0f - beta1) * g;
+  const float vi = beta2 * v[i] + (1.0f - beta2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  w[i] = wi;
+  wb[i] = to_bf16(wi);
+}
+
+__global__ __launch_bounds__(64) void tile_check_kernel(const uint16_t* a, const uint16_t* b, float* c) {
+  __shared__ uint16_t as[32 * 8], bs[8 * 32];
+  for (int i = threadIdx.x; i < 256; i += 64) {
+    as[i] = a[i];
+    bs[i] = b[i];
+  }
+  __syncthreads();
+  f32x16 acc = {};
+  mma_tile(acc, as, 8, 1, bs, 32, 1, 8);
+  for (int j = 0; j < 16; ++j) c[acc_row(j) * 32 + acc_col()] = acc[j];
+}
+
+}  // namespace
+
+hipError_t launch_to_bf16(const float* w, uint16_t* wb, int n, hipStream_t stream) {
+  to_bf16_kernel<<<(n + 255) / 256, 256, 0, stream>>>(w, wb, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_forward(const uint16_t* wb, const float* b_fp32, const float* x, int64_t n, float* y,
+                          hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  const int64_t groups = (n + 63) / 64;
+  const unsigned grid = static_cast<unsigned>(groups < 8192 ? groups : 8192);
+  forward_kernel<<<grid, 64, 0, stream>>>(wb, b_fp32, x, n, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradients(const uint16_t* wb, const float* b_fp32, const float* x, const float* target, int64_t n,
+                            int64_t batch, float* partial, float* loss_partial, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  const int64_t groups = (n + 63) / 64;
+  gradients_kernel<<<static_cast<unsigned>(groups), 64, 0, stream>>>(wb, b_fp32, x, target, n,
+                                                                     2.0f / static_cast<float>(batch), partial,
+                                                                     loss_partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(const float* partial, int waves, float* grad, float* w, float* m, float* v, uint16_t* wb,
+                       const Adam& a, hipStream_t stream) {
+  const double bc1 = 1.0 - __builtin_pow(static_cast<double>(a.beta1), a.step);
+  const double bc2 = 1.0 - __builtin_pow(static_cast<double>(a.beta2), a.step);
+  adam_kernel<<<(kParams + 255) / 256, 256, 0, stream>>>(partial, waves, grad, w, m, v, wb, a.lr, a.beta1, a.beta2,
+                                                         a.eps, static_cast<float>(bc1), static_cast<float>(bc2));
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_check(const uint16_t* a, const uint16_t* b, float* c, hipStream_t stream) {
+  tile_check_kernel<<<1, 64, 0, stream>>>(a, b, c);
+  return hipGetLastError();
+}
+
+}  // namespace mlp
+}  // namespace bsx

@@ -9,7 +9,11 @@
 #include <vector>
 
 #include "../../../include/bsx/bsx_scene.h"
+#include "../../../include/bsx/bsx_mlp.h"
+#include "../handles.h"
+#include "../mlp/proposer.h"
 #include "../sensor/model.h"
+#include "learned_kernels.h"
 #include "scene_kernels.h"
 
 namespace bsx {
@@ -105,6 +109,11 @@ class Scene {
     return out;
   }
 
+  int64_t* scan_scratch(size_t P) {
+    scan_.ensure(scan_scratch_count(P) * 8);
+    return scan_.as<int64_t>();
+  }
+
   bsx_device_runs sample_counts(const bsx_view& view, const int32_t* counts, int64_t t_w, int32_t slot,
                                 hipStream_t stream, uint64_t* total_out) {
     const ViewDev v = view_of(view);
@@ -177,11 +186,125 @@ class Scene {
   std::vector<std::unique_ptr<WindowSlot>> slots_;
 };
 
+// Training examples of the proposer from dense traces of the scene (see learned.hip).
+void learned_examples(Scene& sc, const bsx_view& view, const bsx_learned_examples_spec& sp, float* x, float* target,
+                      hipStream_t stream, uint64_t* n_active_out) {
+  const ViewDev v = Scene::view_of(view);
+  const size_t P = static_cast<size_t>(v.width) * v.height;
+  if (sp.n_dense < sp.max_steps + 3 * 256 + 2 || sp.max_steps < 1 || sp.delta_ns < 1 || sp.n < 0 ||
+      !(sp.theta_ref > 0.0f) || !(sp.theta_band > 0.0f) || !(sp.tol_log > 0.0f)) {
+    throw std::invalid_argument("learned_examples: need n_dense >= max_steps + 770, max_steps >= 1, delta >= 1, "
+                                "n >= 0 and positive thresholds and tolerance");
+  }
+  DeviceBuffer trace, ref, active, pixels;
+  trace.ensure(P * sp.n_dense * 4);
+  ref.ensure(P * sp.n_dense * 4);
+  active.ensure(P);
+  hip_check(launch_dense_trace(sc.dev(), v, sp.t0_ns, sp.delta_ns, sp.n_dense, sp.dark, sp.theta_ref,
+                               trace.as<float>(), ref.as<float>(), active.as<uint8_t>(), stream),
+            "dense trace");
+  std::vector<uint8_t> act(P);
+  hip_check(hipMemcpyAsync(act.data(), active.get(), P, hipMemcpyDeviceToHost, stream), "active pixels");
+  hip_check(hipStreamSynchronize(stream), "dense trace");
+  // Candidates: every varying pixel, and constant ones in a tenth of their number (spread evenly).
+  std::vector<uint32_t> cand, quiet;
+  for (size_t p = 0; p < P; ++p) (act[p] ? cand : quiet).push_back(static_cast<uint32_t>(p));
+  if (n_active_out != nullptr) *n_active_out = cand.size();
+  const size_t extra = cand.empty() ? quiet.size() : std::min(quiet.size(), cand.size() / 10);
+  for (size_t k = 0; k < extra; ++k) cand.push_back(quiet[k * quiet.size() / std::max<size_t>(extra, 1)]);
+  if (cand.empty()) throw std::invalid_argument("learned_examples: the view has no pixels");
+  pixels.ensure(cand.size() * 4);
+  hip_check(hipMemcpyAsync(pixels.get(), cand.data(), cand.size() * 4, hipMemcpyHostToDevice, stream), "pixels");
+  hip_check(launch_examples(trace.as<const float>(), ref.as<const float>(), static_cast<uint32_t>(P), sp.n_dense,
+                            pixels.as<const uint32_t>(), static_cast<uint32_t>(cand.size()), sp.delta_ns,
+                            sp.max_steps, sp.theta_band, sp.tol_log, sp.seed, sp.n, x, target, stream),
+            "examples");
+  hip_check(hipStreamSynchronize(stream), "examples");
+}
+
+// The learned sampler of one view: the per-pixel controller state and the window passes.
+class LearnedSampler {
+ public:
+  LearnedSampler(Scene& sc, const bsx_view& view, const bsx_learned_rules& r, Proposer& p)
+      : sc_(sc), view_(view), p_(p) {
+    const ViewDev v = Scene::view_of(view);
+    const bsx_sampler_rules& q = r.plain;
+    if (!(q.refine_tol_log > 0.0f) || q.max_refine_levels < 0 || q.max_refine_levels > 7 || !(r.tol_log > 0.0f) ||
+        !(r.theta_band > 0.0f) || !(r.theta_ref > 0.0f) || !(r.margin > 0.0f) || r.min_dt_ns < 2) {
+      throw std::invalid_argument("learned sampler: need positive tolerances, thresholds and margin, "
+                                  "0 <= max_refine_levels <= 7 and min_dt_ns >= 2");
+    }
+    r_ = LearnedRules{q.dark, q.refine_tol_log, r.tol_log, r.theta_band, r.theta_ref, r.margin, q.max_refine_levels,
+                      q.refine_signature, static_cast<int64_t>(q.min_split_ns), r.min_dt_ns};
+    P_ = static_cast<size_t>(v.width) * v.height;
+    t_.ensure(P_ * 4 * 8);
+    L_.ensure(P_ * 4 * 4);
+    n_.ensure(P_ * 4);
+    ref_.ensure(P_ * 4);
+    e0_.ensure(P_ * 4);
+    sig_.ensure(P_ * 4);
+    work_.ensure(16);
+  }
+
+  LearnedStateDev state() const {
+    return LearnedStateDev{t_.as<int64_t>(), L_.as<float>(), n_.as<int32_t>(), ref_.as<float>(), e0_.as<float>(),
+                           sig_.as<uint32_t>()};
+  }
+
+  void init(int64_t t0, hipStream_t stream) {
+    hip_check(launch_learned_init(sc_.dev(), Scene::view_of(view_), t0, r_.dark, state(), stream), "learned init");
+    hip_check(hipStreamSynchronize(stream), "learned init");
+    t_init_ = t0;
+    initialized_ = true;
+  }
+
+  bsx_device_runs window(int64_t t_w, int32_t slot, hipStream_t stream, uint64_t* total_out, uint64_t* proposals,
+                         uint64_t* fallbacks) {
+    if (!initialized_) throw std::logic_error("learned sampler: call init first");
+    if ((t_w & (kWindowNs - 1)) != 0) throw std::invalid_argument("learned sampler: t_w must be a multiple of 2^23 ns");
+    const ViewDev v = Scene::view_of(view_);
+    WindowSlot& w = sc_.slot_at(slot);
+    w.counts.ensure(P_ * 4);
+    w.row_start.ensure(P_ * 8);
+    sc_.ensure_ids(P_);
+    int64_t* scan = sc_.scan_scratch(P_);
+    hip_check(hipMemsetAsync(work_.get(), 0, 16, stream), "zero work");
+    hip_check(launch_learned_window(sc_.dev(), v, r_, p_.weights_bf16(), p_.weights(), t_w, state(),
+                                    w.counts.as<int32_t>(), nullptr, nullptr, nullptr,
+                                    work_.as<unsigned long long>(), false, stream),
+              "learned count");
+    hip_check(launch_exclusive_scan(w.counts.as<int32_t>(), P_, w.row_start.as<int64_t>(), scan, stream), "scan");
+    int64_t total = 0;
+    unsigned long long wk[2] = {0, 0};
+    hip_check(hipMemcpyAsync(&total, scan + scan_scratch_count(P_) - 1, 8, hipMemcpyDeviceToHost, stream), "total");
+    hip_check(hipMemcpyAsync(wk, work_.get(), 16, hipMemcpyDeviceToHost, stream), "work");
+    hip_check(hipStreamSynchronize(stream), "learned count");
+    w.t.ensure(std::max<int64_t>(total, 1) * 8);
+    w.irr.ensure(std::max<int64_t>(total, 1) * 4);
+    hip_check(launch_learned_window(sc_.dev(), v, r_, p_.weights_bf16(), p_.weights(), t_w, state(), nullptr,
+                                    w.row_start.as<const int64_t>(), w.t.as<int64_t>(), w.irr.as<float>(), nullptr,
+                                    true, stream),
+              "learned write");
+    hip_check(hipStreamSynchronize(stream), "learned write");
+    if (total_out != nullptr) *total_out = static_cast<uint64_t>(total);
+    if (proposals != nullptr) *proposals = wk[0];
+    if (fallbacks != nullptr) *fallbacks = wk[1];
+    return sc_.runs_of(w, P_, total);
+  }
+
+ private:
+  Scene& sc_;
+  bsx_view view_;
+  Proposer& p_;
+  LearnedRules r_{};
+  size_t P_ = 0;
+  DeviceBuffer t_, L_, n_, ref_, e0_, sig_, work_;
+  int64_t t_init_ = 0;
+  bool initialized_ = false;
+};
+
 }  // namespace bsx
 
-struct bsx_scene {
-  bsx::Scene* s;
-};
 
 // Status mapping shared with capi.cpp's conventions.
 extern "C" int bsx_internal_set_error(int code, const char* msg);
@@ -251,6 +374,52 @@ int bsx_scene_sample_counts(bsx_scene* s, const bsx_view* view, const int32_t* c
       throw std::invalid_argument("bsx_scene_sample_counts: null");
     }
     *out = s->s->sample_counts(*view, counts, t_w, slot, static_cast<hipStream_t>(stream), total_samples);
+  });
+}
+
+int bsx_scene_learned_examples(bsx_scene* s, const bsx_view* view, const bsx_learned_examples_spec* spec, float* x,
+                                float* target, void* stream, uint64_t* n_active_pixels) {
+  return scene_guarded([&] {
+    if (s == nullptr || view == nullptr || spec == nullptr || (spec->n > 0 && (x == nullptr || target == nullptr))) {
+      throw std::invalid_argument("bsx_scene_learned_examples: null");
+    }
+    bsx::learned_examples(*s->s, *view, *spec, x, target, static_cast<hipStream_t>(stream), n_active_pixels);
+  });
+}
+
+int bsx_learned_sampler_create(bsx_scene* s, const bsx_view* view, const bsx_learned_rules* rules, bsx_mlp* proposer,
+                               bsx_learned_sampler** out) {
+  return scene_guarded([&] {
+    if (s == nullptr || view == nullptr || rules == nullptr || proposer == nullptr || proposer->p == nullptr ||
+        out == nullptr) {
+      throw std::invalid_argument("bsx_learned_sampler_create: null");
+    }
+    *out = nullptr;
+    auto* ls = new bsx::LearnedSampler(*s->s, *view, *rules, *proposer->p);
+    *out = new bsx_learned_sampler{ls};
+  });
+}
+
+int bsx_learned_sampler_destroy(bsx_learned_sampler* ls) {
+  return scene_guarded([&] {
+    if (ls == nullptr) return;
+    delete ls->ls;
+    delete ls;
+  });
+}
+
+int bsx_learned_sampler_init(bsx_learned_sampler* ls, int64_t t0_ns, void* stream) {
+  return scene_guarded([&] {
+    if (ls == nullptr || ls->ls == nullptr) throw std::invalid_argument("bsx_learned_sampler_init: null");
+    ls->ls->init(t0_ns, static_cast<hipStream_t>(stream));
+  });
+}
+
+int bsx_learned_sampler_window(bsx_learned_sampler* ls, int64_t t_w, int32_t slot, void* stream, bsx_device_runs* out,
+                               uint64_t* total_samples, uint64_t* proposals, uint64_t* fallbacks) {
+  return scene_guarded([&] {
+    if (ls == nullptr || ls->ls == nullptr || out == nullptr) throw std::invalid_argument("bsx_learned_sampler_window: null");
+    *out = ls->ls->window(t_w, slot, static_cast<hipStream_t>(stream), total_samples, proposals, fallbacks);
   });
 }
 

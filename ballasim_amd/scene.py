@@ -154,3 +154,19 @@ def rig_views(offsets=RIG_OFFSETS):
 def lognormal_counts(rng, n, mu=np.log(20.0), sigma=1.5, lo=1, hi=4000):
     """Per-pixel sample counts of the ragged-batch workload: round(exp(N(mu, sigma))) clipped to [lo, hi]."""
     return np.clip(np.rint(np.exp(rng.normal(mu, sigma, size=n))), lo, hi).astype(np.int32)
+
+
+def learned_eval_scene(seed=0, width=1280, height=720, n_disks=12):
+    """Config-1-style scene at a larger size: n_disks disks like config 1's (radius 40 px, 8-octave value
+    noise of contrast 0.5, 800 px/s in a seeded direction) over a 100 lux background, placed so that
+    their 500 ms paths stay mostly in the frame."""
+    rng = np.random.default_rng(seed)
+    shapes = []
+    for _ in range(n_disks):
+        ang = float(rng.uniform(0, 2 * np.pi))
+        vx, vy = 800.0 * np.cos(ang), 800.0 * np.sin(ang)
+        cx = float(rng.uniform(0.25, 0.75) * width) - 0.25 * vx
+        cy = float(rng.uniform(0.25, 0.75) * height) - 0.25 * vy
+        shapes.append(shape("disk", cx, cy, vx, vy, 40.0, 40.0, 100.0, 0.5, int(rng.integers(1, 2 ** 31)), octaves=8,
+                            cell=64.0))
+    return Scene(shapes, 100.0), view(width, height)
