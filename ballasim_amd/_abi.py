@@ -172,6 +172,10 @@ SIGNATURES = {
     "bsx_learned_sampler_init": (C.c_int, [P, C.c_int64, P]),
     "bsx_learned_sampler_window": (C.c_int, [P, C.c_int64, C.c_int32, P, C.POINTER(DeviceRuns), C.POINTER(C.c_uint64),
                                              C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "bsx_output_create": (C.c_int, [P, P, C.c_int64, C.POINTER(P)]),
+    "bsx_output_destroy": (C.c_int, [P]),
+    "bsx_output_window": (C.c_int, [P, C.POINTER(DeviceEvents), C.c_int64, P, P]),
+    "bsx_output_flush": (C.c_int, [P, P, P]),
     "bsx_draw_maps": (C.c_int, [C.POINTER(Params), C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P]),
     "bsx_scene_create": (C.c_int, [C.POINTER(Shape), C.c_int32, C.c_float, C.POINTER(P)]),
     "bsx_scene_destroy": (C.c_int, [P]),

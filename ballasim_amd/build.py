@@ -24,6 +24,8 @@ SOURCES = [
     "sensor/chain.cpp",
     "sensor/rig.cpp",
     "mlp/proposer.cpp",
+    "output/output.cpp",
+    "output/output.hip",
     "mlp/mlp.hip",
     "sensor/kernels.hip",
     "scene/scene_host.cpp",

@@ -13,9 +13,11 @@
 #include <vector>
 
 #include "../../include/bsx/bsx_mlp.h"
+#include "../../include/bsx/bsx_output.h"
 #include "../../include/bsx/bsx_rig.h"
 #include "../../include/bsx/bsx_sensor.h"
 #include "mlp/proposer.h"
+#include "output/output.h"
 #include "sensor/kernels.h"
 #include "sensor/model.h"
 #include "sensor/params.h"
@@ -479,6 +481,62 @@ int bsx_mlp_tile_check(const uint16_t* a, const uint16_t* b, float* c) {
                                                nullptr),
                    "tile check");
     bsx::hip_check(hipMemcpy(c, dc.get(), 4096, hipMemcpyDeviceToHost), "download");
+  });
+}
+
+namespace {
+
+bsx::OutputStage& output_of(bsx_output* o) {
+  if (o == nullptr || o->o == nullptr) throw std::invalid_argument("bsx: null output handle");
+  return *o->o;
+}
+
+void fill_output(const bsx::OutputColumns& c, size_t held, bsx_output_events* out) {
+  *out = bsx_output_events{c.x.size(),          c.x.data(),         c.y.data(),       c.p.data(),
+                           c.t_us.data(),       c.t_ns_rem.data(),  c.t_reported.data(), c.t_cross.data(),
+                           c.label.data(),      c.flags.data(),     held};
+}
+
+}  // namespace
+
+int bsx_output_create(bsx_sensor* camera, const bsx_crop* crop, int64_t t_offset_us, bsx_output** out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    *out = nullptr;
+    bsx::SensorModel& m = model_of(camera);
+    bsx::CropBox box{};
+    if (crop != nullptr) box = bsx::CropBox{crop->x0, crop->y0, crop->x1, crop->y1};
+    auto* o = new bsx::OutputStage(m.width(), m.height(), m.readout(), crop != nullptr ? &box : nullptr, t_offset_us);
+    *out = new bsx_output{o};
+  });
+}
+
+int bsx_output_destroy(bsx_output* o) {
+  return guarded([&] {
+    if (o == nullptr) return;
+    delete o->o;
+    delete o;
+  });
+}
+
+int bsx_output_window(bsx_output* o, const bsx_device_events* events, int64_t t_w, void* stream,
+                      bsx_output_events* out) {
+  return guarded([&] {
+    need_ptr(events, "events");
+    need_ptr(out, "out");
+    bsx::OutputStage& s = output_of(o);
+    const bsx::OutputColumns& c = s.window(reinterpret_cast<const bsx::RawEventDev*>(events->data), events->count, t_w,
+                                           static_cast<hipStream_t>(stream));
+    fill_output(c, s.held(), out);
+  });
+}
+
+int bsx_output_flush(bsx_output* o, void* stream, bsx_output_events* out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    bsx::OutputStage& s = output_of(o);
+    const bsx::OutputColumns& c = s.flush(static_cast<hipStream_t>(stream));
+    fill_output(c, s.held(), out);
   });
 }
 

@@ -7,6 +7,7 @@ class Rig;
 class Proposer;
 class Scene;
 class LearnedSampler;
+class OutputStage;
 }  // namespace bsx
 
 struct bsx_sensor {
@@ -23,4 +24,7 @@ struct bsx_scene {
 };
 struct bsx_learned_sampler {
   bsx::LearnedSampler* ls;
+};
+struct bsx_output {
+  bsx::OutputStage* o;
 };

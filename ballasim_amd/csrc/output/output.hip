@@ -1,0 +1,168 @@
+// Kernels of the output stage: event build with crop and readout, key passes, rocPRIM's stable radix
+// sort, gather with the window cut, and the C4 columns.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "output_kernels.h"
+
+namespace bsx {
+
+namespace {
+
+// S1 class of a sensor label (signal 0, noise 2, leak 3, hot 4).
+__device__ __forceinline__ uint8_t label_s1(uint8_t c) {
+  return c == 0 ? 0 : static_cast<uint8_t>((c & 3) + 1);
+}
+
+constexpr uint64_t kBias = uint64_t{1} << 63;
+
+__global__ void build_kernel(const RawEventDev* raw, size_t n, uint32_t width, Readout r, int crop, CropBox box,
+                             OutEventDev* out, unsigned long long* scratch) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool keep = false;
+  OutEventDev o{};
+  if (i < n) {
+    const RawEventDev e = raw[i];
+    const int32_t x = static_cast<int32_t>(e.pixel % width), y = static_cast<int32_t>(e.pixel / width);
+    keep = !crop || (x >= box.x0 && x < box.x1 && y >= box.y0 && y < box.y1);
+    o.t_cross = e.t_cross;
+    o.t_reported = reported_time(r, e.t_cross);
+    o.x = static_cast<uint16_t>(x);
+    o.y = static_cast<uint16_t>(y);
+    o.flags = e.flags;
+    o.p = e.polarity;
+    o.label = label_s1(e.label);
+  }
+  // wave-aggregated slots (the order of the kept events does not matter: the output key is unique)
+  const unsigned long long mask = __ballot(keep);
+  const int lane = __lane_id();
+  unsigned long long base = 0;
+  if (mask != 0) {
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    if (lane == leader) base = atomicAdd(scratch, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, leader);
+  }
+  if (keep) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = o;
+}
+
+__global__ void minmax_kernel(const OutEventDev* ev, size_t n_held, size_t n_max, unsigned long long* scratch) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t n = n_held + static_cast<size_t>(scratch[0]);
+  unsigned long long a = ~0ull, b = 0ull, c = ~0ull, d = 0ull;
+  if (i < n && i < n_max) {
+    a = b = static_cast<uint64_t>(ev[i].t_cross) ^ kBias;
+    c = d = static_cast<uint64_t>(ev[i].t_reported) ^ kBias;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a = min(a, static_cast<unsigned long long>(__shfl_xor(a, off)));
+    b = max(b, static_cast<unsigned long long>(__shfl_xor(b, off)));
+    c = min(c, static_cast<unsigned long long>(__shfl_xor(c, off)));
+    d = max(d, static_cast<unsigned long long>(__shfl_xor(d, off)));
+  }
+  if (__lane_id() == 0 && a != ~0ull) {
+    atomicMin(scratch + 1, a);
+    atomicMax(scratch + 2, b);
+    atomicMin(scratch + 3, c);
+    atomicMax(scratch + 4, d);
+  }
+}
+
+__global__ void keys_kernel(const OutEventDev* ev, const uint32_t* perm, uint32_t* perm_init, size_t n, int pass,
+                            int y_bits, int64_t tc_min, int64_t tr_min, uint64_t* keys) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t e = static_cast<uint32_t>(i);
+  if (perm != nullptr) e = perm[i];
+  else if (perm_init != nullptr) perm_init[i] = e;
+  const OutEventDev o = ev[e];
+  keys[i] = pass == 0   ? static_cast<uint64_t>(o.t_cross - tc_min)
+            : pass == 1 ? (static_cast<uint64_t>(o.x) << y_bits) | o.y
+                        : static_cast<uint64_t>(o.t_reported - tr_min);
+}
+
+__global__ void gather_kernel(const OutEventDev* ev, const uint32_t* perm, size_t n, int64_t cut, OutEventDev* sorted,
+                              unsigned long long* scratch) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool ready = false;
+  if (i < n) {
+    const OutEventDev o = ev[perm[i]];
+    sorted[i] = o;
+    ready = o.t_reported < cut;
+  }
+  const unsigned long long m = __ballot(ready);
+  if (__lane_id() == 0 && m != 0) atomicAdd(scratch + 5, static_cast<unsigned long long>(__popcll(m)));
+}
+
+__global__ void columns_kernel(const OutEventDev* ev, size_t n, int64_t t_offset_us, uint16_t* x, uint16_t* y,
+                               uint8_t* p, int64_t* t_us, uint16_t* t_ns_rem, int64_t* t_rep, int64_t* t_cross,
+                               uint8_t* label, uint16_t* flags) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const OutEventDev o = ev[i];
+  const int64_t us = floor_div64(o.t_reported, 1000);
+  x[i] = o.x;
+  y[i] = o.y;
+  p[i] = o.p;
+  t_us[i] = us - t_offset_us;
+  t_ns_rem[i] = static_cast<uint16_t>(o.t_reported - 1000 * us);
+  t_rep[i] = o.t_reported;
+  t_cross[i] = o.t_cross;
+  label[i] = o.label;
+  flags[i] = o.flags;
+}
+
+inline unsigned grid(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+}  // namespace
+
+hipError_t launch_out_build(const RawEventDev* raw, size_t n, uint32_t width, const Readout& r, bool crop,
+                            const CropBox& box, OutEventDev* out, unsigned long long* scratch, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  build_kernel<<<grid(n), 256, 0, stream>>>(raw, n, width, r, crop ? 1 : 0, box, out, scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_out_minmax(const OutEventDev* ev, size_t n_held, size_t n_max, unsigned long long* scratch,
+                             hipStream_t stream) {
+  if (n_max == 0) return hipSuccess;
+  minmax_kernel<<<grid(n_max), 256, 0, stream>>>(ev, n_held, n_max, scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_out_keys(const OutEventDev* ev, const uint32_t* perm, uint32_t* perm_init, size_t n, int pass,
+                           int y_bits, int64_t tc_min, int64_t tr_min, uint64_t* keys, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  keys_kernel<<<grid(n), 256, 0, stream>>>(ev, perm, perm_init, n, pass, y_bits, tc_min, tr_min, keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_out_sort(uint64_t* keys, uint64_t* keys_out, uint32_t* perm, uint32_t* perm_out, size_t n, int bits,
+                           DeviceBuffer& temp, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  size_t bytes = 0;
+  hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, keys, keys_out, perm, perm_out, n, 0u,
+                                           static_cast<unsigned>(bits), stream);
+  if (e != hipSuccess) return e;
+  void* t = temp.ensure(bytes > 0 ? bytes : 1);
+  return rocprim::radix_sort_pairs(t, bytes, keys, keys_out, perm, perm_out, n, 0u, static_cast<unsigned>(bits),
+                                   stream);
+}
+
+hipError_t launch_out_gather(const OutEventDev* ev, const uint32_t* perm, size_t n, int64_t cut, OutEventDev* sorted,
+                             unsigned long long* scratch, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  gather_kernel<<<grid(n), 256, 0, stream>>>(ev, perm, n, cut, sorted, scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_out_columns(const OutEventDev* ev, size_t n, int64_t t_offset_us, uint16_t* x, uint16_t* y,
+                              uint8_t* p, int64_t* t_us, uint16_t* t_ns_rem, int64_t* t_rep, int64_t* t_cross,
+                              uint8_t* label, uint16_t* flags, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  columns_kernel<<<grid(n), 256, 0, stream>>>(ev, n, t_offset_us, x, y, p, t_us, t_ns_rem, t_rep, t_cross, label,
+                                              flags);
+  return hipGetLastError();
+}
+
+}  // namespace bsx

@@ -93,15 +93,4 @@ Readout readout_of(const SensorParams& p, uint64_t seed, uint32_t camera) {
   return r;
 }
 
-int64_t reported_time(const Readout& r, int64_t t_cross_ns) {
-  int64_t t = t_cross_ns + r.delay_ns;
-  if (r.quantum_ns > 0) {
-    const int64_t d = t - r.phase_ns;
-    int64_t q = d / r.quantum_ns;
-    if (d % r.quantum_ns < 0) --q;  // floor division
-    t = r.phase_ns + r.quantum_ns * q;
-  }
-  return t + r.clock_offset_ns + std::llround(r.clock_drift_ppm * 1e-6 * static_cast<double>(t));
-}
-
 }  // namespace bsx

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "params.h"
+#include "readout.h"
 
 namespace bsx {
 
@@ -25,12 +26,7 @@ struct SensorMaps {
 SensorMaps draw_sensor_maps(const SensorParams& p, uint64_t seed, uint32_t camera, uint32_t n_pixels,
                             int num_threads = 0);
 
-// The readout of one crossing (reference: Readout / reported_time, src/sensor/readout.h:13-26).
-struct Readout {
-  int64_t quantum_ns, delay_ns, phase_ns, clock_offset_ns;
-  double clock_drift_ppm;
-};
+// The readout of a camera (the counter phase drawn from seed and camera).
 Readout readout_of(const SensorParams& p, uint64_t seed, uint32_t camera);
-int64_t reported_time(const Readout& r, int64_t t_cross_ns);
 
 }  // namespace bsx

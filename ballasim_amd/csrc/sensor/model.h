@@ -12,6 +12,7 @@
 
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "kconst.h"
@@ -33,6 +34,10 @@ class DeviceBuffer {
   void* ensure(size_t bytes);  // keeps the old contents only when no growth is needed
   void* get() const { return p_; }
   size_t bytes() const { return bytes_; }
+  void swap(DeviceBuffer& o) noexcept {
+    std::swap(p_, o.p_);
+    std::swap(bytes_, o.bytes_);
+  }
   template <class T>
   T* as() const {
     return static_cast<T*>(p_);
@@ -104,6 +109,9 @@ class SensorModel {
   LaunchChain& chain() { return *chain_; }
   bsx_sensor_info_t info() const;
   const SensorMaps& maps() const { return maps_; }
+  const Readout& readout() const { return readout_; }
+  uint32_t width() const { return width_; }
+  uint32_t height() const { return height_; }
   const SensorParams& params() const { return params_; }
 
  private:

@@ -128,6 +128,10 @@ def ref_lib():
         lib.ref_reported.argtypes = [P, P]
         lib.ref_maps.argtypes = [P, P, P, P, P, P]
         lib.ref_states.argtypes = [P, P]
+        lib.ref_output_order.argtypes = [P, P, P, P, C.c_size_t, C.c_uint32, P]
+        lib.ref_readout.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_double, P, C.c_size_t, P,
+                                    C.c_int64, P]
+        lib.ref_label_s1.argtypes = [P, C.c_size_t, P]
         lib.ref_stats.argtypes = [P, P]
         _ref = lib
     return _ref

@@ -10,7 +10,9 @@
 #include <vector>
 
 #include "../include/bsx/bsx_sensor.h"
+#include "sensor/output_order.h"
 #include "sensor/params.h"
+#include "sensor/readout.h"
 #include "sensor/sensor_model.h"
 
 using namespace balla::sensor;
@@ -216,6 +218,28 @@ void ref_stats(void* h, uint64_t* out) {
   const uint64_t v[10] = {s.substeps, s.substep_attempts, s.settled_substeps, s.cells,      s.refinements,
                           s.leaf_intervals, s.fallbacks, s.input_clamps,     s.quarantines, s.pieces};
   std::memcpy(out, v, sizeof(v));
+}
+
+// The reference's output order (output_order.h) of n events: perm[i] = the index of the i-th event.
+int ref_output_order(const int64_t* t_cross, const uint16_t* x, const uint16_t* y, const int64_t* t_reported,
+                     size_t n, uint32_t height, uint32_t* perm) {
+  std::vector<uint32_t> p;
+  output_order({t_cross, n}, {x, n}, {y, n}, {t_reported, n}, order_y_bits(height), p);
+  std::memcpy(perm, p.data(), n * sizeof(uint32_t));
+  return 0;
+}
+
+// reported_time of n crossings, and the window cut of t_end, under one readout.
+void ref_readout(int64_t quantum, int64_t delay, int64_t phase, int64_t offset, double drift_ppm,
+                 const int64_t* t_cross, size_t n, int64_t* out, int64_t t_end, int64_t* cut) {
+  const Readout r{quantum, delay, phase, offset, drift_ppm};
+  for (size_t i = 0; i < n; ++i) out[i] = reported_time(r, t_cross[i]);
+  *cut = output_cut(r, t_end);
+}
+
+// The S1 label of each sensor label (label_s1).
+void ref_label_s1(const uint8_t* c3, size_t n, uint8_t* out) {
+  for (size_t i = 0; i < n; ++i) out[i] = label_s1(c3[i]);
 }
 
 }  // extern "C"
