@@ -43,7 +43,7 @@ ADVANCE_CHAIN = {
 # Measured HBM traffic of the instantaneous chain on this workload, bytes per sample: rocprofv3
 # FETCH_SIZE x 2 + WRITE_SIZE summed over the chain's kernels, averaged over the timed launches (the
 # factor 2 and 1 from a 1 GiB device copy read by the same counters: profiles/advance_chain_1280x720.md).
-TRAFFIC_BYTES_PER_SAMPLE = 68.8
+TRAFFIC_BYTES_PER_SAMPLE = 69.9
 
 
 def sensor_params(bsx, photoreceptor):
@@ -143,10 +143,10 @@ def bench_rig(args):
 
     cams = cameras()
     rig = Rig(cams)
-    trials = rig.autotune(runs[0], WINDOW, reps=2)
-    tuning = rig.tuning()
     for i in range(args.warmup):
         rig.advance_device(runs[i], (i + 1) * WINDOW)
+    trials = rig.autotune(runs[args.warmup], (args.warmup + 1) * WINDOW, reps=2)
+    tuning = rig.tuning()
     for m in cams:
         m.reset_stats()
     k0 = rig.stats()["kernel_ms"]
@@ -247,14 +247,144 @@ def cpu_baseline_rig(p, runs, views, scene, budget_s):
             "events_per_s": events / elapsed}
 
 
+MLP_FWD_FLOP = 2 * (12 * 64 + 64 * 64 + 64)  # 9,856 FLOP per forward row (multiply-adds of the three layers)
+MLP_TRAIN_FLOP = 3 * MLP_FWD_FLOP  # forward + backward (input and weight gradients) per training row
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_learned(args):
+    """Trains the proposer on dense-trace examples of seeded config-1-style scenes, then compares the
+    learned controller with the plain one on a held-out 1280x720 scene against a dense-sampled reference."""
+    import numpy as np
+    import torch
+
+    import ballasim_amd as bsx
+    from ballasim_amd import scene as S
+    from ballasim_amd.learned import LearnedSampler, Proposer, event_error, learned_examples, learned_rules
+
+    p = rig_params(bsx, "instantaneous")
+    dark = float(p.dark_current_fA) / float(p.illuminance_scale)
+    band, tol, margin = 0.02, 0.005, 0.5
+    n_train, n_scenes = args.train_examples, 20
+    n_per = n_train // n_scenes
+    X = torch.empty((n_per * n_scenes, 12), dtype=torch.float32, device="cuda")
+    T = torch.empty(n_per * n_scenes, dtype=torch.float32, device="cuda")
+    t0 = time.perf_counter()
+    for i in range(n_scenes):
+        sc, v = S.config1_scene(1000 + i)
+        learned_examples(sc, v, X[i * n_per:(i + 1) * n_per], T[i * n_per:(i + 1) * n_per], dark, t0_ns=30_000_000,
+                         seed=1000 + i, theta_band=band, tol_log=tol)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    prop = Proposer(1)
+    w = prop.params
+    w[-1] = float(T.mean())
+    prop.params = w
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    schedule = ((3e-3, 1), (1e-3, 1), (3e-4, 1))
+    e0.record()
+    for lr, ep in schedule:
+        loss = prop.train(X, T, batch=4096, epochs=ep, lr=lr)
+    e1.record()
+    torch.cuda.synchronize()
+    train_ms = e0.elapsed_time(e1)
+    rows = X.shape[0] * sum(ep for _, ep in schedule)
+    train_tflops = rows * MLP_TRAIN_FLOP / (train_ms * 1e-3) / 1e12
+    # inference throughput of the batched forward kernel
+    Y = torch.empty(X.shape[0], dtype=torch.float32, device="cuda")
+    prop.predict(X, Y)
+    e0.record()
+    for _ in range(3):
+        prop.predict(X, Y)
+    e1.record()
+    torch.cuda.synchronize()
+    infer_ms = e0.elapsed_time(e1) / 3
+    infer_tflops = X.shape[0] * MLP_FWD_FLOP / (infer_ms * 1e-3) / 1e12
+    del X, T, Y
+    torch.cuda.empty_cache()
+
+    sc, v = S.learned_eval_scene(args.seed + 7)
+    frame = sc.render(v, 0)
+    n_win = max(1, int(round(args.eval_ms * 1e6 / WINDOW)))
+
+    def run(kind):
+        m = SensorModel(p, v.width, v.height, 1, ["constant"], 0, device="cuda")
+        m.init(0, frame.reshape(-1, 1))
+        ls = None
+        if kind == "learned":
+            ls = LearnedSampler(sc, v, learned_rules(S.rules(dark, tol=tol), tol_log=tol, theta_band=band,
+                                                     margin=margin), prop)
+            ls.init(0)
+        evs, samples, fallbacks, proposals = [], 0, 0, 0
+        for i in range(n_win):
+            if kind == "dense":
+                r, tot = sc.sample_window(v, S.rules(dark, fixed_period_ns=10_000), i * WINDOW, slot=0)
+            elif kind == "plain":
+                r, tot = sc.sample_window(v, S.rules(dark, tol=0.01), i * WINDOW, slot=0)
+            else:
+                r, tot, pr, fb = ls.window(i * WINDOW, slot=0)
+                fallbacks += fb
+                proposals += pr
+            samples += tot
+            ptr, cnt = m.advance_device_runs(r, (i + 1) * WINDOW)
+            evs.append(m.download_events(ptr, cnt))
+        ev = {k: np.concatenate([e[k] for e in evs]) for k in evs[0]}
+        return ev, samples, proposals, fallbacks
+
+    from ballasim_amd.sensor import SensorModel
+    ref, ref_samples, _, _ = run("dense")
+    res = {}
+    for kind in ("plain", "learned"):
+        ev, samples, proposals, fallbacks = run(kind)
+        err = event_error(ev, ref, v.width)
+        res[kind] = dict(samples=samples, events=err["n_test"], samples_per_event=samples / max(1, err["n_test"]),
+                         count_mismatch=err["count_mismatch"], median_abs_dt_ns=err["median_abs_dt_ns"],
+                         p90_abs_dt_ns=err["p90_abs_dt_ns"], proposals=proposals, fallbacks=fallbacks)
+    line = {
+        "metric": "pixel-samples/s + events/s on 1 MI355X; irradiance samples/event at fixed error",
+        "value": res["learned"]["samples_per_event"],
+        "unit": "irradiance samples/event",
+        "n_gpus": 1, "steps": n_win, "warmup": 0, "ms_per_step": None,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (MFMA, fp32 accumulation)",
+        "data": "synthetic (config-1-style procedural scenes: training seeds 1000-1019, evaluation seed %d)"
+                % (args.seed + 7),
+        "config": {"workload": "learned proposer 12-64-64-1 SiLU trained on %d dense-trace examples (dt 10 us) of "
+                               "20 seeded 346x260 disk scenes; evaluated at 1280x720 (12 textured disks, 800 px/s, "
+                               "100 lux) over %d windows (%.0f ms) against a 10-us dense reference; thresholds "
+                               "0.2 +- 0.02, refractory 100 us, instantaneous response, noise off"
+                               % (n_per * n_scenes, n_win, n_win * WINDOW / 1e6),
+                   "band": band, "tol_log": tol, "margin": margin},
+        "plain": res["plain"], "learned": res["learned"],
+        "dense_reference": {"samples": ref_samples, "events": int(len(ref["t_cross_ns"]))},
+        "samples_per_event_gain_vs_plain": res["plain"]["samples_per_event"] / res["learned"]["samples_per_event"],
+        "training": {"examples": int(n_per * n_scenes), "generation_s": gen_s, "rows_trained": int(rows),
+                     "train_ms": train_ms, "rows_per_s": rows / (train_ms * 1e-3), "final_loss": loss},
+        "roofline": {"bound": "mfma", "achieved": train_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": train_tflops / BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "gradients_kernel + adam_kernel (training step)",
+                     "inference_tflops": infer_tflops, "inference_rows_per_s": X_rows_per_s(infer_ms, n_per * n_scenes)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def X_rows_per_s(ms, rows):
+    return rows / (ms * 1e-3)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed windows (default 58; rig: 8)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--photoreceptor", default="instantaneous", choices=["instantaneous", "cascade"])
-    ap.add_argument("--workload", default="plain", choices=["plain", "rig"],
-                    help="plain: the 1280x720 sensor (default); rig: the 4-camera ragged batch")
+    ap.add_argument("--workload", default="plain", choices=["plain", "rig", "learned"],
+                    help="plain: the 1280x720 sensor (default); rig: the 4-camera ragged batch; learned: the "
+                         "proposer's training and the learned controller against the plain one")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shape (no autotuner trials before the timed windows)")
+    ap.add_argument("--train-examples", type=int, default=20_000_000, help="learned: training examples")
+    ap.add_argument("--eval-ms", type=float, default=500.0, help="learned: evaluation span (ms)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work (0: skip)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -279,6 +409,8 @@ def main():
         args.steps = 8 if args.workload == "rig" else 58
     if args.workload == "rig":
         return bench_rig(args)
+    if args.workload == "learned":
+        return bench_learned(args)
     import numpy as np
 
     import ballasim_amd as bsx
@@ -303,16 +435,24 @@ def main():
         totals.append(total)
     gen_s = time.perf_counter() - t_gen
     print(f"# generated {n_win} windows, {sum(totals)} samples in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    # The walk's launch shape from the autotuner, timed on the first timed window's input after the warm-up
+    # windows (the camera as a rig of one; the autotuner changes no state).
+    from ballasim_amd.sensor import Rig
+    rig = Rig([sensor])
     for i in range(args.warmup):
-        sensor.advance_device_runs(runs[i], (i + 1) * WINDOW)
+        rig.advance_device([runs[i]], (i + 1) * WINDOW)
+    trials = rig.autotune([runs[args.warmup]], (args.warmup + 1) * WINDOW, reps=2) if args.autotune else []
+    tuning = rig.tuning()
     sensor.reset_stats()
+    k0 = rig.stats()["kernel_ms"]
     ev_counts = []
     t0 = time.perf_counter()
     for i in range(args.warmup, n_win):
-        _, count = sensor.advance_device_runs(runs[i], (i + 1) * WINDOW)
+        (_, count), = rig.advance_device([runs[i]], (i + 1) * WINDOW)
         ev_counts.append(count)
-    t1 = time.perf_counter()  # advance_device returns after its launch and counter read-back (synchronized)
-    st = sensor.stats()
+    t1 = time.perf_counter()  # a rig call returns after its graph launch and counter read-back (synchronized)
+    st = dict(sensor.stats())
+    st["kernel_ms"] = rig.stats()["kernel_ms"] - k0
     elapsed = t1 - t0
     samples = sum(totals[args.warmup:])
     events = sum(ev_counts)
@@ -340,6 +480,8 @@ def main():
                    "pixels": P, "samples": samples, "events": events},
         "events_per_s": events / elapsed,
         "samples_per_event": samples / max(events, 1),
+        "tuning": tuning,
+        "autotune_trials": len(trials),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (TRAFFIC_BYTES_PER_SAMPLE * samples / args.steps
