@@ -176,6 +176,7 @@ SIGNATURES = {
     "bsx_output_destroy": (C.c_int, [P]),
     "bsx_output_window": (C.c_int, [P, C.POINTER(DeviceEvents), C.c_int64, P, P]),
     "bsx_output_flush": (C.c_int, [P, P, P]),
+    "bsx_scene_render_requests": (C.c_int, [P, C.POINTER(View), C.c_int64, P, C.c_size_t, P, C.c_int32, P, P, P, P]),
     "bsx_draw_maps": (C.c_int, [C.POINTER(Params), C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P]),
     "bsx_scene_create": (C.c_int, [C.POINTER(Shape), C.c_int32, C.c_float, C.POINTER(P)]),
     "bsx_scene_destroy": (C.c_int, [P]),

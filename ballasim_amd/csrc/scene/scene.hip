@@ -147,6 +147,30 @@ __global__ __launch_bounds__(256) void counts_write_kernel(SceneDev sc, ViewDev 
   irr[s] = eval_scene(sc, mask, x, y, static_cast<double>(t_w) * 1e-9 + static_cast<double>(dt) * 1e-9).e;
 }
 
+// The requests of the engine's RenderHook: request i (pixel, time t_base + t_off_ns) writes the scene's
+// irradiance at the pixel centre into row out_index[i] of irr (group 0; further groups 0), its
+// signature into sig[out_index[i]] and, when klass is not null, the class word (textured shape or
+// background) into klass[out_index[i]].
+__global__ __launch_bounds__(256) void requests_kernel(SceneDev sc, ViewDev view, int64_t t_base,
+                                                       const SampleRequestDev* req, size_t n, const uint32_t* out_index,
+                                                       int32_t groups, float* irr, uint64_t* sig, uint32_t* klass) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SampleRequestDev r = req[i];
+  const uint32_t p = r.pixel;
+  const double x = view.ox + (static_cast<double>(p % view.width) + 0.5) * view.scale;
+  const double y = view.oy + (static_cast<double>(p / view.width) + 0.5) * view.scale;
+  uint64_t mask[kMaskWords];
+  for (int w = 0; w < kMaskWords; ++w) mask[w] = 0;
+  for (int k = 0; k < sc.n; ++k) mask[k >> 6] |= 1ull << (k & 63);
+  const int64_t t = t_base + r.t_off_ns;
+  const Sample s = eval_scene(sc, mask, x, y, static_cast<double>(t) * 1e-9);
+  const uint32_t row = out_index != nullptr ? out_index[i] : static_cast<uint32_t>(i);
+  for (int g = 0; g < groups; ++g) irr[static_cast<int64_t>(row) * groups + g] = g == 0 ? s.e : 0.0f;
+  sig[row] = s.sig;
+  if (klass != nullptr) klass[row] = s.sig != 0 ? kClassTexturedWord : 0u;
+}
+
 // Exclusive scan of int32 counts into int64 offsets: per-block scans (wave shuffles, then the
 // block's wave totals through LDS), the block totals scanned by one block, then added back.
 constexpr int kScanBlock = 1024;
@@ -253,6 +277,15 @@ hipError_t launch_counts_write(const SceneDev& sc, const ViewDev& view, int64_t 
   if (total <= 0) return hipSuccess;
   counts_write_kernel<<<static_cast<unsigned>((total + 255) / 256), 256, 0, stream>>>(
       sc, view, t_w, counts, row_start, view.width * view.height, total, t, irr);
+  return hipGetLastError();
+}
+
+hipError_t launch_requests(const SceneDev& sc, const ViewDev& view, int64_t t_base, const SampleRequestDev* req,
+                           size_t n, const uint32_t* out_index, int32_t groups, float* irr, uint64_t* sig,
+                           uint32_t* klass, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  requests_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(sc, view, t_base, req, n, out_index,
+                                                                             groups, irr, sig, klass);
   return hipGetLastError();
 }
 

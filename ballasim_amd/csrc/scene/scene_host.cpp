@@ -423,4 +423,23 @@ int bsx_learned_sampler_window(bsx_learned_sampler* ls, int64_t t_w, int32_t slo
   });
 }
 
+int bsx_scene_render_requests(bsx_scene* s, const bsx_view* view, int64_t t_base_ns, const bsx_sample_request* requests,
+                              size_t n, const uint32_t* out_index, int32_t groups, float* irr, uint64_t* sig,
+                              uint32_t* klass, void* stream) {
+  return scene_guarded([&] {
+    if (s == nullptr || view == nullptr) throw std::invalid_argument("bsx_scene_render_requests: null");
+    if (groups < 1 || groups > BSX_MAX_LIGHT_GROUPS) {
+      throw std::invalid_argument("bsx_scene_render_requests: groups must lie in [1, 8]");
+    }
+    if (n == 0) return;
+    if (requests == nullptr || irr == nullptr || sig == nullptr) {
+      throw std::invalid_argument("bsx_scene_render_requests: null requests or outputs");
+    }
+    const bsx::ViewDev v = bsx::Scene::view_of(*view);
+    bsx::hip_check(bsx::launch_requests(s->s->dev(), v, t_base_ns, reinterpret_cast<const bsx::SampleRequestDev*>(requests),
+                                        n, out_index, groups, irr, sig, klass, static_cast<hipStream_t>(stream)),
+                   "render requests");
+  });
+}
+
 }  // extern "C"

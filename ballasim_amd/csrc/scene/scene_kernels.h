@@ -7,6 +7,24 @@
 
 namespace bsx {
 
+// The engine's sample request (reference: balla::backend::SampleRequest, src/backend/api.h:25-33), 24 bytes.
+struct SampleRequestDev {
+  uint16_t cam;
+  uint16_t flags;
+  uint32_t pixel;
+  uint32_t k_begin;
+  uint32_t k_count;
+  int32_t t_off_ns;
+  uint32_t pose_slot;
+};
+static_assert(sizeof(SampleRequestDev) == 24, "the engine's 24-byte SampleRequest");
+// The class word of a sample on a textured shape (reference: kClassTextured, src/render/types.h:47).
+inline constexpr uint32_t kClassTexturedWord = 1u << 13;
+
+hipError_t launch_requests(const SceneDev& sc, const ViewDev& view, int64_t t_base, const SampleRequestDev* req,
+                           size_t n, const uint32_t* out_index, int32_t groups, float* irr, uint64_t* sig,
+                           uint32_t* klass, hipStream_t stream);
+
 hipError_t launch_render(const SceneDev& sc, const ViewDev& view, int64_t t_ns, float* out, hipStream_t stream);
 hipError_t launch_count(const SceneDev& sc, const ViewDev& view, const bsx_sampler_rules& r, int64_t t_w,
                         int32_t* counts, hipStream_t stream);

@@ -62,6 +62,43 @@ class Scene:
                                                      C.byref(total)))
         return runs, total.value
 
+    def render_requests(self, view, t_base_ns, requests, n, out_index, irr, sig, klass=None, groups=1, stream=0):
+        """The engine's RenderHook::render on this scene for n requests: device tensors of the 24-byte
+        requests (REQUEST_DTYPE records, any dtype holding n * 24 bytes), out_index (n u32 rows, or None for
+        row i), irr (rows x groups f32), sig (rows u64) and klass (rows u32, or None). The sizes are checked
+        here, before the launch: every output row a request names must exist."""
+        n = int(n)
+        if n < 0:
+            raise ValueError("render_requests: n must be >= 0")
+
+        def nbytes(a):
+            return a.numel() * a.element_size()
+
+        rows = n
+        if out_index is not None:
+            if nbytes(out_index) < 4 * n:
+                raise ValueError("render_requests: out_index holds fewer than n entries")
+            if n:
+                rows = int(out_index[:n].to(dtype=__import__("torch").int64).max()) + 1
+                if int(out_index[:n].to(dtype=__import__("torch").int64).min()) < 0:
+                    raise ValueError("render_requests: negative output row")
+        if nbytes(requests) < 24 * n:
+            raise ValueError("render_requests: requests holds fewer than n 24-byte records")
+        if nbytes(irr) < 4 * groups * rows or nbytes(sig) < 8 * rows or (klass is not None and nbytes(klass) < 4 * rows):
+            raise ValueError("render_requests: an output array is smaller than the rows the requests name")
+
+        def addr(a):
+            return None if a is None else C.c_void_p(a.data_ptr())
+        st = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
+        _abi.check(self._lib.bsx_scene_render_requests(self._h, C.byref(view), int(t_base_ns), addr(requests), n,
+                                                       addr(out_index), int(groups), addr(irr), addr(sig),
+                                                       addr(klass), C.c_void_p(st)))
+
+
+# The engine's SampleRequest (src/backend/api.h:25-33) as a numpy record.
+REQUEST_DTYPE = np.dtype([("cam", np.uint16), ("flags", np.uint16), ("pixel", np.uint32), ("k_begin", np.uint32),
+                          ("k_count", np.uint32), ("t_off_ns", np.int32), ("pose_slot", np.uint32)])
+
 
 def runs_to_host(runs, groups=1):
     """A copy of device runs on the host: (ids, t, irr (S, groups), offsets (n + 1))."""

@@ -87,6 +87,29 @@ int bsx_scene_sample_window(bsx_scene* s, const bsx_view* view, const bsx_sample
 int bsx_scene_sample_counts(bsx_scene* s, const bsx_view* view, const int32_t* counts, int64_t t_w, int32_t slot,
                             void* stream, bsx_device_runs* out, uint64_t* total_samples);
 
+/* The engine's sample request (reference: balla::backend::SampleRequest, src/backend/api.h:25-33), the same
+ * 24-byte layout, so the engine's request arrays are passed as they are. */
+typedef struct bsx_sample_request {
+  uint16_t cam;
+  uint16_t flags;
+  uint32_t pixel;     /* y * width + x */
+  uint32_t k_begin;   /* sub-sample range (the analytic scene is evaluated at the pixel centre for all of them) */
+  uint32_t k_count;
+  int32_t t_off_ns;   /* time = t_base_ns + t_off_ns */
+  uint32_t pose_slot;
+} bsx_sample_request;
+
+/* The engine's RenderHook::render on the device scene (reference: src/engine/window.h:66-80; the time base as
+ * the engine's hook adapter forms it, src/engine/bindings/engine_module.cpp:236): request i writes the scene's
+ * irradiance at its pixel centre and time into row out_index[i] of irr (groups floats per row: the scene's
+ * one light in group 0, 0 in the others), its signature (1 + shape index, 0 = background) into
+ * sig[out_index[i]] and, when klass is not null, the class word (1 << 13, textured, on a shape; 0 on the
+ * background) into klass[out_index[i]]. out_index null: row i. All arrays in device memory; work queued on
+ * `stream`. */
+int bsx_scene_render_requests(bsx_scene* s, const bsx_view* view, int64_t t_base_ns, const bsx_sample_request* requests,
+                              size_t n, const uint32_t* out_index, int32_t groups, float* irr, uint64_t* sig,
+                              uint32_t* klass, void* stream);
+
 /* Copies `bytes` from device memory to host memory (synchronous): a caller without its own HIP
  * bindings reads runs or raw events with it. */
 int bsx_copy_to_host(void* dst, const void* src, size_t bytes);

@@ -185,3 +185,40 @@ def test_runs_of_smaller_view_survive_a_larger_view():
     scene.sample_counts(big, S.lognormal_counts(rng, big.width * big.height, hi=8), 0, slot=1)
     ids, _, _, _ = S.runs_to_host(r0)
     assert np.array_equal(ids, np.arange(small.width * small.height))
+
+
+def test_render_requests_equal_frames():
+    """The RenderHook seam: requests at (pixel, t_base + t_off) in engine order, results in out_index rows,
+    equal the frame rendered at each request's time (the same scene evaluation)."""
+    import torch
+
+    from ballasim_amd import scene as S
+    scene, view = S.config1_scene(4)
+    P = view.width * view.height
+    rng = np.random.default_rng(8)
+    n, G = 4000, 2
+    t_base = 5 * S.WINDOW_NS
+    offs = np.array([0, 1_000_000, 2_500_000, 8_000_000], np.int64)
+    req = np.zeros(n, S.REQUEST_DTYPE)
+    req["pixel"] = rng.integers(0, P, n)
+    req["t_off_ns"] = offs[rng.integers(0, len(offs), n)]
+    req["k_count"] = 64
+    out_index = rng.permutation(n).astype(np.uint32)
+    dev = torch.device("cuda:0")
+    rq = torch.from_numpy(req.view(np.uint8).copy()).to(dev)
+    oi = torch.from_numpy(out_index.view(np.int32).copy()).to(dev)
+    irr = torch.full((n, G), -1.0, dtype=torch.float32, device=dev)
+    sig = torch.zeros(n, dtype=torch.int64, device=dev)
+    kl = torch.zeros(n, dtype=torch.int32, device=dev)
+    with pytest.raises(ValueError):  # a request count beyond the arrays is refused before any launch
+        scene.render_requests(view, t_base, rq, n + 1, oi, irr, sig, kl, groups=G)
+    scene.render_requests(view, t_base, rq, n, oi, irr, sig, kl, groups=G)
+    torch.cuda.synchronize()
+    irr, sig, kl = irr.cpu().numpy(), sig.cpu().numpy(), kl.cpu().numpy()
+    frames = {int(o): scene.render(view, t_base + int(o)) for o in offs}
+    for i in range(n):
+        row = out_index[i]
+        assert irr[row, 0] == frames[int(req["t_off_ns"][i])][req["pixel"][i]]
+        assert irr[row, 1] == 0.0
+        assert (kl[row] != 0) == (sig[row] != 0)
+    assert (sig != 0).any() and (sig == 0).any()
