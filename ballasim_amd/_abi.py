@@ -170,8 +170,8 @@ SIGNATURES = {
     "bsx_learned_sampler_create": (C.c_int, [P, C.POINTER(View), P, P, C.POINTER(P)]),
     "bsx_learned_sampler_destroy": (C.c_int, [P]),
     "bsx_learned_sampler_init": (C.c_int, [P, C.c_int64, P]),
-    "bsx_learned_sampler_window": (C.c_int, [P, C.c_int64, C.c_int32, P, C.POINTER(DeviceRuns), C.POINTER(C.c_uint64),
-                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "bsx_learned_sampler_window": (C.c_int, [P, C.c_int64, C.c_int32, P, P, C.POINTER(DeviceRuns),
+                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "bsx_output_create": (C.c_int, [P, P, C.c_int64, C.POINTER(P)]),
     "bsx_output_destroy": (C.c_int, [P]),
     "bsx_output_window": (C.c_int, [P, C.POINTER(DeviceEvents), C.c_int64, P, P]),

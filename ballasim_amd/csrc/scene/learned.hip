@@ -12,6 +12,8 @@ namespace {
 
 using mlp::kIn;
 
+constexpr int64_t kNotBlindLink = INT64_MIN;  // the sensor state's "not blind" (no pending release)
+
 // The controller's view of a pixel: up to four latest samples (index 0 the newest) of the log
 // irradiance L = ln(E + dark), their times, and the reference level it tracks as an ideal event
 // sensor of threshold theta_ref would (ref moves by theta_ref per crossing).
@@ -22,7 +24,24 @@ struct History {
   float ref;
   float e0;      // the newest sample's irradiance
   uint32_t sig;  // and its signature (the shape seen, 0: background)
-  __device__ __forceinline__ void push(int64_t tn, float en, uint32_t sn, float Ln, float theta_ref) {
+  // The band about ref in which no event starts: [ref - th_off, ref + th_on]; a pending release of a
+  // physical reset (kNoRelease: none) leaves the reference unknown until it passes.
+  float th_on, th_off;
+  bool coupled, ideal;
+  float eps_ci;
+  int64_t refractory;
+  int64_t release;
+  static constexpr int64_t kNoRelease = INT64_MIN;
+
+  // Room to the nearer band edge at level L (0 while a release is pending).
+  __device__ __forceinline__ float room(float Lv) const {
+    if (release != kNoRelease) return 0.0f;
+    return fminf(ref + th_on - Lv, Lv - (ref - th_off));
+  }
+
+  __device__ __forceinline__ void push(int64_t tn, float en, uint32_t sn, float Ln, float dark) {
+    const int64_t tp = t[0];
+    const float ep = e0;
     for (int k = 3; k > 0; --k) {
       t[k] = t[k - 1];
       L[k] = L[k - 1];
@@ -32,8 +51,38 @@ struct History {
     e0 = en;
     sig = sn;
     n = n < 4 ? n + 1 : 4;
-    while (Ln - ref >= theta_ref) ref += theta_ref;
-    while (ref - Ln >= theta_ref) ref -= theta_ref;
+    // The sensor interpolates linearly in irradiance between samples: the log level at time tx.
+    const auto at = [&](int64_t tx) {
+      const float w = tn > tp ? static_cast<float>(static_cast<double>(tx - tp) / static_cast<double>(tn - tp)) : 1.0f;
+      return logf(ep + (en - ep) * w + dark);
+    };
+    if (!coupled) {  // an ideal sensor of step th_on
+      while (Ln - ref >= th_on) ref += th_on;
+      while (ref - Ln >= th_on) ref -= th_on;
+      return;
+    }
+    if (release != kNoRelease && release <= tn) {
+      ref = at(release) + eps_ci;
+      release = kNoRelease;
+    }
+    if (release != kNoRelease) return;
+    const float d = Ln - ref;
+    if (!(d >= th_on || d <= -th_off)) return;
+    if (ideal) {
+      while (Ln - ref >= th_on) ref += th_on;
+      while (ref - Ln >= th_off) ref -= th_off;
+      return;
+    }
+    // physical reset: the crossing between the two samples (log-linear estimate), then blind for the
+    // refractory period and a new reference at the release
+    const float Lp = at(tp);
+    const float edge = d >= th_on ? ref + th_on : ref - th_off;
+    const float f = Ln != Lp ? fminf(fmaxf((edge - Lp) / (Ln - Lp), 0.0f), 1.0f) : 1.0f;
+    release = tp + static_cast<int64_t>(f * static_cast<float>(tn - tp)) + refractory;
+    if (release <= tn) {
+      ref = at(release) + eps_ci;
+      release = kNoRelease;
+    }
   }
 };
 
@@ -61,7 +110,7 @@ __device__ __forceinline__ void features(const History& h, float theta_band, flo
     hi = fmaxf(hi, L[k]);
   }
   f[10] = 10.0f * (hi - lo);
-  f[11] = 10.0f * (theta_band - fabsf(L[0] - h.ref));
+  f[11] = h.coupled ? 10.0f * h.room(L[0]) : 10.0f * (theta_band - fabsf(L[0] - h.ref));
 }
 
 __device__ __forceinline__ float log_irr(float e, float dark) { return logf(e + dark); }
@@ -145,9 +194,11 @@ __global__ __launch_bounds__(256) void examples_kernel(const float* trace, const
   const int hist = gaps[0] + gaps[1] + gaps[2];
   const int span = J - 1 - max_steps - hist;
   const int j0 = hist + static_cast<int>(unit(hash64(s++)) * span);
-  History h;
+  History h{};
   h.n = 4;
   h.ref = ref[static_cast<int64_t>(j0) * P + p];
+  h.coupled = false;
+  h.release = History::kNoRelease;
   int j = j0;
   for (int k = 0; k < 4; ++k) {
     h.t[k] = static_cast<int64_t>(j) * delta;
@@ -214,8 +265,9 @@ __global__ __launch_bounds__(256) void learned_init_kernel(SceneDev sc, ViewDev 
 template <bool kWrite>
 __global__ __launch_bounds__(64) void learned_window_kernel(SceneDev sc, ViewDev view, LearnedRules r, const uint16_t* wb,
                                                             const float* w, int64_t t_w, LearnedState st,
-                                                            int32_t* counts, const int64_t* row_start, int64_t* t_out,
-                                                            float* irr_out, unsigned long long* work) {
+                                                            SensorLink link, int32_t* counts,
+                                                            const int64_t* row_start, int64_t* t_out, float* irr_out,
+                                                            unsigned long long* work) {
   __shared__ mlp::WeightsLds W;
   __shared__ mlp::WaveLds S;
   mlp::load_weights(W, wb, w);
@@ -238,6 +290,33 @@ __global__ __launch_bounds__(64) void learned_window_kernel(SceneDev sc, ViewDev
     h.e0 = st.e0[p];
     h.sig = st.sig[p];
   }
+  h.release = History::kNoRelease;
+  h.coupled = valid && link.theta_on != nullptr;
+  h.th_on = h.th_off = r.theta_ref;
+  if (h.coupled) {
+    // The sensor's band at the window start: Delta = L + ln(scale / i_scale) - s3_ref - n3_ref + dr_ref (the
+    // controller's L = ln(E + dark) stands for ln(dark_p / scale + E)).
+    h.ref = link.s3_ref[p] + link.n3_ref[p] - link.dr_ref[p] - logf(link.scale[p] / link.i_scale[p]);
+    h.th_on = link.theta_on[p];
+    h.th_off = link.theta_off[p];
+    h.ideal = link.ideal_reset != 0;
+    h.eps_ci = link.eps_ci;
+    h.refractory = link.refractory_ns;
+    const int64_t bu = link.blind_until[p];
+    if (bu != kNotBlindLink && bu >= h.t[0]) h.release = bu;
+  }
+  // The plain controller's lattice step of this pixel (its level rule), the interval of the near mode.
+  int64_t lattice = kWindowNs;
+  if (v_hat > 0.0f) {
+    const double lv = ceil(log2(static_cast<double>(v_hat) / view.scale * (kWindowNs * 1e-9) / r.d_px));
+    const int level = lv != lv ? r.l_min : static_cast<int>(fmin(fmax(lv, static_cast<double>(r.l_min)),
+                                                                 static_cast<double>(r.l_max)));
+    lattice = kWindowNs >> level;
+  }
+  // A pixel that took no samples in earlier windows was static there (no shape reached it): its value at
+  // t_w is its newest sample's. The history moves to t_w with that value (no render, no sample), so that
+  // proposals start at the window and not at the stale sample.
+  if (valid && h.t[0] < t_w) h.push(t_w, h.e0, h.sig, h.L[0], r.dark);
   const int64_t t_end = t_w + kWindowNs;
   bool active = valid && v_hat > 0.0f && h.t[0] < t_end;
   uint32_t count = 0, proposals = 0, fallbacks = 0;
@@ -248,7 +327,7 @@ __global__ __launch_bounds__(64) void learned_window_kernel(SceneDev sc, ViewDev
       irr_out[base + count] = e;
     }
     ++count;
-    h.push(ts, e, sg, log_irr(e, r.dark), r.theta_ref);
+    h.push(ts, e, sg, log_irr(e, r.dark), r.dark);
   };
   const auto at = [&](int64_t ts) { return eval_scene(sc, mask, x, y, static_cast<double>(ts) * 1e-9); };
   while (__ballot(active) != 0) {
@@ -261,7 +340,17 @@ __global__ __launch_bounds__(64) void learned_window_kernel(SceneDev sc, ViewDev
     if (active) {
       ++proposals;
       const double dt_pred = exp(2.0 * static_cast<double>(out) + kLnDtCenter) * r.margin;
-      const int64_t dt = static_cast<int64_t>(fmin(fmax(dt_pred, static_cast<double>(r.min_dt)), 0x1p23));
+      // Near a band edge (or with the reference unknown) the plain controller's density: two lattice steps,
+      // judged by eta_L at the midpoint as the plain controller judges its lattice triples.
+      const float room0 = h.room(h.L[0]);
+      const bool near = room0 < r.near;
+      double dt_max = near ? static_cast<double>(2 * lattice) : 0x1p23;
+      if (!near && h.n >= 2) {
+        // far: no longer than the recent slope needs to reach the near zone
+        const double slope = fabs(static_cast<double>(h.L[0] - h.L[1])) / static_cast<double>(h.t[0] - h.t[1]);
+        if (slope > 0.0) dt_max = fmin(dt_max, static_cast<double>(room0 - r.near) / slope);
+      }
+      const int64_t dt = static_cast<int64_t>(fmin(fmax(fmin(dt_pred, dt_max), static_cast<double>(r.min_dt)), 0x1p23));
       const int64_t ta = h.t[0];
       const int64_t tb = ta + dt < t_end ? ta + dt : t_end;
       const float Ea = h.e0;
@@ -270,14 +359,28 @@ __global__ __launch_bounds__(64) void learned_window_kernel(SceneDev sc, ViewDev
       if (tb - ta < 2) {
         emit(tb, sb.e, sb.sig);
       } else {
-        // The a-posteriori check: the chord's error at the interval's midpoint (rendered, a sample).
+        // The a-posteriori check at the interval's midpoint (rendered, a sample): far from the edges the
+        // chord's log error against the allowed error (tol_log, or the band room less tol_log while the
+        // three values stay inside the band); near them eta_L, as the plain rules. A signature flip
+        // (another shape or the background seen) fails it in both.
         const int64_t tm = ta + (tb - ta) / 2;
         const Sample sm = at(tm);
         const float La = h.L[0], Lb = log_irr(sb.e, r.dark), Lm = log_irr(sm.e, r.dark);
-        const float err = fabsf(logf(0.5f * (Ea + sb.e) + r.dark) - Lm);
-        const float room = r.theta_band - fmaxf(fmaxf(fabsf(La - h.ref), fabsf(Lb - h.ref)), fabsf(Lm - h.ref));
-        const float allowed = room > 0.0f ? fmaxf(r.tol_log, room - r.tol_log) : r.tol_log;
-        if (err <= allowed) {
+        bool ok;
+        if (near) {
+          ok = midpoint_eta(r.dark, Ea, sm.e, sb.e) <= r.tol_eta &&
+               !(r.signature && (sig_a != sm.sig || sm.sig != sb.sig));
+        } else {
+          const float err = fabsf(logf(0.5f * (Ea + sb.e) + r.dark) - Lm);
+          const float room = h.coupled ? fminf(fminf(h.room(La), h.room(Lb)), h.room(Lm))
+                                       : r.theta_band - fmaxf(fmaxf(fabsf(La - h.ref), fabsf(Lb - h.ref)),
+                                                              fabsf(Lm - h.ref));
+          const float allowed = room > 0.0f ? fmaxf(r.tol_log, room - r.tol_log) : r.tol_log;
+          // an interval that reaches the near zone is refined as the plain controller would
+          ok = err <= allowed && !(r.signature && (sig_a != sm.sig || sm.sig != sb.sig)) &&
+               !(h.coupled && room < r.near);
+        }
+        if (ok) {
           emit(tm, sm.e, sm.sig);
           emit(tb, sb.e, sb.sig);
         } else {
@@ -374,17 +477,17 @@ hipError_t launch_learned_init(const SceneDev& sc, const ViewDev& view, int64_t 
 }
 
 hipError_t launch_learned_window(const SceneDev& sc, const ViewDev& view, const LearnedRules& r, const uint16_t* wb,
-                                 const float* w, int64_t t_w, const LearnedStateDev& s, int32_t* counts,
-                                 const int64_t* row_start, int64_t* t_out, float* irr_out, unsigned long long* work,
-                                 bool write, hipStream_t stream) {
+                                 const float* w, int64_t t_w, const LearnedStateDev& s, const SensorLink& link,
+                                 int32_t* counts, const int64_t* row_start, int64_t* t_out, float* irr_out,
+                                 unsigned long long* work, bool write, hipStream_t stream) {
   const uint32_t P = view.width * view.height;
   const LearnedState st{s.t, s.L, s.n, s.ref, s.e0, s.sig};
   if (write)
-    learned_window_kernel<true><<<(P + 63) / 64, 64, 0, stream>>>(sc, view, r, wb, w, t_w, st, counts, row_start,
-                                                                  t_out, irr_out, work);
+    learned_window_kernel<true><<<(P + 63) / 64, 64, 0, stream>>>(sc, view, r, wb, w, t_w, st, link, counts,
+                                                                  row_start, t_out, irr_out, work);
   else
-    learned_window_kernel<false><<<(P + 63) / 64, 64, 0, stream>>>(sc, view, r, wb, w, t_w, st, counts, row_start,
-                                                                   t_out, irr_out, work);
+    learned_window_kernel<false><<<(P + 63) / 64, 64, 0, stream>>>(sc, view, r, wb, w, t_w, st, link, counts,
+                                                                   row_start, t_out, irr_out, work);
   return hipGetLastError();
 }
 

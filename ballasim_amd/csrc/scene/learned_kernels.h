@@ -26,6 +26,29 @@ struct LearnedRules {
   int32_t signature;
   int64_t min_split;
   int64_t min_dt;
+  // Near a threshold edge (room below `near`) the controller samples as the plain one: intervals of two
+  // lattice steps of the plain level rule (d_px, l_min, l_max), judged by eta_L at their midpoint.
+  float near;
+  int32_t l_min, l_max;
+  double d_px;
+};
+
+// The sensor a learned sampler feeds, read at each window start (null pointers: not coupled, the
+// controller tracks an ideal reference of step theta_ref instead): per pixel its thresholds, scale and
+// reference state (the instantaneous response's Delta = ln(I / i_scale) - s3_ref - n3_ref + dr_ref),
+// a pending release (blind_until) and the reset rule.
+struct SensorLink {
+  const float* theta_on;
+  const float* theta_off;
+  const float* scale;
+  const float* i_scale;
+  const float* s3_ref;
+  const float* n3_ref;
+  const float* dr_ref;
+  const int64_t* blind_until;
+  float eps_ci;
+  int32_t ideal_reset;
+  int64_t refractory_ns;
 };
 
 // Per-pixel controller state (structure of arrays, P pixels): four sample times and log values
@@ -48,8 +71,8 @@ hipError_t launch_examples(const float* trace, const float* ref, uint32_t P, int
 hipError_t launch_learned_init(const SceneDev& sc, const ViewDev& view, int64_t t0, float dark,
                                const LearnedStateDev& s, hipStream_t stream);
 hipError_t launch_learned_window(const SceneDev& sc, const ViewDev& view, const LearnedRules& r, const uint16_t* wb,
-                                 const float* w, int64_t t_w, const LearnedStateDev& s, int32_t* counts,
-                                 const int64_t* row_start, int64_t* t_out, float* irr_out, unsigned long long* work,
-                                 bool write, hipStream_t stream);
+                                 const float* w, int64_t t_w, const LearnedStateDev& s, const SensorLink& link,
+                                 int32_t* counts, const int64_t* row_start, int64_t* t_out, float* irr_out,
+                                 unsigned long long* work, bool write, hipStream_t stream);
 
 }  // namespace bsx

@@ -234,8 +234,12 @@ class LearnedSampler {
       throw std::invalid_argument("learned sampler: need positive tolerances, thresholds and margin, "
                                   "0 <= max_refine_levels <= 7 and min_dt_ns >= 2");
     }
-    r_ = LearnedRules{q.dark, q.refine_tol_log, r.tol_log, r.theta_band, r.theta_ref, r.margin, q.max_refine_levels,
-                      q.refine_signature, static_cast<int64_t>(q.min_split_ns), r.min_dt_ns};
+    if (!(r.near >= 0.0f) || !(q.d_px > 0) || q.l_min < 0 || q.l_max > 13 || q.l_min > q.l_max) {
+      throw std::invalid_argument("learned sampler: need near >= 0, d_px > 0 and 0 <= l_min <= l_max <= 13");
+    }
+    r_ = LearnedRules{q.dark,      q.refine_tol_log, r.tol_log, r.theta_band, r.theta_ref, r.margin,
+                      q.max_refine_levels, q.refine_signature, static_cast<int64_t>(q.min_split_ns),
+                      r.min_dt_ns, r.near, q.l_min, q.l_max, q.d_px};
     P_ = static_cast<size_t>(v.width) * v.height;
     t_.ensure(P_ * 4 * 8);
     L_.ensure(P_ * 4 * 4);
@@ -259,8 +263,22 @@ class LearnedSampler {
   }
 
   bsx_device_runs window(int64_t t_w, int32_t slot, hipStream_t stream, uint64_t* total_out, uint64_t* proposals,
-                         uint64_t* fallbacks) {
+                         uint64_t* fallbacks, const SensorModel* sensor) {
     if (!initialized_) throw std::logic_error("learned sampler: call init first");
+    SensorLink link{};
+    if (sensor != nullptr) {
+      const bsx_sensor_info_t info = sensor->info();
+      if (static_cast<size_t>(info.width) * info.height != P_) {
+        throw std::invalid_argument("learned sampler: the sensor's size differs from the view's");
+      }
+      if (sensor->params().photoreceptor != BSX_PHOTORECEPTOR_INSTANTANEOUS) {
+        throw std::invalid_argument("learned sampler: only an instantaneous-response sensor can be linked");
+      }
+      const CameraDev cd = sensor->current_view();
+      link = SensorLink{cd.mp.theta_on, cd.mp.theta_off, cd.mp.scale,   cd.in.i_scale,       cd.in.s3_ref,
+                        cd.in.n3_ref,   cd.in.dr_ref,    cd.in.blind_until, cd.kc.eps_ci, cd.kc.ideal_reset,
+                        cd.kc.refractory_ns};
+    }
     if ((t_w & (kWindowNs - 1)) != 0) throw std::invalid_argument("learned sampler: t_w must be a multiple of 2^23 ns");
     const ViewDev v = Scene::view_of(view_);
     WindowSlot& w = sc_.slot_at(slot);
@@ -269,7 +287,7 @@ class LearnedSampler {
     sc_.ensure_ids(P_);
     int64_t* scan = sc_.scan_scratch(P_);
     hip_check(hipMemsetAsync(work_.get(), 0, 16, stream), "zero work");
-    hip_check(launch_learned_window(sc_.dev(), v, r_, p_.weights_bf16(), p_.weights(), t_w, state(),
+    hip_check(launch_learned_window(sc_.dev(), v, r_, p_.weights_bf16(), p_.weights(), t_w, state(), link,
                                     w.counts.as<int32_t>(), nullptr, nullptr, nullptr,
                                     work_.as<unsigned long long>(), false, stream),
               "learned count");
@@ -281,7 +299,7 @@ class LearnedSampler {
     hip_check(hipStreamSynchronize(stream), "learned count");
     w.t.ensure(std::max<int64_t>(total, 1) * 8);
     w.irr.ensure(std::max<int64_t>(total, 1) * 4);
-    hip_check(launch_learned_window(sc_.dev(), v, r_, p_.weights_bf16(), p_.weights(), t_w, state(), nullptr,
+    hip_check(launch_learned_window(sc_.dev(), v, r_, p_.weights_bf16(), p_.weights(), t_w, state(), link, nullptr,
                                     w.row_start.as<const int64_t>(), w.t.as<int64_t>(), w.irr.as<float>(), nullptr,
                                     true, stream),
               "learned write");
@@ -415,11 +433,13 @@ int bsx_learned_sampler_init(bsx_learned_sampler* ls, int64_t t0_ns, void* strea
   });
 }
 
-int bsx_learned_sampler_window(bsx_learned_sampler* ls, int64_t t_w, int32_t slot, void* stream, bsx_device_runs* out,
-                               uint64_t* total_samples, uint64_t* proposals, uint64_t* fallbacks) {
+int bsx_learned_sampler_window(bsx_learned_sampler* ls, int64_t t_w, int32_t slot, bsx_sensor* sensor, void* stream,
+                               bsx_device_runs* out, uint64_t* total_samples, uint64_t* proposals, uint64_t* fallbacks) {
   return scene_guarded([&] {
     if (ls == nullptr || ls->ls == nullptr || out == nullptr) throw std::invalid_argument("bsx_learned_sampler_window: null");
-    *out = ls->ls->window(t_w, slot, static_cast<hipStream_t>(stream), total_samples, proposals, fallbacks);
+    if (sensor != nullptr && sensor->model == nullptr) throw std::invalid_argument("bsx_learned_sampler_window: null sensor");
+    *out = ls->ls->window(t_w, slot, static_cast<hipStream_t>(stream), total_samples, proposals, fallbacks,
+                          sensor != nullptr ? sensor->model : nullptr);
   });
 }
 

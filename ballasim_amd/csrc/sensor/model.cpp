@@ -386,6 +386,15 @@ CameraDev SensorModel::stage(const RunsDev& runs, const uint32_t* pixel, size_t 
   return cd;
 }
 
+CameraDev SensorModel::current_view() const {
+  CameraDev cd{};
+  cd.kc = kconst();
+  cd.mp = maps_view(d_maps_, P_);
+  cd.in = state_view(cur_);
+  cd.t_state = t_state_;
+  return cd;
+}
+
 void SensorModel::grow_events(unsigned long long need, CameraDev& cd) {
   cap_ = std::max<unsigned long long>(cap_ * 2, need + need / 4);
   d_events_.ensure(cap_ * sizeof(RawEventDev));

@@ -110,6 +110,9 @@ class SensorModel {
   bsx_sensor_info_t info() const;
   const SensorMaps& maps() const { return maps_; }
   const Readout& readout() const { return readout_; }
+  // The call constants, maps and current state as a descriptor (for a sampler that reads the sensor's
+  // reference state between calls).
+  CameraDev current_view() const;
   uint32_t width() const { return width_; }
   uint32_t height() const { return height_; }
   const SensorParams& params() const { return params_; }

@@ -93,7 +93,7 @@ class ExamplesSpec(C.Structure):
 
 class LearnedRules(C.Structure):
     _fields_ = [("plain", _abi.SamplerRules), ("tol_log", C.c_float), ("theta_band", C.c_float),
-                ("theta_ref", C.c_float), ("margin", C.c_float), ("min_dt_ns", C.c_int64)]
+                ("theta_ref", C.c_float), ("margin", C.c_float), ("near", C.c_float), ("min_dt_ns", C.c_int64)]
 
 
 def learned_examples(scene, view, x, target, dark, t0_ns=0, delta_ns=10_000, n_dense=4096, max_steps=838,
@@ -109,8 +109,8 @@ def learned_examples(scene, view, x, target, dark, t0_ns=0, delta_ns=10_000, n_d
     return n_active.value
 
 
-def learned_rules(plain, tol_log=0.01, theta_band=0.14, theta_ref=0.2, margin=1.0, min_dt_ns=4096):
-    return LearnedRules(plain, tol_log, theta_band, theta_ref, margin, min_dt_ns)
+def learned_rules(plain, tol_log=0.01, theta_band=0.14, theta_ref=0.2, margin=1.0, near=0.0, min_dt_ns=4096):
+    return LearnedRules(plain, tol_log, theta_band, theta_ref, margin, near, min_dt_ns)
 
 
 class LearnedSampler:
@@ -138,12 +138,14 @@ class LearnedSampler:
     def init(self, t0_ns, stream=0):
         _abi.check(self._lib.bsx_learned_sampler_init(self._h, int(t0_ns), _stream(stream)))
 
-    def window(self, t_w, slot=0, stream=0):
-        """(DeviceRuns, samples, proposals, fallbacks) of window (t_w, t_w + 2^23]."""
+    def window(self, t_w, slot=0, stream=0, sensor=None):
+        """(DeviceRuns, samples, proposals, fallbacks) of window (t_w, t_w + 2^23]; sensor: the SensorModel
+        these runs feed, whose reference state the controller reads at the window start (or None)."""
         runs = _abi.DeviceRuns()
         total, prop, fb = C.c_uint64(), C.c_uint64(), C.c_uint64()
-        _abi.check(self._lib.bsx_learned_sampler_window(self._h, int(t_w), int(slot), _stream(stream), C.byref(runs),
-                                                        C.byref(total), C.byref(prop), C.byref(fb)))
+        sh = sensor._h if sensor is not None else None
+        _abi.check(self._lib.bsx_learned_sampler_window(self._h, int(t_w), int(slot), sh, _stream(stream),
+                                                        C.byref(runs), C.byref(total), C.byref(prop), C.byref(fb)))
         return runs, total.value, prop.value, fb.value
 
 

@@ -264,7 +264,7 @@ def bench_learned(args):
 
     p = rig_params(bsx, "instantaneous")
     dark = float(p.dark_current_fA) / float(p.illuminance_scale)
-    band, tol, margin = 0.02, 0.005, 0.5
+    band, tol, margin, near = 0.2, 0.01, 0.5, 0.05
     n_train, n_scenes = args.train_examples, 20
     n_per = n_train // n_scenes
     X = torch.empty((n_per * n_scenes, 12), dtype=torch.float32, device="cuda")
@@ -307,22 +307,23 @@ def bench_learned(args):
     frame = sc.render(v, 0)
     n_win = max(1, int(round(args.eval_ms * 1e6 / WINDOW)))
 
-    def run(kind):
+    def run(kind, d_px=0.25):
         m = SensorModel(p, v.width, v.height, 1, ["constant"], 0, device="cuda")
         m.init(0, frame.reshape(-1, 1))
         ls = None
         if kind == "learned":
             ls = LearnedSampler(sc, v, learned_rules(S.rules(dark, tol=tol), tol_log=tol, theta_band=band,
-                                                     margin=margin), prop)
+                                                     margin=margin, near=near), prop)
             ls.init(0)
         evs, samples, fallbacks, proposals = [], 0, 0, 0
         for i in range(n_win):
             if kind == "dense":
                 r, tot = sc.sample_window(v, S.rules(dark, fixed_period_ns=10_000), i * WINDOW, slot=0)
             elif kind == "plain":
-                r, tot = sc.sample_window(v, S.rules(dark, tol=0.01), i * WINDOW, slot=0)
+                r, tot = sc.sample_window(v, S.rules(dark, tol=0.01, d_px=d_px), i * WINDOW, slot=0)
             else:
-                r, tot, pr, fb = ls.window(i * WINDOW, slot=0)
+                # the controller reads the sensor's reference state at each window start
+                r, tot, pr, fb = ls.window(i * WINDOW, slot=0, sensor=m)
                 fallbacks += fb
                 proposals += pr
             samples += tot
@@ -333,13 +334,24 @@ def bench_learned(args):
 
     from ballasim_amd.sensor import SensorModel
     ref, ref_samples, _, _ = run("dense")
-    res = {}
-    for kind in ("plain", "learned"):
-        ev, samples, proposals, fallbacks = run(kind)
+
+    def summary(ev, samples, proposals=0, fallbacks=0):
         err = event_error(ev, ref, v.width)
-        res[kind] = dict(samples=samples, events=err["n_test"], samples_per_event=samples / max(1, err["n_test"]),
-                         count_mismatch=err["count_mismatch"], median_abs_dt_ns=err["median_abs_dt_ns"],
-                         p90_abs_dt_ns=err["p90_abs_dt_ns"], proposals=proposals, fallbacks=fallbacks)
+        return dict(samples=samples, events=err["n_test"], samples_per_event=samples / max(1, err["n_test"]),
+                    count_mismatch=err["count_mismatch"], median_abs_dt_ns=err["median_abs_dt_ns"],
+                    p90_abs_dt_ns=err["p90_abs_dt_ns"], proposals=proposals, fallbacks=fallbacks)
+
+    res = {"plain": summary(*run("plain")), "learned": summary(*run("learned"))}
+    # The plain controller's trade of samples against error (its lattice density d_px), to compare at equal error:
+    # the plain samples per event that reach the learned controller's median timing error (log-log interpolation).
+    curve = [dict(d_px=d, **summary(*run("plain", d))) for d in (0.125, 0.25, 0.5, 1.0)]
+    target = res["learned"]["median_abs_dt_ns"]
+    pts = sorted((c["median_abs_dt_ns"], c["samples_per_event"]) for c in curve)
+    equal_err_spe = None
+    for (e0_, s0_), (e1_, s1_) in zip(pts, pts[1:]):
+        if e0_ <= target <= e1_ and e1_ > e0_:
+            f = (np.log(target) - np.log(e0_)) / (np.log(e1_) - np.log(e0_))
+            equal_err_spe = float(np.exp(np.log(s0_) + f * (np.log(s1_) - np.log(s0_))))
     line = {
         "metric": "pixel-samples/s + events/s on 1 MI355X; irradiance samples/event at fixed error",
         "value": res["learned"]["samples_per_event"],
@@ -353,10 +365,14 @@ def bench_learned(args):
                                "100 lux) over %d windows (%.0f ms) against a 10-us dense reference; thresholds "
                                "0.2 +- 0.02, refractory 100 us, instantaneous response, noise off"
                                % (n_per * n_scenes, n_win, n_win * WINDOW / 1e6),
-                   "band": band, "tol_log": tol, "margin": margin},
-        "plain": res["plain"], "learned": res["learned"],
+                   "band": band, "tol_log": tol, "margin": margin, "near": near,
+                   "sensor_linked": True},
+        "plain": res["plain"], "learned": res["learned"], "plain_density_curve": curve,
         "dense_reference": {"samples": ref_samples, "events": int(len(ref["t_cross_ns"]))},
         "samples_per_event_gain_vs_plain": res["plain"]["samples_per_event"] / res["learned"]["samples_per_event"],
+        "plain_samples_per_event_at_learned_median_error": equal_err_spe,
+        "gain_at_equal_median_error": (equal_err_spe / res["learned"]["samples_per_event"]
+                                       if equal_err_spe is not None else None),
         "training": {"examples": int(n_per * n_scenes), "generation_s": gen_s, "rows_trained": int(rows),
                      "train_ms": train_ms, "rows_per_s": rows / (train_ms * 1e-3), "final_loss": loss},
         "roofline": {"bound": "mfma", "achieved": train_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",

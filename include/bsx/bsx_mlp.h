@@ -84,6 +84,7 @@ typedef struct bsx_learned_rules {
   float tol_log;           /* the a-posteriori check's log tolerance */
   float theta_band, theta_ref;
   float margin;            /* factor on the proposed interval */
+  float near;              /* band room below which the controller samples as the plain one (0: never) */
   int64_t min_dt_ns;       /* lower bound of a proposal */
 } bsx_learned_rules;
 
@@ -103,9 +104,15 @@ int bsx_learned_sampler_destroy(bsx_learned_sampler* ls);
 /* The controller state at t0 (each pixel's sample at t0, the reference there). */
 int bsx_learned_sampler_init(bsx_learned_sampler* ls, int64_t t0_ns, void* stream);
 /* Window (t_w, t_w + 2^23] of every pixel: runs in pixel order in the scene's buffer set `slot` (as
- * bsx_scene_sample_window), the sample count, the network's proposals and the fallbacks taken. */
-int bsx_learned_sampler_window(bsx_learned_sampler* ls, int64_t t_w, int32_t slot, void* stream, bsx_device_runs* out,
-                               uint64_t* total_samples, uint64_t* proposals, uint64_t* fallbacks);
+ * bsx_scene_sample_window), the sample count, the network's proposals and the fallbacks taken.
+ * sensor: the instantaneous-response model these runs feed (same pixels), or null. When given, the
+ * controller reads the sensor's state at the window start: each pixel's band is the sensor's own
+ * reference and thresholds, a pending release leaves it unknown (near mode) until it passes, and
+ * crossings within the window move it as the sensor's reset rule does (a crossing time estimated
+ * between samples). Near a band edge (room below `near`) the interval is two of the plain
+ * controller's lattice steps for the pixel, judged by eta_L and signature flips at the midpoint. */
+int bsx_learned_sampler_window(bsx_learned_sampler* ls, int64_t t_w, int32_t slot, bsx_sensor* sensor, void* stream,
+                               bsx_device_runs* out, uint64_t* total_samples, uint64_t* proposals, uint64_t* fallbacks);
 
 #ifdef __cplusplus
 }

@@ -2,7 +2,8 @@
 config-1-style scenes, training by the library's kernels, and the learned sampler on a held-out seed
 against the plain error-tolerance controller and against the same sampler with the untrained network.
 The judge of event error is a dense-sampled reference (10 us): count mismatch and median timing error
-(ballasim_amd.learned.event_error).
+(ballasim_amd.learned.event_error). The learned sampler reads the sensor's reference state at each window
+start (its band of no event start) and samples as the plain controller near the band edges.
 
 What the test pins: the examples are well formed (finite features, targets within the step range);
 training lowers the loss; every learned run is a valid sample stream the sensor accepts; the learned
@@ -40,7 +41,7 @@ def test_learned_controller_gain():
     from ballasim_amd.learned import LearnedSampler, Proposer, event_error, learned_examples, learned_rules
     p = _sensor_params(bsx)
     dark = float(p.dark_current_fA) / float(p.illuminance_scale)
-    band, tol, margin = 0.02, 0.005, 0.5
+    band, tol, margin, near = 0.2, 0.01, 0.5, 0.05
     n_per, seeds = 250_000, [101, 102]
     X = torch.empty((n_per * len(seeds), 12), dtype=torch.float32, device="cuda:0")
     T = torch.empty(n_per * len(seeds), dtype=torch.float32, device="cuda:0")
@@ -67,7 +68,7 @@ def test_learned_controller_gain():
 
     sc, v = S.config1_scene(7)
     frame = sc.render(v, 0)
-    n_win = 6
+    n_win = 12
 
     def run(kind, prop=None):
         m = bsx.SensorModel(p, v.width, v.height, 1, ["constant"], 0, device="cuda")
@@ -75,7 +76,7 @@ def test_learned_controller_gain():
         ls = None
         if prop is not None:
             ls = LearnedSampler(sc, v, learned_rules(S.rules(dark, tol=tol), tol_log=tol, theta_band=band,
-                                                     margin=margin), prop)
+                                                     margin=margin, near=near), prop)
             ls.init(0)
         evs, samples, fallbacks = [], 0, 0
         for i in range(n_win):
@@ -84,8 +85,16 @@ def test_learned_controller_gain():
             elif kind == "plain":
                 r, tot = sc.sample_window(v, S.rules(dark, tol=0.01), i * W, slot=0)
             else:
-                r, tot, _, fb = ls.window(i * W, slot=0)
+                r, tot, _, fb = ls.window(i * W, slot=0, sensor=m)
                 fallbacks += fb
+                # every sample inside the window, strictly increasing per pixel (the device path does not
+                # check sample times against the model time)
+                _, ts, _, offs = S.runs_to_host(r)
+                assert len(ts) == tot and (tot == 0 or (ts.min() > i * W and ts.max() <= (i + 1) * W))
+                d = np.diff(ts)
+                starts = np.zeros(len(ts), bool)
+                starts[offs[:-1][offs[:-1] < len(ts)]] = True
+                assert np.all(d[~starts[1:]] > 0)
             samples += tot
             ptr, cnt = m.advance_device_runs(r, (i + 1) * W)
             evs.append(m.download_events(ptr, cnt))
@@ -104,4 +113,4 @@ def test_learned_controller_gain():
     assert out["learned"]["samples_per_event"] < out["plain"]["samples_per_event"]
     assert out["learned"]["samples_per_event"] < out["untrained"]["samples_per_event"] or \
         out["learned"]["fallbacks"] < out["untrained"]["fallbacks"]
-    assert out["learned"]["count_mismatch"] < 0.02
+    assert out["learned"]["count_mismatch"] < 0.01
