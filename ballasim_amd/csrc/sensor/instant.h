@@ -139,11 +139,17 @@ struct InstantLane {
   //     one level can be crossed after lo, and "crossed at t" is monotone in t: the transition is
   //     bracketed by interpolation and bisection;
   //   - falling I with leak: Delta(t) lies between fl(S(t) + dl(lo)) and fl(S(t) + dl(c)), both
-  //     non-increasing (S = the leak-free part), so the first crossing of the lower level lies
-  //     between their transitions, which are found as above and scanned in between; a segment where
-  //     the upper envelope allows an ON crossing, or the gap exceeds kScanLimit, takes the bisection.
-  // Returns false when the bisection must decide (mixed group slopes, n3_ref != 0).
-  static constexpr int64_t kScanLimit = 64;
+  //     non-increasing (S = the leak-free part), so the first crossing of the lower level lies in
+  //     [tL, tU] between their transitions, which are found as above. Inside, Delta = fl(S + dl)
+  //     with S non-increasing and dl(t) non-decreasing, so fl(S(c') + dl(a')) bounds Delta from
+  //     below on a node [a', c'], and where dl(a') = dl(c') Delta is non-increasing on the node: a
+  //     time-ordered bisection of [tL, tU] drops the nodes whose bound stays above the level and
+  //     brackets the crossing in the first node of constant dl that reaches it. (The generic
+  //     bisection's bounds of I cannot drop nodes where Delta runs level with -theta_off, where the
+  //     leak balances a slow fall, and visits most of their integers.) A segment where the upper
+  //     envelope allows an ON crossing takes the generic bisection.
+  // Returns false when the generic bisection must decide (mixed group slopes, n3_ref != 0, the ON
+  // level within reach of a falling segment under leak).
 
   BSX_HD BSX_INLINE float s_at(int64_t t, float& x) const {
     x = log_ratio(photocurrent(t));
@@ -180,7 +186,7 @@ struct InstantLane {
     return b;
   }
 
-  BSX_HD BSX_INLINE bool transition_search(int64_t lo, int64_t hi, Hit& out) {
+  BSX_HD BSX_INLINE bool transition_search(int64_t lo, int64_t hi, Hit& out, uint32_t* stack, int stride) {
     if (n3_ref != 0.0f || !(leak >= 0.0f)) return false;
     bool up = true, down = true;
     if (tb != kNoBreakpoint) {
@@ -245,19 +251,46 @@ struct InstantLane {
         return v <= -theta_off;
       });
     }
+    // Nodes [tL + a, tL + c'] (offsets: a piece spans less than 2^23 ns, kSearchDepth levels); the
+    // pending right siblings are kept as their right ends, a popped node starting after the last one.
     const int64_t last = vmin(tU, c);
-    if (last - tL > kScanLimit) return false;
-    for (int64_t t = tL; t <= last; ++t) {
-      float xt;
-      const float D = s_at(t, xt) + dl_at(t);
+    uint32_t a = 0, e = static_cast<uint32_t>(last - tL);
+    int sp = 0;
+    while (true) {
+      float x_e;
+      const float S_e = s_at(tL + e, x_e);
+      const float dl_a = dl_at(tL + a), dl_e = dl_at(tL + e);
       ++n_leaves;
-      if (D <= -theta_off) {
-        out.found = true;
-        out.on = false;
-        out.t = t;
-        out.x = xt;
-        return true;
+      if (S_e + dl_a <= -theta_off) {
+        if (dl_a == dl_e) {  // Delta non-increasing on the node, and Delta(e) = S_e + dl_e fires
+          int64_t t = tL + a;
+          if (a != e) {
+            float x_a;
+            const float D_a = s_at(t, x_a) + dl_a;
+            ++n_leaves;
+            if (!(D_a <= -theta_off))
+              t = first_true(t, D_a, tL + e, S_e + dl_e, -theta_off, [&](int64_t m, float& v) {
+                float xm;
+                v = s_at(m, xm) + dl_at(m);
+                return v <= -theta_off;
+              });
+          }
+          out.found = true;
+          out.on = false;
+          out.t = t;
+          out.x = log_ratio(photocurrent(t));
+          return true;
+        }
+        const uint32_t m = a + (e - a) / 2;  // e > a: dl differs at the ends
+        stack[sp * stride] = e;
+        ++sp;
+        e = m;
+        continue;
       }
+      if (sp == 0) break;
+      --sp;
+      a = e + 1;
+      e = stack[sp * stride];
     }
     return true;  // tU = c + 1: no crossing (a finite tU always holds one)
   }
@@ -582,7 +615,7 @@ struct InstantLane {
   template <class Sink>
   BSX_HD BSX_INLINE void search_step(Sink& sink, uint32_t* stack, int stride) {
     Hit hit{false, false, s_lo, 0.0f};
-    if (!transition_search(s_lo, e_cur, hit)) hit = search(s_lo, e_cur, stack, stride);
+    if (!transition_search(s_lo, e_cur, hit, stack, stride)) hit = search(s_lo, e_cur, stack, stride);
     if (!hit.found) {
       t_search = vmax(t_search, e_cur);
       t_cur = e_cur;
