@@ -39,6 +39,10 @@ struct InstantLane {
   const BSX_GLOBAL int64_t* ts;
   const BSX_GLOBAL float* irr;
   const BSX_GLOBAL InstantPrep* prep;
+  // Phase A's block summaries of the camera's segment space (null: none) and the run's first
+  // position in it.
+  const BSX_GLOBAL InstantBlock* blocks = nullptr;
+  int64_t pos0 = 0;
   int n, skip;
   int64_t t_state;
   // State.
@@ -72,6 +76,16 @@ struct InstantLane {
     const float f = static_cast<float>(t - ta) / static_cast<float>(tb - ta);
     for (int g = 0; g < kMaxG; ++g)
       if (g < G) I = I + scale * (va[g] + (vb[g] - va[g]) * f);
+    return I;
+  }
+
+  // photocurrent(ta) of a segment with an end sample, without its interpolation: there f = +0, so
+  // (vb - va) f is a zero and va + that zero is va (a zero's sign can differ, which the sum that
+  // starts at dark then absorbs: the same bits).
+  BSX_HD BSX_INLINE float photocurrent_start() const {
+    float I = dark;
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < G) I = I + scale * va[g];
     return I;
   }
 
@@ -395,47 +409,48 @@ struct InstantLane {
 
   using Prep = InstantPrep;
 
-  // The segment's values as segment_prep forms them (also the check of the fast path on the host).
+  // The segment's values as phase A forms them (also the check of the fast path on the host).
   BSX_HD BSX_INLINE Prep prep_segment() const {
-    constexpr float eps = FloatEps<float>::value;
     Prep q;
-    const float Ia = photocurrent(ta), Ic = photocurrent(tb - 1);
-    q.x_a = log_ratio(Ia);
-    q.x_c = log_ratio(Ic);
-    q.x_hi = nan_<float>();
-    q.x_lo = nan_<float>();
-    if (leak != 0.0f && !(Ic >= Ia)) {
-      const float hi = vmax(Ia, Ic), lo = vmin(Ia, Ic);
-      const float margin = 64.0f * eps * (hi + 2.0f * 0.0f);
-      q.x_hi = log_ratio(vmin(hi + 0.125f * 0.0f, inf_<float>()) + margin);
-      const float lo_b = vmax(lo - 0.125f * 0.0f, dark) - margin;
-      if (lo_b > 0.0f) q.x_lo = log_ratio(lo_b);
-    }
+    q.x_a = log_ratio(photocurrent_start());
+    q.x_c = log_ratio(photocurrent(tb - 1));
     return q;
   }
 
   // The whole-segment search from t_a on the prepared values: 0 no crossing in [t_a, t_b), 1 a
   // crossing at t_a itself (the hit is filled), 2 undecided (the exact search must run).
+  //   - Without leak, or with I(t_b - 1) > I(t_a) (known when x_c > x_a, or when no group falls),
+  //     this is the root test of instant_may_fire: Delta at t_b - 1 against both levels, or against
+  //     theta_on.
+  //   - A falling segment under leak (no group rises): S(t) = ln(I(t) / i_scale) - s3_ref - n3_ref
+  //     does not increase and dl(t) does not decrease on [t_a, t_b - 1] (the monotone steps of the
+  //     transition searches), so fl(S(c) + dl(t_a)) <= Delta(t) <= fl(S(t_a) + dl(c)) there, c =
+  //     t_b - 1. When both bounds stay inside the levels, no integer of the segment fires, and the
+  //     reference's bisection (which reports only integers that fire) finds none either. These
+  //     bounds need no logarithms beyond x_a and x_c, and are tighter than instant_may_fire's.
+  //   - Groups moving both ways with x_c <= x_a: undecided.
   BSX_HD BSX_INLINE int fast_search(const Prep& q, Hit& h) const {
-    constexpr float eps = FloatEps<float>::value;
     h.x = q.x_a;
     if (fires(ta, h)) return 1;
     if (tb - ta == 1) return 0;
     if (tb - ta == 2) return 2;  // the root is a leaf: the exact search tests it
     const int64_t c = tb - 1;
-    if (leak == 0.0f) {
-      const float D = delta_at(c, q.x_c);
-      return D >= theta_on || D <= -theta_off ? 2 : 0;
+    const float D_c = delta_at(c, q.x_c);
+    if (leak == 0.0f) return D_c >= theta_on || D_c <= -theta_off ? 2 : 0;
+    if (!(leak > 0.0f)) return 2;
+    bool up = true, down = true;
+    for (int g = 0; g < kMaxG; ++g) {
+      if (g < G) {
+        if (vb[g] < va[g]) up = false;
+        if (vb[g] > va[g]) down = false;
+      }
     }
-    if (q.x_hi != q.x_hi) return delta_at(c, q.x_c) >= theta_on ? 2 : 0;  // rising
-    const float dl_a = leak * ns_to_s64<float>(ta - t_ref) + dr_ref;
-    const float dl_c = leak * ns_to_s64<float>(c - t_ref) + dr_ref;
-    const float base = 0.0f - n3_ref - s3_ref;
-    const float x_hi = q.x_hi;
-    if (x_hi + base + dl_c + 16.0f * eps * (1.0f + fabs_(x_hi) + fabs_(s3_ref) + fabs_(dl_c)) >= theta_on) return 2;
-    if (q.x_lo != q.x_lo) return 2;
-    const float x_lo = q.x_lo;
-    return x_lo + base + dl_a - 16.0f * eps * (1.0f + fabs_(x_lo) + fabs_(s3_ref) + fabs_(dl_a)) <= -theta_off ? 2 : 0;
+    if (up || q.x_c > q.x_a) return D_c >= theta_on ? 2 : 0;
+    if (!down) return 2;
+    const float S_a = (q.x_a - s3_ref) + (0.0f - n3_ref);
+    const float S_c = (q.x_c - s3_ref) + (0.0f - n3_ref);
+    if (S_a + dl_at(c) >= theta_on) return 2;
+    return S_c + dl_at(ta) <= -theta_off ? 2 : 0;
   }
 
   // Does the walk, reaching the start of the current segment [ta, tb) in the state the lane holds
@@ -530,6 +545,7 @@ struct InstantLane {
   static constexpr int kWalk = 0, kSearch = 1, kDone = 2;
 
   int next_i;          // the next input sample to become a segment end
+  InstantBlock blk;    // the summary of the block whose first segment ends at next_i (x_min NaN: none)
   int64_t nt;          // the prefetched segment end (kNoBreakpoint: none)
   float nv[kMaxG];
   Prep q_next, q_cur;  // the prefetched and the current segment's prepared values
@@ -557,6 +573,7 @@ struct InstantLane {
     }
     nt = kNoBreakpoint;
     if (next_i < n) load_sample(next_i, nt, nv);
+    load_block();
     t_cur = ta;
     q_next = Prep{};
     if (prep != nullptr && next_i < n) q_next = prep[next_i];
@@ -565,11 +582,62 @@ struct InstantLane {
     tb = kNoBreakpoint;
   }
 
+  // ---- Quiet blocks ------------------------------------------------------------------------------
+  // At the start of a segment whose position in the camera's segment space is a multiple of
+  // kBlockSegments, the walk may pass the whole block of kBlockSegments segments [ta, t_last) at
+  // once. Each segment's groups move one way, so its x(t) = ln(I(t) / i_scale) lies between its x_a
+  // and x_c, hence in [x_min, x_max], and dl(t) lies in [dl(ta), dl(t_last - 1)] (leak >= 0): every
+  // Delta(t) = fl(S(x(t)) + dl(t)) of the block lies in [fl(S(x_min) + dl(ta)),
+  // fl(S(x_max) + dl(t_last - 1))] (S(x) = (x - s3_ref) + (0 - n3_ref), the monotone steps of the
+  // transition searches). When that range is inside the levels, no blindness is pending, no shot
+  // arrival falls before t_last and no 2^23-ns cut inside, the segment-by-segment walk would fire no
+  // event in the block and leave exactly this: t_search = max(t_search, t_last), one sub-step and one
+  // piece per segment, and the block's last sample as the held segment start (phase A's tests are not
+  // counted as leaf intervals).
+  BSX_HD BSX_INLINE void load_block() {
+    blk.x_min = nan_<float>();
+    if (blocks == nullptr || next_i + (kBlockSegments - 1) >= n) return;
+    const int64_t p = pos0 + next_i;
+    if ((p & (kBlockSegments - 1)) == 0) blk = blocks[p / kBlockSegments];
+  }
+
+  BSX_HD BSX_INLINE bool block_quiet() const {
+    if (!(blk.x_min == blk.x_min)) return false;
+    const int64_t tl = blk.t_last;
+    const bool free = (blind_until == kNotBlind) & (shot_next >= tl) & (tl <= t_end_) &
+                      (((ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= tl) & (leak >= 0.0f);
+    if (!free) return false;
+    const float S_hi = (blk.x_max - s3_ref) + (0.0f - n3_ref);
+    const float S_lo = (blk.x_min - s3_ref) + (0.0f - n3_ref);
+    return (S_hi + dl_at(tl - 1) < theta_on) & (S_lo + dl_at(ta) > -theta_off);
+  }
+
+  BSX_HD BSX_INLINE void skip_block() {
+    const int last = next_i + (kBlockSegments - 1);
+    n_substeps += kBlockSegments;
+    n_pieces += kBlockSegments;
+    t_search = vmax(t_search, blk.t_last);
+    load_sample(last, ta, va);
+    t_cur = ta;
+    next_i = last + 1;
+    nt = kNoBreakpoint;
+    if (next_i < n) {
+      load_sample(next_i, nt, nv);
+      if (prep != nullptr) q_next = prep[next_i];
+    }
+    load_block();
+  }
+
   template <class Sink>
   BSX_HD BSX_INLINE int walk(Sink& sink, int budget) {
     while (true) {
       if (!seg_open) {
         if (!(t_cur < t_end_)) return kDone;
+        if (block_quiet()) {
+          skip_block();
+          if (--budget <= 0) return kWalk;
+          continue;
+        }
         tb = nt;
         for (int g = 0; g < kMaxG; ++g)
           if (g < G) vb[g] = nv[g];
@@ -582,6 +650,7 @@ struct InstantLane {
         } else {
           nt = kNoBreakpoint;
         }
+        load_block();
         stop = vmin(t_end_, tb);
         seg_open = true;
       }

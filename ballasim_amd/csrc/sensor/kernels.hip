@@ -115,6 +115,29 @@ __device__ __forceinline__ void wave_segmented_count(uint32_t* dst, int key, uin
   if (key >= 0 && (lane == 63 || kn != key) && v != 0) atomicAdd(dst + key, v);
 }
 
+// Phase A's block summaries: the 16 lanes of positions [16 b, 16 b + 16) (16-aligned: a block's
+// positions start at a multiple of 256) reduce their x ranges; the block is valid when all 16 join
+// and share one run. Every block with a position below total is written.
+__device__ __forceinline__ void block_summary(InstantBlock* blocks, int64_t q, int64_t total, int run, bool joins,
+                                              float lo, float hi, int64_t t_b) {
+  const int lane = __lane_id();
+  const int first = __shfl(run, lane & ~(kBlockSegments - 1));
+  int ok = joins && run >= 0 && run == first ? 1 : 0;
+  for (int off = 1; off < kBlockSegments; off <<= 1) {
+    ok &= __shfl_xor(ok, off);
+    lo = vmin(lo, __shfl_xor(lo, off));
+    hi = vmax(hi, __shfl_xor(hi, off));
+  }
+  const int64_t t_last = __shfl(t_b, lane | (kBlockSegments - 1));
+  if ((lane & (kBlockSegments - 1)) == 0 && q < total) {
+    InstantBlock b;
+    b.x_min = ok ? lo : nan_<float>();
+    b.x_max = ok ? hi : nan_<float>();
+    b.t_last = t_last;
+    blocks[q / kBlockSegments] = b;
+  }
+}
+
 // The run i of position q (the last i with pos[i] <= q; pos non-decreasing), by bisection.
 __device__ __forceinline__ int64_t run_of_position(const int64_t* pos, int64_t n_runs, int64_t q) {
   int64_t lo = 0, hi = n_runs;
@@ -165,6 +188,7 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
   const uint32_t* const pixel = cd.pixel;
   const int64_t* const pos = cd.pos;
   InstantPrep* const prep = cd.prep;
+  InstantBlock* const blocks = cd.blocks;
   float* const work = cd.work;
   int* const first_loud = cd.first_loud;
   const int64_t t_state = cd.t_state, total = cd.total, n_runs = cd.n_runs;
@@ -173,6 +197,10 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
   int run = -1;
   float cost = 0.0f;
   int loud = INT32_MAX;
+  // This lane's part of its block summary: the segment's x range, its end time, whether it may join.
+  bool joins = false;
+  float b_lo = 0.0f, b_hi = 0.0f;
+  int64_t t_b = 0;
   if (q < total) {
     const int64_t i = i_q, j = q - pos[i];
     run = static_cast<int>(i);
@@ -199,7 +227,7 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
     bool empty = false;
     if (j == 0) {
       if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
-        prep[q] = InstantPrep{0.0f, 0.0f, 0.0f, 0.0f};
+        prep[q] = InstantPrep{0.0f, 0.0f};
         empty = true;
       } else {
         L.ta = t_state;
@@ -219,12 +247,25 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
       prep[q] = r;
       const float theta = vmin(L.theta_on, L.theta_off);
       const bool quiet = L.quiet_segment(r);
+      bool up = true, down = true;
+      for (int g = 0; g < kMaxG; ++g) {
+        if (g < kc.groups) {
+          if (L.vb[g] < L.va[g]) up = false;
+          if (L.vb[g] > L.va[g]) down = false;
+        }
+      }
+      joins = up || down;
+      b_lo = vmin(r.x_a, r.x_c);
+      b_hi = vmax(r.x_a, r.x_c);
+      t_b = L.tb;
       loud = quiet ? INT32_MAX : static_cast<int>(j);
-      cost = quiet ? 0.0f : 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) / theta;
+      // (a schedule only: the hardware reciprocal instead of a correctly rounded division)
+      cost = quiet ? 0.0f : 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) * __builtin_amdgcn_rcpf(theta);
     }
   }
   if (work != nullptr) wave_segmented_add(work, run, cost);
   if (first_loud != nullptr) wave_segmented_min(first_loud, run, loud);
+  if (blocks != nullptr) block_summary(blocks, q, total, run, joins, b_lo, b_hi, t_b);
 }
 
 // Before phase A: zero each camera's counters, the order histogram and the queue counter (block 0),
@@ -394,13 +435,19 @@ __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams,
     L.ideal_reset = kc.ideal_reset != 0;
     const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
     L.prep = nullptr;
+    L.blocks = nullptr;
+    L.pos0 = 0;
     int loud_start = 0;
     if (row >= 0) {
       const int64_t start = runs.row_start[row];
       L.ts = runs.t + start;
       L.irr = runs.irr + start * kc.groups;
       L.n = runs.row_count[row];
-      if (cam->prep != nullptr) L.prep = cam->prep + cam->pos[row];
+      if (cam->prep != nullptr) {
+        L.pos0 = cam->pos[row];
+        L.prep = cam->prep + L.pos0;
+        L.blocks = cam->blocks;
+      }
       if (cam->first_loud != nullptr) loud_start = vmin(cam->first_loud[row], L.n);
     } else {
       L.ts = nullptr;
@@ -491,8 +538,10 @@ __device__ __forceinline__ void walk_groups(uint32_t n_total, unsigned int* queu
   }
 }
 
+// Three waves per SIMD: the register allocation is held to 168 (a few bytes of scratch per lane
+// beyond it), which measured faster than 173 registers at two waves.
 template <int kMaxG>
-__global__ __launch_bounds__(kInstantBlock) void instant_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
+__global__ __launch_bounds__(kInstantBlock) __attribute__((amdgpu_waves_per_eu(3))) void instant_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
                                                                    const uint32_t* order, unsigned int* queue,
                                                                    WalkTuning tu, const CallDev* call) {
   // The lanes' bisection stacks, one column per lane (conflict-free: a level is a row of the block;

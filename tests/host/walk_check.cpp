@@ -1,0 +1,169 @@
+// The instantaneous lane program on the host, on random runs: the walk with phase A's per-segment
+// values and block summaries (formed here by the same rules as chain_prep_kernel: the records from
+// prep_segment, the blocks' x ranges over 16 aligned positions of one run whose segments move one
+// way) must give exactly the events and final state of the walk with neither (exact searches only,
+// segment by segment). Prints the counts; exit status 1 on any difference.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "instant.h"
+
+using bsx::InstantBlock;
+using bsx::InstantLane;
+using bsx::InstantPrep;
+using bsx::RawEventDev;
+
+struct VecSink {
+  std::vector<RawEventDev> ev;
+  void push(const RawEventDev& e) { ev.push_back(e); }
+};
+
+static bool same_event(const RawEventDev& a, const RawEventDev& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
+int main(int argc, char** argv) {
+  const int trials = argc > 1 ? std::atoi(argv[1]) : 2000;
+  const unsigned long long seed = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 7;
+  constexpr int kG = 2;
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  std::normal_distribution<double> N01(0.0, 1.0);
+  std::vector<uint32_t> stack(InstantLane<kG>::kSearchDepth);
+  long n_events = 0, n_blocks_skipped = 0, bad = 0;
+  for (int it = 0; it < trials; ++it) {
+    const int G = 1 + it % 2;
+    const int n = 20 + static_cast<int>(U(rng) * 300);
+    const int64_t t_state = static_cast<int64_t>(U(rng) * 1e9);
+    // samples: a log random walk with rare jumps; some runs quiet, some moving
+    std::vector<int64_t> ts(n);
+    std::vector<float> irr(static_cast<size_t>(n) * G);
+    const double vol = it % 3 == 0 ? 0.002 : it % 3 == 1 ? 0.02 : 0.1;
+    double lv[kG] = {std::log(0.5 + U(rng)), std::log(0.2 + U(rng))};
+    int64_t t = t_state + (it % 7 == 0 ? 0 : 1 + static_cast<int64_t>(U(rng) * 5000));
+    for (int i = 0; i < n; ++i) {
+      ts[i] = t;
+      for (int g = 0; g < G; ++g) {
+        lv[g] += vol * N01(rng) + (U(rng) < 0.02 ? 0.5 * N01(rng) : 0.0);
+        irr[static_cast<size_t>(i) * G + g] = static_cast<float>(std::exp(lv[g]));
+      }
+      t += 1 + static_cast<int64_t>(U(rng) * (it % 4 == 0 ? 2e6 : 3e4));
+    }
+    const int64_t t_end = ts[n - 1] + static_cast<int64_t>(U(rng) * 1e5);
+    InstantLane<kG> base{};
+    base.pixel = static_cast<uint32_t>(it);
+    base.G = G;
+    base.theta_on = static_cast<float>(0.15 + 0.05 * (U(rng) - 0.5));
+    base.theta_off = static_cast<float>(0.15 + 0.05 * (U(rng) - 0.5));
+    base.leak = it % 5 == 0 ? 0.0f : static_cast<float>(base.theta_on * (0.05 + 5.0 * U(rng)));
+    base.dark = static_cast<float>(1e-3 * U(rng) + 1e-6);
+    base.scale = 1.0f;
+    base.irr_max = 1e30f;
+    base.eps_ci = 1e-4f;
+    base.label_cut = 0.5f;
+    base.refractory_ns = it % 2 == 0 ? 50000 : 0;
+    base.ideal_reset = it % 6 == 5;
+    base.ts = ts.data();
+    base.irr = irr.data();
+    base.n = n;
+    base.t_state = t_state;
+    base.skip = ts[0] == t_state ? 1 : 0;
+    for (int g = 0; g < G; ++g) base.held[g] = static_cast<float>(0.5 + U(rng));
+    base.i_scale = static_cast<float>(0.3 + U(rng));
+    base.s3_ref = static_cast<float>(std::log(base.held[0] / base.i_scale) + 0.05 * N01(rng));
+    base.n3_ref = 0.0f;
+    base.dr_ref = 0.0f;
+    base.t_ref = t_state;
+    base.blind_until = bsx::kNotBlind;
+    base.t_search = t_state + 1;
+    base.flags = 0;
+    base.shot_rate = it % 4 == 3 ? 2e3 : 0.0;
+    base.seed = 99;
+    base.camera = 0;
+    base.shot_next = base.shot_rate > 0.0 ? t_state + InstantLane<kG>::shot_gap(base.shot_rate, 99, 0, base.pixel, 0, nullptr)
+                                          : INT64_MAX;
+    base.shot_index = 0;
+    base.n_substeps = base.n_pieces = base.n_leaves = 0;
+    // phase A: records and block summaries, the run placed at position pos0 of a segment space
+    const int64_t pos0 = static_cast<int64_t>(U(rng) * 64);
+    std::vector<InstantPrep> prep(n);
+    std::vector<float> lo(n), hi(n);
+    std::vector<int> joins(n);
+    for (int j = 0; j < n; ++j) {
+      InstantLane<kG> L = base;
+      const bool empty = j == 0 && ts[0] == t_state;
+      if (empty) {
+        prep[j] = InstantPrep{0.0f, 0.0f};
+        joins[j] = 0;
+        continue;
+      }
+      if (j == 0) {
+        L.ta = t_state;
+        for (int g = 0; g < G; ++g) L.va[g] = L.held[g];
+      } else {
+        L.ta = ts[j - 1];
+        for (int g = 0; g < G; ++g) L.va[g] = L.clampv(irr[static_cast<size_t>(j - 1) * G + g]);
+      }
+      L.tb = ts[j];
+      bool up = true, down = true;
+      for (int g = 0; g < G; ++g) {
+        L.vb[g] = L.clampv(irr[static_cast<size_t>(j) * G + g]);
+        if (L.vb[g] < L.va[g]) up = false;
+        if (L.vb[g] > L.va[g]) down = false;
+      }
+      prep[j] = L.prep_segment();
+      lo[j] = std::fmin(prep[j].x_a, prep[j].x_c);
+      hi[j] = std::fmax(prep[j].x_a, prep[j].x_c);
+      joins[j] = up || down;
+    }
+    const int64_t n_blocks = (pos0 + n + 15) / 16;
+    std::vector<InstantBlock> blocks(n_blocks, InstantBlock{NAN, NAN, 0});
+    for (int64_t b = 0; b < n_blocks; ++b) {
+      const int64_t j0 = 16 * b - pos0;
+      if (j0 < 0 || j0 + 15 >= n) continue;
+      bool ok = true;
+      float mn = lo[j0], mx = hi[j0];
+      for (int64_t j = j0; j < j0 + 16; ++j) {
+        ok = ok && joins[j];
+        mn = std::fmin(mn, lo[j]);
+        mx = std::fmax(mx, hi[j]);
+      }
+      if (ok) blocks[b] = InstantBlock{mn, mx, ts[j0 + 15]};
+    }
+    VecSink s0, s1;
+    InstantLane<kG> A = base, B = base;
+    A.prep = nullptr;
+    A.blocks = nullptr;
+    B.prep = prep.data();
+    B.blocks = blocks.data();
+    B.pos0 = pos0;
+    A.advance(t_end, s0);
+    // the walk with the prepared values, in the kernel's walk/search alternation with a small budget
+    // (budget 1: the walk returns at every segment or block boundary, where the skips are counted)
+    B.begin(t_end);
+    const int budget = 1 + it % 3;
+    auto step = [&]() {
+      if (!B.seg_open && B.t_cur < t_end && B.block_quiet()) ++n_blocks_skipped;
+      return B.walk(s1, budget);
+    };
+    for (int m = step(); m != InstantLane<kG>::kDone; m = step())
+      if (m == InstantLane<kG>::kSearch) B.search_step(s1, stack.data(), 1);
+    B.end();
+    bool ok = s0.ev.size() == s1.ev.size();
+    for (size_t k = 0; ok && k < s0.ev.size(); ++k) ok = same_event(s0.ev[k], s1.ev[k]);
+    ok = ok && A.t_search == B.t_search && A.t_ref == B.t_ref && A.blind_until == B.blind_until &&
+         std::memcmp(&A.s3_ref, &B.s3_ref, 4) == 0 && std::memcmp(&A.dr_ref, &B.dr_ref, 4) == 0 &&
+         A.flags == B.flags && A.shot_next == B.shot_next && A.shot_index == B.shot_index &&
+         A.n_substeps == B.n_substeps && A.n_pieces == B.n_pieces && std::memcmp(A.held, B.held, sizeof(A.held)) == 0;
+    n_events += static_cast<long>(s0.ev.size());
+    if (!ok && ++bad <= 10)
+      std::printf("difference at trial %d: events %zu vs %zu, t_search %lld vs %lld, pieces %u vs %u\n", it, s0.ev.size(),
+                  s1.ev.size(), static_cast<long long>(A.t_search), static_cast<long long>(B.t_search), A.n_pieces,
+                  B.n_pieces);
+  }
+  std::printf("walk trials %d events %ld blocks_skipped %ld differences %ld\n", trials, n_events, n_blocks_skipped,
+              bad);
+  return bad != 0;
+}

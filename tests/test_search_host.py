@@ -1,8 +1,12 @@
-"""The instantaneous lane's exact searches compiled for the host (g++, the same fp32 policy as the
-device build: -ffp-contract=off, no fast-math): on falling segments under leak where Delta runs
-level with the OFF threshold, the monotone transition search must equal the generic time-ordered
-bisection (the reference's instant_search, src/sensor/pixel.h:1884-1916) in every field, and need
-far fewer evaluations (the reason it exists)."""
+"""The instantaneous lane's searches compiled for the host (g++, the same fp32 policy as the device
+build: -ffp-contract=off, no fast-math), against the generic time-ordered bisection (the
+reference's instant_search, src/sensor/pixel.h:1884-1916):
+  - on falling segments under leak where Delta runs level with the OFF threshold, the monotone
+    transition search must equal it in every field, with far fewer evaluations (the reason it
+    exists);
+  - phase A's whole-segment verdicts must agree with it wherever they decide;
+  - the walk with phase A's records and block summaries must give exactly the events and final
+    state of the segment-by-segment walk on exact searches alone."""
 import pathlib
 import re
 import shutil
@@ -14,14 +18,23 @@ HERE = pathlib.Path(__file__).resolve().parent
 SENSOR = HERE.parent / "ballasim_amd" / "csrc" / "sensor"
 
 
-@pytest.fixture(scope="module")
-def search_check(tmp_path_factory):
+def _build(tmp_path_factory, name):
     if shutil.which("g++") is None:
         pytest.skip("g++ not found")
-    exe = tmp_path_factory.mktemp("search") / "search_check"
+    exe = tmp_path_factory.mktemp(name) / name
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", f"-I{SENSOR}", "-o", str(exe),
-                    str(HERE / "host" / "search_check.cpp")], check=True)
+                    str(HERE / "host" / f"{name}.cpp")], check=True)
     return exe
+
+
+@pytest.fixture(scope="module")
+def search_check(tmp_path_factory):
+    return _build(tmp_path_factory, "search_check")
+
+
+@pytest.fixture(scope="module")
+def walk_check(tmp_path_factory):
+    return _build(tmp_path_factory, "walk_check")
 
 
 @pytest.mark.parametrize("seed", [12345, 777])
@@ -36,3 +49,28 @@ def test_transition_search_equals_bisection(search_check, seed):
     assert generic == 0                  # every case here is decided by the transition search
     assert 0.1 * trials < found < 0.9 * trials   # both outcomes are exercised
     assert ev_t * 10 < ev_b
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_fast_search_agrees_with_bisection(search_check, seed):
+    """Phase A's whole-segment verdicts (no crossing, or a crossing at the segment's start) on rising,
+    falling and mixed segments, with and without leak, against the generic bisection."""
+    r = subprocess.run([str(search_check), "50000", str(seed), "fast"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    m = re.search(r"fast trials (\d+) quiet (\d+) at_start (\d+) undecided (\d+) mismatches (\d+)", r.stdout)
+    assert m, r.stdout
+    trials, quiet, at_start, undecided, bad = map(int, m.groups())
+    assert bad == 0
+    assert min(quiet, at_start, undecided) > 0.05 * trials  # every verdict is exercised
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_walk_with_block_skips_equals_exact_walk(walk_check, seed):
+    r = subprocess.run([str(walk_check), "3000", str(seed)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    m = re.search(r"walk trials (\d+) events (\d+) blocks_skipped (\d+) differences (\d+)", r.stdout)
+    assert m, r.stdout
+    trials, events, skipped, bad = map(int, m.groups())
+    assert bad == 0
+    assert events > 10 * trials      # runs that fire
+    assert skipped > trials // 4     # and quiet blocks that the walk passes at once
