@@ -8,6 +8,7 @@
 // the first integer ns where Delta >= theta_on or <= -theta_off, found per sample segment by a
 // time-ordered bisection that drops a node when the bounds of I keep Delta inside the levels. The
 // reset is physical (blind on (t_c, t_c + T_r], re-reference at release) or ideal (r += p theta).
+// A hot pixel's events are flagged kFlagHotPixel and labelled HOT unless SIGNAL (fire :1563-1569).
 //
 // Every arithmetic step is the reference's, in its order, because the fp32 results must equal the
 // reference's fp32 CPU path bit for bit (events, crossing times, increments and state). What differs
@@ -34,6 +35,7 @@ struct InstantLane {
   int64_t refractory_ns;
   int G;
   bool ideal_reset;
+  bool hot = false;  // a hot pixel: its events carry kFlagHotPixel, and HOT for any label but SIGNAL
   // The call's input run, and its segments' prepared values (prep[j]: the segment ending at input
   // sample j; null: every search is exact).
   const BSX_GLOBAL int64_t* ts;
@@ -383,11 +385,12 @@ struct InstantLane {
     } else {
       label = split_label(ds, dl, D, label_cut);
     }
+    if (hot && label != kLabelSignal) label = kLabelHot;
     RawEventDev ev;
     ev.pixel = pixel;
     ev.polarity = on ? 1 : 0;
     ev.label = label;
-    ev.flags = static_cast<uint16_t>(flags | (shot ? kFlagShot : 0u));
+    ev.flags = static_cast<uint16_t>(flags | (shot ? kFlagShot : 0u) | (hot ? kFlagHotPixel : 0u));
     ev.t_cross = t;
     ev.ds = ds;
     ev.dn = dn;

@@ -15,6 +15,7 @@ inline constexpr int64_t kNoBreakpoint = INT64_MAX;
 
 // Event labels and flag bits (reference: src/sensor/pixel.h:70-78).
 inline constexpr uint8_t kLabelSignal = 0, kLabelNoise = 1, kLabelLeak = 2, kLabelHot = 3;
+inline constexpr uint16_t kFlagHotPixel = 1u << 5;
 inline constexpr uint16_t kFlagQuarantined = 1u << 9;
 inline constexpr uint16_t kFlagSolverFallback = 1u << 10;
 inline constexpr uint16_t kFlagFirstAfterInit = 1u << 12;
@@ -50,6 +51,7 @@ struct MapsDev {
   const BSX_GLOBAL float* dark;
   const BSX_GLOBAL float* scale;  // illuminance_scale x gain, formed in fp32 as the reference's scale_
   const BSX_GLOBAL float* phi0;
+  const BSX_GLOBAL uint8_t* hot;  // 0: none, 1: hot (its events are flagged and labelled HOT)
 };
 
 // Working state (the fp32 kernel's view of the reference's PixelState, src/sensor/pixel.h:105-121;

@@ -433,6 +433,7 @@ __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams,
     L.refractory_ns = kc.refractory_ns;
     L.G = kc.groups;
     L.ideal_reset = kc.ideal_reset != 0;
+    L.hot = mp.hot[p] != 0;
     const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
     L.prep = nullptr;
     L.blocks = nullptr;
@@ -618,6 +619,7 @@ __device__ __forceinline__ void cascade_lane(const CameraDev* cams, int n_cams, 
     L.refractory_ns = kc.refractory_ns;
     L.G = kc.groups;
     L.ideal_reset = kc.ideal_reset != 0;
+    L.hot = mp.hot[p] != 0;
     L.cc = cascade::Consts{kc.k, kc.a2, kc.a3, kc.tau_o, kc.tau_sf, kc.eps_accept, kc.eps_drop};
     const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
     L.prep = nullptr;

@@ -11,10 +11,14 @@ namespace bsx {
 namespace {
 
 constexpr double kTwoPi = 0x1.921fb54442d18p+2;
+// Philox blocks of kSensorMap (reference: src/sensor/maps.cpp:21-23).
+constexpr uint64_t kMapNormals = 0;   // xi_on, xi_off, xi_leak, xi_dark
+constexpr uint64_t kMapUniforms = 1;  // u_phase, u_hot, u_type, u_rate
+constexpr uint64_t kMapHotRate = 2;   // xi_rate of the log-normal hot law (hot pixels only)
 
-// Block 0 of kSensorMap: (xi_on, xi_off, xi_leak, xi_dark) by Box-Muller on two uniform pairs.
-void map_normals(uint64_t seed, uint32_t camera, uint32_t pixel, double* xi) {
-  const Word4 r = draw(seed, kPurposeSensorMap, camera, pixel, 0);
+// Four normals of one block of kSensorMap by Box-Muller on two uniform pairs.
+void map_normals(uint64_t seed, uint32_t camera, uint32_t pixel, uint64_t block, double* xi) {
+  const Word4 r = draw(seed, kPurposeSensorMap, camera, pixel, block);
   const double u0 = open_uniform64(r.v[0]), u1 = open_uniform64(r.v[1]);
   const double u2 = open_uniform64(r.v[2]), u3 = open_uniform64(r.v[3]);
   const double r0 = std::sqrt(-2.0 * pm::log(u0)), r1 = std::sqrt(-2.0 * pm::log(u2));
@@ -28,6 +32,7 @@ void map_normals(uint64_t seed, uint32_t camera, uint32_t pixel, double* xi) {
 }
 
 inline double dmax(double a, double b) { return a < b ? b : a; }
+inline double dclamp(double v, double lo, double hi) { return v < lo ? lo : hi < v ? hi : v; }
 
 }  // namespace
 
@@ -40,20 +45,41 @@ SensorMaps draw_sensor_maps(const SensorParams& p, uint64_t seed, uint32_t camer
   m.dark_fA.resize(n_pixels);
   m.phi0.resize(n_pixels);
   m.hot.assign(n_pixels, 0);
+  m.hot_rate_hz.assign(n_pixels, 0.0);
   m.gain.assign(n_pixels, 1.0f);
   const double leak_temp = arrhenius(p.leak_ea_eV, p.temperature_K, p.reference_temperature_K);
   const double dark_temp = arrhenius(p.dark_ea_eV, p.temperature_K, p.reference_temperature_K);
   const double rho = p.theta_onoff_corr, rho_c = std::sqrt(1.0 - rho * rho);
   const double ln10 = pm::log(10.0);
+  const double ln_rmin = pm::log(p.hot_rate_min_hz), ln_rmax = pm::log(p.hot_rate_max_hz);
   const auto one = [&](uint32_t px) {
     double xi[4];
-    map_normals(seed, camera, px, xi);
-    const Word4 u = draw(seed, kPurposeSensorMap, camera, px, 1);
-    const double u_phase = open_uniform64(u.v[0]);
+    map_normals(seed, camera, px, kMapNormals, xi);
+    const Word4 u = draw(seed, kPurposeSensorMap, camera, px, kMapUniforms);
+    const double u_phase = open_uniform64(u.v[0]), u_hot = open_uniform64(u.v[1]);
+    const double u_rate = open_uniform64(u.v[3]);
     const double xi_off = rho == 0.0 ? xi[1] : rho * xi[0] + rho_c * xi[1];
     const double th_on = dmax(p.theta_floor, p.theta_on + p.sigma_theta_on * xi[0]);
     const double th_off = dmax(p.theta_floor, p.theta_off + p.sigma_theta_off * xi_off);
-    const double leak_rate = p.leak_rate_hz * pm::exp(p.leak_sigma_log10 * xi[2] * ln10) * leak_temp;
+    double leak_rate = p.leak_rate_hz * pm::exp(p.leak_sigma_log10 * xi[2] * ln10) * leak_temp;
+    if (u_hot < p.hot_fraction) {
+      // A hot pixel: its target rate log-uniform on [min, max], or log-normal clamped to it. Without
+      // the cascade's noise (the only mode this library runs) every hot pixel is leak-type (the
+      // reference's rule: u_type < hot_leak_fraction, or noise off): its leak cycle meets the rate,
+      // 1 / (T_r + 1 / R_L) = R at R_L = R / (1 - R T_r) under the physical reset, R_L = R under the
+      // ideal one (validate keeps R T_r < 1).
+      double rate;
+      if (p.hot_rate_law == 0) {
+        rate = pm::exp(ln_rmin + u_rate * (ln_rmax - ln_rmin));
+      } else {
+        double z[4];
+        map_normals(seed, camera, px, kMapHotRate, z);
+        rate = dclamp(p.hot_rate_median_hz * pm::exp(p.hot_rate_sigma_ln * z[0]), p.hot_rate_min_hz, p.hot_rate_max_hz);
+      }
+      m.hot_rate_hz[px] = rate;
+      m.hot[px] = 1;
+      leak_rate = p.reset_mode == BSX_RESET_PHYSICAL ? rate / (1.0 - rate * p.refractory_s) : rate;
+    }
     m.theta_on[px] = th_on;
     m.theta_off[px] = th_off;
     m.dark_fA[px] = p.dark_current_fA * pm::exp(p.dark_sigma_ln * xi[3]) * dark_temp;

@@ -161,7 +161,7 @@ KConst SensorModel::kconst() const {
 
 void SensorModel::upload_maps() {
   const size_t a = align256(P_ * 4);
-  d_maps_.ensure(6 * a);
+  d_maps_.ensure(7 * a);
   std::vector<float> buf(P_);
   const float scale32 = static_cast<float>(params_.illuminance_scale);
   const auto put = [&](int slot, const auto& fn) {
@@ -175,6 +175,8 @@ void SensorModel::upload_maps() {
   put(3, [&](size_t i) { return static_cast<float>(maps_.dark_fA[i]); });
   put(4, [&](size_t i) { return scale32 * maps_.gain[i]; });  // kc.scale x gain in fp32
   put(5, [&](size_t i) { return static_cast<float>(maps_.phi0[i]); });
+  hip_check(hipMemcpy(static_cast<char*>(d_maps_.get()) + 6 * a, maps_.hot.data(), P_, hipMemcpyHostToDevice),
+            "upload maps");
 }
 
 StateDev SensorModel::state_view(int which) const {
@@ -212,6 +214,7 @@ static MapsDev maps_view(const DeviceBuffer& d, size_t P) {
   m.dark = reinterpret_cast<const float*>(b + 3 * a);
   m.scale = reinterpret_cast<const float*>(b + 4 * a);
   m.phi0 = reinterpret_cast<const float*>(b + 5 * a);
+  m.hot = reinterpret_cast<const uint8_t*>(b + 6 * a);
   return m;
 }
 

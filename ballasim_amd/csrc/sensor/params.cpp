@@ -155,6 +155,12 @@ void validate_ranges(const SensorParams& p) {
   need(finite(p.tau_hp) && p.tau_hp >= 0 && p.tau_hp <= 100, "tau_hp must lie in [0, 100]");
   need(finite(p.hot_fraction) && p.hot_fraction >= 0 && p.hot_fraction <= 1, "hot_fraction must lie in [0, 1]");
   need(finite(p.hot_rate_min_hz) && p.hot_rate_min_hz > 0, "hot_rate_min_hz must be > 0");
+  need(finite(p.hot_rate_max_hz) && p.hot_rate_max_hz > 0, "hot_rate_max_hz must be > 0");
+  need(p.hot_rate_law == 0 || p.hot_rate_law == 1, "hot_rate_law must be 0 (log-uniform) or 1 (log-normal)");
+  need(finite(p.hot_rate_median_hz) && p.hot_rate_median_hz > 0, "hot_rate_median_hz must be > 0");
+  need(finite(p.hot_rate_sigma_ln) && p.hot_rate_sigma_ln >= 0, "hot_rate_sigma_ln must be >= 0");
+  need(finite(p.hot_leak_fraction) && p.hot_leak_fraction >= 0 && p.hot_leak_fraction <= 1,
+       "hot_leak_fraction must lie in [0, 1]");
   need(finite(p.readout_quantum_s) && p.readout_quantum_s >= 0, "readout_quantum_s must be >= 0");
   need(finite(p.readout_delay_s) && p.readout_delay_s >= 0, "readout_delay_s must be >= 0");
   need(finite(p.clock_drift_ppm), "clock_drift_ppm must be finite");
@@ -183,7 +189,6 @@ void validate(const SensorParams& p) {
   // Features of the reference this backend does not run: refused by name, never approximated.
   need(p.veiling_glare == 0, "veiling_glare > 0 is not supported by the bsx backend");
   need(p.tau_hp == 0, "the high-pass filter (tau_hp > 0) is not supported by the bsx backend");
-  need(p.hot_fraction == 0, "hot pixels (hot_fraction > 0) are not supported by the bsx backend");
   need(p.limiter == 0, "the rate limiter is not supported by the bsx backend");
   need(p.initial_state == 0, "initial_state stationary is not supported by the bsx backend");
   need(!noise_on(p), "the cascade's in-path noise (noise on) is not supported by the bsx backend yet");

@@ -369,6 +369,7 @@ void orc_log_check(int threads, uint64_t* violations, uint64_t* hash) {
 
 typedef struct {
   float theta_on, theta_off, leak, dark, scale, phi0;
+  uint8_t hot;
 } pmap32;
 
 struct orc_model {
@@ -379,6 +380,8 @@ struct orc_model {
   size_t P;
   int threads;
   double *theta_on, *theta_off, *leak, *dark, *phi0;
+  uint8_t* hot;
+  double* hot_rate;
   pmap32* m32;
   orc_state* st;
   int64_t t_state;
@@ -404,29 +407,58 @@ static double arrhenius(double ea, double T, double T0) {
   return orc_exp(-ea / 8.617333262e-5 * (1.0 / T - 1.0 / T0));
 }
 
-/* The frozen maps (src/sensor/maps.cpp:26-39, 78-156). */
+/* Four normals of Philox block `block` of the sensor-map purpose by Box-Muller (maps.cpp:26-39). */
+static void map_normals4(const orc_model* m, uint32_t px, uint64_t block, double* xi) {
+  const double two_pi = 0x1.921fb54442d18p+2;
+  uint32_t r[4];
+  orc_draw(m->seed, 0x0002, m->camera, px, block, r);
+  const double u0 = u_open64(r[0]), u1 = u_open64(r[1]), u2 = u_open64(r[2]), u3 = u_open64(r[3]);
+  const double r0 = sqrt(-2.0 * orc_log(u0)), r1 = sqrt(-2.0 * orc_log(u2));
+  double s0, c0, s1, c1;
+  orc_sincos(two_pi * u1, &s0, &c0);
+  orc_sincos(two_pi * u3, &s1, &c1);
+  xi[0] = r0 * c0;
+  xi[1] = r0 * s0;
+  xi[2] = r1 * c1;
+  xi[3] = r1 * s1;
+}
+
+/* The frozen maps (src/sensor/maps.cpp:21-39, 78-156): block 0 the four normals, block 1 the
+ * uniforms (phase, hot, type, rate), block 2 a hot pixel's log-normal rate normal. Hot pixels are
+ * leak-type here (the reference's rule without in-path noise, maps.cpp:119-124). */
 static void draw_maps(orc_model* m) {
   const bsx_params* p = &m->p;
   const double leak_temp = arrhenius(p->leak_ea_eV, p->temperature_K, p->reference_temperature_K);
   const double dark_temp = arrhenius(p->dark_ea_eV, p->temperature_K, p->reference_temperature_K);
   const double rho = p->theta_onoff_corr, rho_c = sqrt(1.0 - rho * rho);
   const double ln10 = orc_log(10.0);
-  const double two_pi = 0x1.921fb54442d18p+2;
+  const double ln_rmin = orc_log(p->hot_rate_min_hz), ln_rmax = orc_log(p->hot_rate_max_hz);
   for (uint32_t px = 0; px < m->P; ++px) {
-    uint32_t r[4], u[4];
-    orc_draw(m->seed, 0x0002, m->camera, px, 0, r);
-    const double u0 = u_open64(r[0]), u1 = u_open64(r[1]), u2 = u_open64(r[2]), u3 = u_open64(r[3]);
-    const double r0 = sqrt(-2.0 * orc_log(u0)), r1 = sqrt(-2.0 * orc_log(u2));
-    double s0, c0, s1, c1;
-    orc_sincos(two_pi * u1, &s0, &c0);
-    orc_sincos(two_pi * u3, &s1, &c1);
-    const double xi[4] = {r0 * c0, r0 * s0, r1 * c1, r1 * s1};
+    uint32_t u[4];
+    double xi[4];
+    map_normals4(m, px, 0, xi);
     orc_draw(m->seed, 0x0002, m->camera, px, 1, u);
-    const double u_phase = u_open64(u[0]);
+    const double u_phase = u_open64(u[0]), u_hot = u_open64(u[1]), u_rate = u_open64(u[3]);
     const double xi_off = rho == 0.0 ? xi[1] : rho * xi[0] + rho_c * xi[1];
     const double th_on = dmaxd(p->theta_floor, p->theta_on + p->sigma_theta_on * xi[0]);
     const double th_off = dmaxd(p->theta_floor, p->theta_off + p->sigma_theta_off * xi_off);
-    const double leak_rate = p->leak_rate_hz * orc_exp(p->leak_sigma_log10 * xi[2] * ln10) * leak_temp;
+    double leak_rate = p->leak_rate_hz * orc_exp(p->leak_sigma_log10 * xi[2] * ln10) * leak_temp;
+    uint8_t hot = 0;
+    double rate = 0.0;
+    if (u_hot < p->hot_fraction) {
+      if (p->hot_rate_law == 0) {
+        rate = orc_exp(ln_rmin + u_rate * (ln_rmax - ln_rmin));
+      } else {
+        double z[4];
+        map_normals4(m, px, 2, z);
+        rate = p->hot_rate_median_hz * orc_exp(p->hot_rate_sigma_ln * z[0]);
+        rate = rate < p->hot_rate_min_hz ? p->hot_rate_min_hz : rate > p->hot_rate_max_hz ? p->hot_rate_max_hz : rate;
+      }
+      hot = 1;
+      leak_rate = p->reset_mode == BSX_RESET_PHYSICAL ? rate / (1.0 - rate * p->refractory_s) : rate;
+    }
+    if (m->hot) m->hot[px] = hot;
+    if (m->hot_rate) m->hot_rate[px] = rate;
     m->theta_on[px] = th_on;
     m->theta_off[px] = th_off;
     m->dark[px] = p->dark_current_fA * orc_exp(p->dark_sigma_ln * xi[3]) * dark_temp;
@@ -438,7 +470,7 @@ static void draw_maps(orc_model* m) {
 }
 
 void orc_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t n, double* theta_on,
-                   double* theta_off, double* leak, double* dark, double* phi0) {
+                   double* theta_off, double* leak, double* dark, double* phi0, uint8_t* hot, double* hot_rate) {
   orc_model m;
   memset(&m, 0, sizeof(m));
   m.p = *p;
@@ -450,6 +482,8 @@ void orc_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t
   m.leak = leak;
   m.dark = dark;
   m.phi0 = phi0;
+  m.hot = hot;
+  m.hot_rate = hot_rate;
   draw_maps(&m);
 }
 
@@ -459,11 +493,10 @@ orc_model* orc_create(const bsx_params* p, uint32_t width, uint32_t height, uint
     set_err(err, err_n, "orc_create: bad size, groups or camera");
     return NULL;
   }
-  if (p->veiling_glare != 0 || p->tau_hp != 0 || p->hot_fraction != 0 ||
+  if (p->veiling_glare != 0 || p->tau_hp != 0 ||
       (p->noise && p->photoreceptor == BSX_PHOTORECEPTOR_CASCADE) ||
       (p->shot_rate_hz > 0 && p->photoreceptor == BSX_PHOTORECEPTOR_CASCADE)) {
-    set_err(err, err_n, "orc_create: glare, high-pass filter, hot pixels and cascade noise (in-path or shot) are not "
-                        "restated");
+    set_err(err, err_n, "orc_create: glare, high-pass filter and cascade noise (in-path or shot) are not restated");
     return NULL;
   }
   orc_model* m = (orc_model*)calloc(1, sizeof(orc_model));
@@ -480,6 +513,8 @@ orc_model* orc_create(const bsx_params* p, uint32_t width, uint32_t height, uint
   m->leak = (double*)malloc(m->P * 8);
   m->dark = (double*)malloc(m->P * 8);
   m->phi0 = (double*)malloc(m->P * 8);
+  m->hot = (uint8_t*)malloc(m->P);
+  m->hot_rate = (double*)malloc(m->P * 8);
   m->m32 = (pmap32*)malloc(m->P * sizeof(pmap32));
   m->st = (orc_state*)calloc(m->P, sizeof(orc_state));
   draw_maps(m);
@@ -491,6 +526,7 @@ orc_model* orc_create(const bsx_params* p, uint32_t width, uint32_t height, uint
     m->m32[i].dark = (float)m->dark[i];
     m->m32[i].scale = scale * 1.0f;
     m->m32[i].phi0 = (float)m->phi0[i];
+    m->m32[i].hot = m->hot[i];
   }
   m->irr_max = (float)(1e9 / (p->illuminance_scale * 1.0));
   m->refractory_ns = llround(p->refractory_s * 1e9);
@@ -511,6 +547,8 @@ void orc_destroy(orc_model* m) {
   free(m->leak);
   free(m->dark);
   free(m->phi0);
+  free(m->hot);
+  free(m->hot_rate);
   free(m->m32);
   free(m->st);
   free(m->ev);
@@ -540,6 +578,7 @@ typedef struct {
   const orc_model* m;
   uint32_t pixel;
   float theta_on, theta_off, leak, dark, scale;
+  int hot;
   const int64_t* t;
   const float* irr;
   int n, skip, G;
@@ -644,11 +683,13 @@ static void fire_ex(pix* k, int64_t t, int on, float s3, float n3, int shot) {
   } else {
     label = split_label(ds, dl, D, cut);
   }
+  /* a hot pixel (pixel.h:1563-1569): HOT unless SIGNAL, and flagged (1 << 5) */
+  if (k->hot && label != 0) label = 3;
   orc_event e;
   e.pixel = k->pixel;
   e.polarity = on ? 1 : 0;
   e.label = label;
-  e.flags = (uint16_t)(k->flags | (shot ? (1u << 13) : 0u));
+  e.flags = (uint16_t)(k->flags | (shot ? (1u << 13) : 0u) | (k->hot ? (1u << 5) : 0u));
   e.t_cross = t;
   e.ds = ds;
   e.dn = dn;
@@ -856,6 +897,7 @@ static void pix_setup(pix* k, const orc_model* m, size_t p) {
   k->leak = m->m32[p].leak;
   k->dark = m->m32[p].dark;
   k->scale = m->m32[p].scale;
+  k->hot = m->m32[p].hot;
   k->G = m->G;
 }
 
@@ -1074,8 +1116,11 @@ void orc_reported(const orc_model* m, int64_t* out) {
   }
 }
 
-void orc_maps(const orc_model* m, double* a, double* b, double* c, double* d, double* e) {
+void orc_maps(const orc_model* m, double* a, double* b, double* c, double* d, double* e, uint8_t* hot,
+              double* hot_rate) {
   for (size_t i = 0; i < m->P; ++i) {
+    if (hot) hot[i] = m->hot[i];
+    if (hot_rate) hot_rate[i] = m->hot_rate[i];
     if (a) a[i] = m->theta_on[i];
     if (b) b[i] = m->theta_off[i];
     if (c) c[i] = m->leak[i];

@@ -62,9 +62,9 @@ typedef struct orc_stats {
 orc_model* orc_create(const bsx_params* p, uint32_t width, uint32_t height, uint64_t seed, uint32_t camera,
                       int32_t groups, int32_t num_threads, char* err, size_t err_n);
 void orc_destroy(orc_model* m);
-/* The frozen maps of n pixels (every output array of n doubles). */
+/* The frozen maps of n pixels (every output array of n values; hot and hot_rate may be null). */
 void orc_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t n, double* theta_on,
-                   double* theta_off, double* leak, double* dark, double* phi0);
+                   double* theta_off, double* leak, double* dark, double* phi0, uint8_t* hot, double* hot_rate);
 int orc_init(orc_model* m, int64_t t0_ns, const float* irradiance0);
 /* The reference's advance_runs: 0 ok, 1 invalid input (message in err). Events are then sorted. */
 int orc_advance_runs(orc_model* m, const uint32_t* ids, size_t n, const int64_t* t, const float* irr,
@@ -74,7 +74,8 @@ size_t orc_event_count(const orc_model* m);
 const orc_event* orc_events(const orc_model* m);
 /* t_reported of the sorted events, into out (orc_event_count entries). */
 void orc_reported(const orc_model* m, int64_t* out);
-void orc_maps(const orc_model* m, double* theta_on, double* theta_off, double* leak, double* dark, double* phi0);
+void orc_maps(const orc_model* m, double* theta_on, double* theta_off, double* leak, double* dark, double* phi0,
+              uint8_t* hot, double* hot_rate);
 void orc_states(const orc_model* m, orc_state* out);
 void orc_get_stats(const orc_model* m, orc_stats* out);
 int64_t orc_time(const orc_model* m);

@@ -61,7 +61,7 @@ def oracle_lib():
         lib.orc_create.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_int32, C.c_int32,
                                    C.c_char_p, C.c_size_t]
         lib.orc_destroy.argtypes = [P]
-        lib.orc_draw_maps.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P]
+        lib.orc_draw_maps.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P, P, P]
         lib.orc_init.argtypes = [P, C.c_int64, P]
         lib.orc_advance_runs.argtypes = [P, P, C.c_size_t, P, P, C.c_size_t, P, P, C.c_int64, C.c_char_p, C.c_size_t]
         lib.orc_event_count.restype = C.c_size_t
@@ -69,7 +69,7 @@ def oracle_lib():
         lib.orc_events.restype = P
         lib.orc_events.argtypes = [P]
         lib.orc_reported.argtypes = [P, P]
-        lib.orc_maps.argtypes = [P, P, P, P, P, P]
+        lib.orc_maps.argtypes = [P, P, P, P, P, P, P, P]
         lib.orc_states.argtypes = [P, P]
         lib.orc_get_stats.argtypes = [P, P]
         lib.orc_time.restype = C.c_int64
@@ -102,8 +102,11 @@ def draw_maps(params, seed, camera, n):
     """The oracle's frozen maps of n pixels."""
     lib = oracle_lib()
     arrs = [np.zeros(n) for _ in range(5)]
-    lib.orc_draw_maps(C.byref(params), seed, camera, n, *[_ptr(a) for a in arrs])
-    return dict(zip(("theta_on", "theta_off", "leak", "dark_fA", "phi0"), arrs))
+    hot, rate = np.zeros(n, np.uint8), np.zeros(n)
+    lib.orc_draw_maps(C.byref(params), seed, camera, n, *[_ptr(a) for a in arrs], _ptr(hot), _ptr(rate))
+    out = dict(zip(("theta_on", "theta_off", "leak", "dark_fA", "phi0"), arrs))
+    out.update(hot=hot, hot_rate_hz=rate)
+    return out
 
 
 def ref_available():
@@ -126,7 +129,7 @@ def ref_lib():
         lib.ref_event_count.argtypes = [P]
         lib.ref_events.argtypes = [P, P]
         lib.ref_reported.argtypes = [P, P]
-        lib.ref_maps.argtypes = [P, P, P, P, P, P]
+        lib.ref_maps.argtypes = [P, P, P, P, P, P, P, P]
         lib.ref_states.argtypes = [P, P]
         lib.ref_output_order.argtypes = [P, P, P, P, C.c_size_t, C.c_uint32, P]
         lib.ref_readout.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_double, P, C.c_size_t, P,
@@ -197,8 +200,11 @@ class OracleModel(_Base):
     def maps(self):
         P = self.width * self.height
         arrs = [np.zeros(P) for _ in range(5)]
-        self.lib.orc_maps(self.h, *[_ptr(a) for a in arrs])
-        return dict(zip(("theta_on", "theta_off", "leak", "dark_fA", "phi0"), arrs))
+        hot, rate = np.zeros(P, np.uint8), np.zeros(P)
+        self.lib.orc_maps(self.h, *[_ptr(a) for a in arrs], _ptr(hot), _ptr(rate))
+        out = dict(zip(("theta_on", "theta_off", "leak", "dark_fA", "phi0"), arrs))
+        out.update(hot=hot, hot_rate_hz=rate)
+        return out
 
     def states(self):
         out = np.zeros(self.width * self.height, STATE_DTYPE)
@@ -250,8 +256,11 @@ class RefModel(_Base):
     def maps(self):
         P = self.width * self.height
         arrs = [np.zeros(P) for _ in range(5)]
-        self.lib.ref_maps(self.h, *[_ptr(a) for a in arrs])
-        return dict(zip(("theta_on", "theta_off", "leak", "dark_fA", "phi0"), arrs))
+        hot, rate = np.zeros(P, np.uint8), np.zeros(P)
+        self.lib.ref_maps(self.h, *[_ptr(a) for a in arrs], _ptr(hot), _ptr(rate))
+        out = dict(zip(("theta_on", "theta_off", "leak", "dark_fA", "phi0"), arrs))
+        out.update(hot=hot, hot_rate_hz=rate)
+        return out
 
     def states(self):
         out = np.zeros(self.width * self.height, REF_STATE_DTYPE)

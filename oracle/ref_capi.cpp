@@ -64,6 +64,13 @@ SensorParams to_ref(const bsx_params& b) {
   p.leak_ea_eV = b.leak_ea_eV;
   p.tau_hp = b.tau_hp;
   p.hot_fraction = b.hot_fraction;
+  p.hot_rate_law = b.hot_rate_law == 1 ? HotRateLaw::kLogNormal : HotRateLaw::kLogUniform;
+  p.hot_rate_min_hz = b.hot_rate_min_hz;
+  p.hot_rate_max_hz = b.hot_rate_max_hz;
+  p.hot_rate_median_hz = b.hot_rate_median_hz;
+  p.hot_rate_sigma_ln = b.hot_rate_sigma_ln;
+  p.hot_leak_fraction = b.hot_leak_fraction;
+  p.hot_reference_current_fA = b.hot_reference_current_fA;
   p.readout_quantum_s = b.readout_quantum_s;
   p.readout_delay_s = b.readout_delay_s;
   p.clock_offset_ns = b.clock_offset_ns;
@@ -196,9 +203,12 @@ void ref_reported(void* h, int64_t* out) {
   for (size_t i = 0; i < r->ev.size(); ++i) out[i] = r->m->reported_time(r->ev[i].t_cross);
 }
 
-void ref_maps(void* h, double* theta_on, double* theta_off, double* leak, double* dark, double* phi0) {
+void ref_maps(void* h, double* theta_on, double* theta_off, double* leak, double* dark, double* phi0, uint8_t* hot,
+              double* hot_rate) {
   const SensorMaps& m = static_cast<RefModel*>(h)->m->maps();
   for (size_t i = 0; i < m.pixels.size(); ++i) {
+    hot[i] = static_cast<uint8_t>(m.pixels[i].hot);
+    hot_rate[i] = m.hot_target_hz[i];
     theta_on[i] = m.pixels[i].theta_on;
     theta_off[i] = m.pixels[i].theta_off;
     leak[i] = m.pixels[i].leak;

@@ -60,7 +60,9 @@ def test_presets_and_params_surface():
     ("eps_ci", 0.01, "eps_ci"),
     ("refractory_s", -1.0, "refractory_s"),
     ("veiling_glare", 0.1, "not supported"),
-    ("hot_fraction", 1e-3, "not supported"),
+    ("tau_hp", 0.01, "not supported"),
+    ("hot_rate_law", 2, "hot_rate_law"),
+    ("hot_leak_fraction", 1.5, "hot_leak_fraction"),
     ("family", "foo", "family"),
 ])
 def test_validate_refuses_by_name(field, value, msg):
@@ -95,3 +97,32 @@ def test_draw_maps_host_entry_point():
     bad.theta_on = -1.0
     with pytest.raises(ValueError):
         _abi.check(lib.bsx_draw_maps(C.byref(bad.c), 1, 0, n, None, None, None, None, None))
+
+
+@pytest.mark.parametrize("law,reset", [(0, "physical"), (1, "ideal"), (1, "physical")])
+def test_draw_maps_hot_pixels_equal_oracle(law, reset):
+    """Hot pixels in the product's host maps (bsx_draw_maps): the raised leaks of leak-type hot pixels
+    equal the oracle's (itself pinned to the compiled reference by test_oracle_vs_ref.py)."""
+    from oracle import oracle as O
+    lib = _abi.load_library()
+    p = bsx.preset("ideal")
+    p.hot_fraction = 0.25
+    p.hot_rate_law = law
+    p.reset_mode = reset
+    p.refractory_s = 1e-4
+    p.hot_rate_min_hz, p.hot_rate_max_hz = 5.0, 3000.0
+    p.hot_rate_median_hz, p.hot_rate_sigma_ln = 80.0, 1.2
+    p.leak_rate_hz = 0.1
+    p.validate()
+    n = 4000
+    a = [np.zeros(n) for _ in range(5)]
+    _abi.check(lib.bsx_draw_maps(C.byref(p.c), 77, 3, n, *[x.ctypes.data_as(C.c_void_p) for x in a]))
+    om = O.draw_maps(p.c, 77, 3, n)
+    for key, x in zip(("theta_on", "theta_off", "leak", "dark_fA", "phi0"), a):
+        assert np.array_equal(x, om[key]), key
+    hot = om["hot"] == 1
+    assert 0.2 < hot.mean() < 0.3
+    # a leak-type hot pixel's leak is theta_on R_L with R_L = R / (1 - R T_r) (physical) or R (ideal)
+    R = om["hot_rate_hz"][hot]
+    RL = R / (1.0 - R * 1e-4) if reset == "physical" else R
+    assert np.allclose(a[2][hot], a[0][hot] * RL, rtol=1e-12)

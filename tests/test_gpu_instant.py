@@ -22,6 +22,11 @@ CASES = {
     # hold thousands of arrivals (inside segments, at segment ends, in dead time)
     "shot_physical": dict(reset_mode="physical", refractory_s=2e-5, leak_rate_hz=0.5, shot_rate_hz=400.0),
     "shot_ideal_leak": dict(refractory_s=1e-5, leak_rate_hz=3.0, shot_rate_hz=150.0, sigma_theta_on=0.02),
+    # hot pixels (leak-type, raised leak): HOT labels and the hot-pixel flag, with shot arrivals too
+    "hot_physical": dict(reset_mode="physical", refractory_s=5e-5, leak_rate_hz=0.2, hot_fraction=0.15,
+                         hot_rate_law=1, hot_rate_min_hz=20.0, hot_rate_max_hz=5000.0, hot_rate_median_hz=300.0,
+                         hot_rate_sigma_ln=1.0, shot_rate_hz=50.0),
+    "hot_ideal_loguniform": dict(refractory_s=1e-5, hot_fraction=0.3, hot_rate_min_hz=50.0, hot_rate_max_hz=4000.0),
 }
 
 
@@ -46,11 +51,13 @@ def test_instant_matches_oracle(case):
     om = ref.maps()
     assert np.array_equal(maps["theta_on"].reshape(-1), om["theta_on"])
     assert np.array_equal(maps["phi0"].reshape(-1), om["phi0"])
+    assert np.array_equal(maps["leak"].reshape(-1), om["leak"])
+    assert np.array_equal(maps["hot"].reshape(-1), om["hot"])
     irr0 = np.exp(rng.normal(3, 1, size=(W * H, G))).astype(np.float32)
     gpu.init(-1000, irr0)
     ref.init(-1000, irr0)
     t = -1000
-    total = 0
+    total = n_hot = 0
     for call in range(4):
         t1 = t + int(rng.integers(1, 3 * 2 ** 23))
         ids, ts, ir, offs = random_runs(rng, W * H, t, t1, G=G)
@@ -61,9 +68,12 @@ def test_instant_matches_oracle(case):
         assert len(got) == len(want) and i == len(got), f"call {call}: first difference at {i}: gpu {a} oracle {b}"
         assert np.array_equal(ev["t_reported_ns"], rep)
         total += len(got)
+        n_hot += int(np.count_nonzero(want["flags"] & (1 << 5)))
         t = t1
     assert total > 1000
     assert gpu.time_ns == t
+    if CASES[case].get("hot_fraction", 0) > 0:
+        assert n_hot > 100
     # the work counters the reference defines for this path (the leaf intervals are this backend's own)
     ks, ko = gpu.stats(), ref.stats()
     for f in ("substeps", "pieces"):

@@ -133,3 +133,60 @@ def test_maps_random_seeds(name):
         ref = O.RefModel(p, 3000, 1, seed, 1, cam, 1, True).maps()
         for k in ours:
             assert np.array_equal(ours[k], ref[k]), k
+
+
+def _random_hot(rng, p):
+    """Hot pixels on (a large fraction, so that every law and reset shows up in small frames)."""
+    p.noise = 0
+    p.hot_fraction = float(rng.uniform(0.05, 0.4))
+    p.hot_rate_law = int(rng.integers(0, 2))
+    p.hot_rate_min_hz = float(np.exp(rng.uniform(0, 3)))
+    p.hot_rate_max_hz = p.hot_rate_min_hz * float(np.exp(rng.uniform(0.5, 4)))
+    p.hot_rate_median_hz = float(np.exp(rng.uniform(1, 5)))
+    p.hot_rate_sigma_ln = float(rng.uniform(0, 2))
+    p.hot_leak_fraction = float(rng.uniform(0, 1))
+    if p.hot_rate_max_hz * p.refractory_s >= 1.0:
+        p.refractory_s = 0.5 / p.hot_rate_max_hz
+    return p
+
+
+@pytest.mark.parametrize("trial", range(4))
+def test_hot_maps_random_seeds(trial):
+    """Hot pixels (maps.cpp:106-124, without in-path noise: all leak-type): the flags, target rates and
+    the raised leaks equal the reference's, under both rate laws and both resets."""
+    rng = np.random.default_rng(7000 + trial)
+    p = _random_hot(rng, preset(["prophesee_gen3", "ideal"][trial % 2]))
+    p.reset_mode = trial // 2
+    seed, cam = int(rng.integers(0, 2 ** 63)), int(rng.integers(0, 256))
+    ours = O.draw_maps(p, seed, cam, 3000)
+    ref = O.RefModel(p, 3000, 1, seed, 1, cam, 1, True).maps()
+    for k in ours:
+        assert np.array_equal(ours[k], ref[k]), k
+    assert 0 < ours["hot"].sum() < 3000
+    assert np.all((ours["hot_rate_hz"] > 0) == (ours["hot"] == 1))
+
+
+@pytest.mark.parametrize("trial", range(4))
+def test_hot_instant_random_sweep(trial):
+    """Events of hot pixels: HOT labels (unless SIGNAL) and the hot-pixel flag, bit for bit."""
+    rng = np.random.default_rng(8000 + trial)
+    p = _random_hot(rng, _random_instant_params(rng))
+    W, H, G = 16, 10, 1
+    seed = int(rng.integers(0, 2 ** 63))
+    a = O.OracleModel(p, W, H, seed, G, 3, 2)
+    b = O.RefModel(p, W, H, seed, G, 3, 2, True)
+    irr0 = np.exp(rng.normal(2, 1.0, (W * H, G))).astype(np.float32)
+    t0 = int(rng.integers(-10 ** 9, 10 ** 9))
+    a.init(t0, irr0)
+    b.init(t0, irr0)
+    t = t0
+    n_hot = 0
+    for _ in range(3):
+        t1 = t + int(rng.integers(2 ** 22, 3 * 2 ** 23))
+        ids, ts, ir, offs = random_runs(rng, W * H, t, t1, G=G, log_sigma=1.0)
+        ea, ra = a.advance_runs(ids, ts, ir, offs, t1)
+        eb, rb = b.advance_runs(ids, ts, ir, offs, t1)
+        assert np.array_equal(ea, eb) and np.array_equal(ra, rb)
+        n_hot += int(np.count_nonzero(ea["flags"] & (1 << 5)))
+        t = t1
+    assert n_hot > 0
