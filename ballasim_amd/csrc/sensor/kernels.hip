@@ -91,19 +91,7 @@ __device__ __forceinline__ void wave_segmented_add(float* dst, int key, float v)
 // kSearchWeight per threshold's worth of log-intensity change (where the searches and events are).
 constexpr float kSearchWeight = 32.0f;
 
-// Minimum of v over the lanes of the wave that share `key` (keys non-decreasing along the lanes),
-// folded once per key into dst[key] by the key's last lane (key < 0: no contribution).
-__device__ __forceinline__ void wave_segmented_min(int* dst, int key, int v) {
-  const int lane = __lane_id();
-  for (int off = 1; off < 64; off <<= 1) {
-    const int vu = __shfl_up(v, off);
-    const int ku = __shfl_up(key, off);
-    if (lane >= off && ku == key) v = vu < v ? vu : v;
-  }
-  const int kn = __shfl_down(key, 1);
-  if (key >= 0 && (lane == 63 || kn != key) && v != INT32_MAX) atomicMin(dst + key, v);
-}
-
+// wave_segmented_add for counts.
 __device__ __forceinline__ void wave_segmented_count(uint32_t* dst, int key, uint32_t v) {
   const int lane = __lane_id();
   for (int off = 1; off < 64; off <<= 1) {
@@ -173,6 +161,85 @@ __device__ __forceinline__ int camera_at(const CameraDev* cams, int n_cams, int6
   return c;
 }
 
+// The runs of a 256-thread block of consecutive positions q0 + threadIdx.x: the runs [first, first +
+// 256) staged in LDS (first = the run of q0), so that each thread finds its run by bisection in LDS
+// and reads its pixel and row start there. Positions beyond the staged runs (a block that spans more
+// than 256 runs, some of them empty) continue with a forward walk in global memory. Called by every
+// thread of the block.
+struct BlockRuns {
+  int64_t pos[256];
+  int64_t row_start[256];
+  uint32_t pixel[256];
+  int64_t first;
+};
+
+struct RunRef {
+  int64_t i, pos, row_start;
+  uint32_t pixel;
+};
+
+__device__ __forceinline__ RunRef stage_block_runs(BlockRuns& br, const int64_t* pos, const RunsDev& runs,
+                                                   const uint32_t* pixel, int64_t n_runs, int64_t q0, int64_t q,
+                                                   int64_t total) {
+  if (threadIdx.x == 0) br.first = q0 < total ? run_of_position(pos, n_runs, q0) : 0;
+  __syncthreads();
+  const int64_t first = br.first;
+  const int64_t left = n_runs - first;
+  const int m = static_cast<int>(left < 256 ? left : 256);
+  if (static_cast<int>(threadIdx.x) < m) {
+    const int64_t r = first + threadIdx.x;
+    br.pos[threadIdx.x] = pos[r];
+    br.row_start[threadIdx.x] = runs.row_start[r];
+    br.pixel[threadIdx.x] = pixel[r];
+  }
+  __syncthreads();
+  RunRef ref{first, 0, 0, 0};
+  if (!(q < total) || m <= 0) return ref;
+  int lo = 0, hi = m;  // br.pos[lo] <= q (br.pos[0] <= q0 <= q); the answer lies in [lo, hi)
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (br.pos[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  int64_t i = first + lo;
+  if (lo == m - 1 && i + 1 < n_runs) {
+    while (i + 1 < n_runs && pos[i + 1] <= q) ++i;
+  }
+  if (i == first + lo) {
+    ref.i = i;
+    ref.pos = br.pos[lo];
+    ref.row_start = br.row_start[lo];
+    ref.pixel = br.pixel[lo];
+  } else {
+    ref.i = i;
+    ref.pos = pos[i];
+    ref.row_start = runs.row_start[i];
+    ref.pixel = pixel[i];
+  }
+  return ref;
+}
+
+// Sum of cost and minimum of loud over the lanes of the wave that share `key` (keys non-decreasing
+// along the lanes), folded once per key into work[key] and first_loud[key] by the key's last lane
+// (key < 0: no contribution; a null destination is skipped).
+__device__ __forceinline__ void wave_segmented_cost_loud(float* work, int* first_loud, int key, float cost, int loud) {
+  const int lane = __lane_id();
+  for (int off = 1; off < 64; off <<= 1) {
+    const float cu = __shfl_up(cost, off);
+    const int lu = __shfl_up(loud, off);
+    const int ku = __shfl_up(key, off);
+    if (lane >= off && ku == key) {
+      cost += cu;
+      loud = lu < loud ? lu : loud;
+    }
+  }
+  const int kn = __shfl_down(key, 1);
+  if (key >= 0 && (lane == 63 || kn != key)) {
+    if (work != nullptr) atomicAdd(work + key, cost);
+    if (first_loud != nullptr && loud != INT32_MAX) atomicMin(first_loud + key, loud);
+  }
+}
+
 // Phase A of the instantaneous response over every segment of every camera's runs: one block of 256
 // consecutive positions of one camera (the segment bases are multiples of 256).
 template <int kMaxG>
@@ -193,7 +260,8 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
   int* const first_loud = cd.first_loud;
   const int64_t t_state = cd.t_state, total = cd.total, n_runs = cd.n_runs;
   const int64_t q0 = g0 - cd.seg_base, q = q0 + threadIdx.x;
-  const int64_t i_q = block_run_of_position(pos, n_runs, q0, q, total);
+  __shared__ BlockRuns br;
+  const RunRef rr = stage_block_runs(br, pos, runs, pixel, n_runs, q0, q, total);
   int run = -1;
   float cost = 0.0f;
   int loud = INT32_MAX;
@@ -202,10 +270,10 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
   float b_lo = 0.0f, b_hi = 0.0f;
   int64_t t_b = 0;
   if (q < total) {
-    const int64_t i = i_q, j = q - pos[i];
+    const int64_t i = rr.i, j = q - rr.pos;
     run = static_cast<int>(i);
-    const uint32_t p = pixel[i];
-    const int64_t s = runs.row_start[i];
+    const uint32_t p = rr.pixel;
+    const int64_t s = rr.row_start;
     InstantLane<kMaxG> L;
     L.G = kc.groups;
     L.irr_max = kc.irr_max;
@@ -263,8 +331,7 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
       cost = quiet ? 0.0f : 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) * __builtin_amdgcn_rcpf(theta);
     }
   }
-  if (work != nullptr) wave_segmented_add(work, run, cost);
-  if (first_loud != nullptr) wave_segmented_min(first_loud, run, loud);
+  wave_segmented_cost_loud(work, first_loud, run, cost, loud);
   if (blocks != nullptr) block_summary(blocks, q, total, run, joins, b_lo, b_hi, t_b);
 }
 
