@@ -85,7 +85,7 @@ void LaunchChain::upload(const std::vector<CameraDev>& cams, int64_t t_end, hipS
   int64_t rb = 0, sb = 0;
   for (const CameraDev& c : cams) {
     rb = std::max(rb, c.run_base + round_up(c.n_runs, 256));
-    sb = std::max(sb, c.seg_base + round_up(c.total, 256));
+    sb = std::max(sb, c.seg_base + round_up(c.total, kSegmentAlign));
   }
   h_call_->n_runs = rb;
   h_call_->n_segments = sb;
@@ -123,7 +123,7 @@ void LaunchChain::launch(std::vector<CameraDev>& cams, bool instant, int64_t t_e
     c.seg_base = sb;
     base += static_cast<uint64_t>(c.kc.n_pixels);
     rb += round_up(c.n_runs, 256);
-    sb += round_up(c.total, 256);
+    sb += round_up(c.total, kSegmentAlign);
     groups = std::max(groups, c.kc.groups);
   }
   if (base > 0xffffffffull) throw std::invalid_argument("launch chain: more than 2^32 - 1 pixels in one launch");
@@ -139,7 +139,7 @@ void LaunchChain::launch(std::vector<CameraDev>& cams, bool instant, int64_t t_e
   } else if (rb > run_capacity_ || sb > seg_capacity_) {
     // grow with headroom, so that the calls that follow (about the same size) replay the graph
     run_capacity_ = std::max(run_capacity_, round_up(rb + rb / 4, 256));
-    seg_capacity_ = std::max(seg_capacity_, round_up(sb + sb / 4, 256));
+    seg_capacity_ = std::max(seg_capacity_, round_up(sb + sb / 4, kSegmentAlign));
     graph_valid_ = false;
   }
   const void* before[6] = {d_cams_.get(), d_call_.get(), d_key_.get(), d_perm_.get(), d_hist_.get(), d_queue_.get()};

@@ -98,6 +98,10 @@ struct InstantPrep {
 // against each other, the least and greatest of their x_a and x_c, and the time of the last one's end
 // sample; otherwise x_min = x_max = NaN.
 inline constexpr int kBlockSegments = 16;
+// Phase A: consecutive positions per thread, and the positions of a 256-thread block, to which every
+// camera's segment base is aligned.
+inline constexpr int kPrepPerThread = 4;
+inline constexpr int64_t kSegmentAlign = 256 * kPrepPerThread;
 struct InstantBlock {
   float x_min, x_max;
   int64_t t_last;
@@ -108,8 +112,8 @@ struct RawEventDev;
 // One camera's view for a launch. A launch covers one camera or a rig of several: their pixels form
 // one index space (camera c holds [base, base + kc.n_pixels)), their runs one run space ([run_base,
 // run_base + n_runs)) and their segments one segment space ([seg_base, seg_base + total)); run_base
-// and seg_base are multiples of 256, so that no 256-thread block of a run- or segment-indexed kernel
-// spans two cameras.
+// is a multiple of 256 and seg_base of kSegmentAlign, so that no block of a run- or segment-indexed
+// kernel spans two cameras.
 struct CameraDev {
   KConst kc;
   MapsDev mp;

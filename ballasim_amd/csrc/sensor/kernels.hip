@@ -103,26 +103,30 @@ __device__ __forceinline__ void wave_segmented_count(uint32_t* dst, int key, uin
   if (key >= 0 && (lane == 63 || kn != key) && v != 0) atomicAdd(dst + key, v);
 }
 
-// Phase A's block summaries: the 16 lanes of positions [16 b, 16 b + 16) (16-aligned: a block's
-// positions start at a multiple of 256) reduce their x ranges; the block is valid when all 16 join
-// and share one run. Every block with a position below total is written.
-__device__ __forceinline__ void block_summary(InstantBlock* blocks, int64_t q, int64_t total, int run, bool joins,
-                                              float lo, float hi, int64_t t_b) {
+// Phase A's block summaries: the kBlockSegments / kPrepPerThread lanes of positions [16 b, 16 b + 16)
+// (a block of phase A starts at a multiple of kSegmentAlign) combine their threads' x ranges; the
+// block is valid when every thread's positions all join and share one run. ok: this thread's
+// positions all joined one run, run_of: that run. The first thread of each block writes the summary,
+// when the block's first position is below total.
+__device__ __forceinline__ void block_summary(InstantBlock* blocks, int64_t q_first, int64_t total, bool ok,
+                                              int run_of, float lo, float hi, int64_t t_last) {
+  constexpr int kLanes = kBlockSegments / kPrepPerThread;
   const int lane = __lane_id();
-  const int first = __shfl(run, lane & ~(kBlockSegments - 1));
-  int ok = joins && run >= 0 && run == first ? 1 : 0;
-  for (int off = 1; off < kBlockSegments; off <<= 1) {
-    ok &= __shfl_xor(ok, off);
+  int kmin = ok ? run_of : -1 - lane, kmax = kmin;
+  for (int off = 1; off < kLanes; off <<= 1) {
+    kmin = min(kmin, __shfl_xor(kmin, off));
+    kmax = max(kmax, __shfl_xor(kmax, off));
     lo = vmin(lo, __shfl_xor(lo, off));
     hi = vmax(hi, __shfl_xor(hi, off));
   }
-  const int64_t t_last = __shfl(t_b, lane | (kBlockSegments - 1));
-  if ((lane & (kBlockSegments - 1)) == 0 && q < total) {
+  const int64_t tl = __shfl(t_last, lane | (kLanes - 1));
+  if ((lane & (kLanes - 1)) == 0 && q_first < total) {
+    const bool valid = kmin == kmax && kmin >= 0;
     InstantBlock b;
-    b.x_min = ok ? lo : nan_<float>();
-    b.x_max = ok ? hi : nan_<float>();
-    b.t_last = t_last;
-    blocks[q / kBlockSegments] = b;
+    b.x_min = valid ? lo : nan_<float>();
+    b.x_max = valid ? hi : nan_<float>();
+    b.t_last = tl;
+    blocks[q_first / kBlockSegments] = b;
   }
 }
 
@@ -171,6 +175,7 @@ struct BlockRuns {
   int64_t row_start[256];
   uint32_t pixel[256];
   int64_t first;
+  int count;
 };
 
 struct RunRef {
@@ -186,6 +191,7 @@ __device__ __forceinline__ RunRef stage_block_runs(BlockRuns& br, const int64_t*
   const int64_t first = br.first;
   const int64_t left = n_runs - first;
   const int m = static_cast<int>(left < 256 ? left : 256);
+  if (threadIdx.x == 0) br.count = m;
   if (static_cast<int>(threadIdx.x) < m) {
     const int64_t r = first + threadIdx.x;
     br.pos[threadIdx.x] = pos[r];
@@ -219,6 +225,22 @@ __device__ __forceinline__ RunRef stage_block_runs(BlockRuns& br, const int64_t*
   return ref;
 }
 
+// Moves ref forward to the run of position q >= ref's position (staged runs from LDS, the others from
+// global memory).
+__device__ __forceinline__ void advance_run(RunRef& ref, const BlockRuns& br, const int64_t* pos, const RunsDev& runs,
+                                            const uint32_t* pixel, int64_t n_runs, int64_t q) {
+  while (ref.i + 1 < n_runs) {
+    const int64_t k = ref.i + 1 - br.first;
+    const bool staged = k < br.count;
+    const int64_t next = staged ? br.pos[k] : pos[ref.i + 1];
+    if (next > q) break;
+    ref.i += 1;
+    ref.pos = next;
+    ref.row_start = staged ? br.row_start[k] : runs.row_start[ref.i];
+    ref.pixel = staged ? br.pixel[k] : pixel[ref.i];
+  }
+}
+
 // Sum of cost and minimum of loud over the lanes of the wave that share `key` (keys non-decreasing
 // along the lanes), folded once per key into work[key] and first_loud[key] by the key's last lane
 // (key < 0: no contribution; a null destination is skipped).
@@ -240,11 +262,14 @@ __device__ __forceinline__ void wave_segmented_cost_loud(float* work, int* first
   }
 }
 
-// Phase A of the instantaneous response over every segment of every camera's runs: one block of 256
-// consecutive positions of one camera (the segment bases are multiples of 256).
+// Phase A of the instantaneous response over every segment of every camera's runs: one block of
+// kSegmentAlign consecutive positions of one camera (the segment bases are multiples of it), each
+// thread kPrepPerThread consecutive positions, so that a pixel's constants are loaded once for its
+// consecutive segments. A thread adds the costs and first loud segments of the runs that end among
+// its positions directly, and those of its last run through the wave's segmented reduction.
 template <int kMaxG>
 __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, int n_cams, const CallDev* call) {
-  const int64_t g0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
+  const int64_t g0 = static_cast<int64_t>(blockIdx.x) * kSegmentAlign;
   if (g0 >= call->n_segments) return;  // beyond this call's segments (a grid sized for a larger one)
   // The block's camera, copied to registers before any store (so that no store forces a reload).
   const CameraDev& cd = cams[camera_at<Space::kSegments>(cams, n_cams, g0)];
@@ -259,80 +284,100 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
   float* const work = cd.work;
   int* const first_loud = cd.first_loud;
   const int64_t t_state = cd.t_state, total = cd.total, n_runs = cd.n_runs;
-  const int64_t q0 = g0 - cd.seg_base, q = q0 + threadIdx.x;
+  const int64_t q0 = g0 - cd.seg_base, qt = q0 + static_cast<int64_t>(threadIdx.x) * kPrepPerThread;
   __shared__ BlockRuns br;
-  const RunRef rr = stage_block_runs(br, pos, runs, pixel, n_runs, q0, q, total);
-  int run = -1;
-  float cost = 0.0f;
-  int loud = INT32_MAX;
-  // This lane's part of its block summary: the segment's x range, its end time, whether it may join.
-  bool joins = false;
-  float b_lo = 0.0f, b_hi = 0.0f;
-  int64_t t_b = 0;
-  if (q < total) {
-    const int64_t i = rr.i, j = q - rr.pos;
-    run = static_cast<int>(i);
+  RunRef rr = stage_block_runs(br, pos, runs, pixel, n_runs, q0, qt, total);
+  // the partial of the run being accumulated (acc_run), and this thread's block-summary part
+  int acc_run = -1;
+  float cost_acc = 0.0f;
+  int loud_acc = INT32_MAX;
+  bool blk_ok = true;
+  int blk_run = -1;
+  float b_lo = inf_<float>(), b_hi = -inf_<float>();
+  int64_t t_last = 0;
+  InstantLane<kMaxG> L;
+  int64_t loaded = -1;  // the run whose pixel constants L holds
+  for (int k = 0; k < kPrepPerThread; ++k) {
+    const int64_t q = qt + k;
+    if (!(q < total)) {
+      blk_ok = false;
+      continue;
+    }
+    advance_run(rr, br, pos, runs, pixel, n_runs, q);
+    const int run = static_cast<int>(rr.i);
+    if (run != acc_run) {
+      if (acc_run >= 0) {
+        if (work != nullptr) atomicAdd(work + acc_run, cost_acc);
+        if (first_loud != nullptr && loud_acc != INT32_MAX) atomicMin(first_loud + acc_run, loud_acc);
+      }
+      acc_run = run;
+      cost_acc = 0.0f;
+      loud_acc = INT32_MAX;
+    }
+    if (k == 0) blk_run = run;
+    else if (blk_run != run) blk_ok = false;
     const uint32_t p = rr.pixel;
-    const int64_t s = rr.row_start;
-    InstantLane<kMaxG> L;
-    L.G = kc.groups;
-    L.irr_max = kc.irr_max;
-    L.dark = mp.dark[p];
-    L.scale = mp.scale[p];
-    L.leak = mp.leak[p];
-    L.theta_on = mp.theta_on[p];
-    L.theta_off = mp.theta_off[p];
-    L.i_scale = in.i_scale[p];
-    L.s3_ref = in.s3_ref[p];
-    L.n3_ref = in.n3_ref[p];
-    L.dr_ref = in.dr_ref[p];
-    L.t_ref = in.t_ref[p];
-    L.blind_until = in.blind_until[p];
-    L.t_search = in.t_search[p];
-    L.shot_next = in.shot_next[p];
-    const int64_t* t = runs.t + s;
-    const float* v = runs.irr + s * kc.groups;
-    bool empty = false;
+    if (rr.i != loaded) {
+      loaded = rr.i;
+      L.G = kc.groups;
+      L.irr_max = kc.irr_max;
+      L.dark = mp.dark[p];
+      L.scale = mp.scale[p];
+      L.leak = mp.leak[p];
+      L.theta_on = mp.theta_on[p];
+      L.theta_off = mp.theta_off[p];
+      L.i_scale = in.i_scale[p];
+      L.s3_ref = in.s3_ref[p];
+      L.n3_ref = in.n3_ref[p];
+      L.dr_ref = in.dr_ref[p];
+      L.t_ref = in.t_ref[p];
+      L.blind_until = in.blind_until[p];
+      L.t_search = in.t_search[p];
+      L.shot_next = in.shot_next[p];
+    }
+    const int64_t j = q - rr.pos;
+    const int64_t* t = runs.t + rr.row_start;
+    const float* v = runs.irr + rr.row_start * kc.groups;
     if (j == 0) {
       if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
         prep[q] = InstantPrep{0.0f, 0.0f};
-        empty = true;
-      } else {
-        L.ta = t_state;
-        for (int g = 0; g < kMaxG; ++g)
-          if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
+        blk_ok = false;
+        continue;
       }
+      L.ta = t_state;
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
     } else {
       L.ta = t[j - 1];
       for (int g = 0; g < kMaxG; ++g)
         if (g < kc.groups) L.va[g] = L.clampv(v[(j - 1) * kc.groups + g]);
     }
-    if (!empty) {
-      L.tb = t[j];
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
-      const InstantPrep r = L.prep_segment();
-      prep[q] = r;
-      const float theta = vmin(L.theta_on, L.theta_off);
-      const bool quiet = L.quiet_segment(r);
-      bool up = true, down = true;
-      for (int g = 0; g < kMaxG; ++g) {
-        if (g < kc.groups) {
-          if (L.vb[g] < L.va[g]) up = false;
-          if (L.vb[g] > L.va[g]) down = false;
-        }
+    L.tb = t[j];
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
+    const InstantPrep r = L.prep_segment();
+    prep[q] = r;
+    const float theta = vmin(L.theta_on, L.theta_off);
+    const bool quiet = L.quiet_segment(r);
+    bool up = true, down = true;
+    for (int g = 0; g < kMaxG; ++g) {
+      if (g < kc.groups) {
+        if (L.vb[g] < L.va[g]) up = false;
+        if (L.vb[g] > L.va[g]) down = false;
       }
-      joins = up || down;
-      b_lo = vmin(r.x_a, r.x_c);
-      b_hi = vmax(r.x_a, r.x_c);
-      t_b = L.tb;
-      loud = quiet ? INT32_MAX : static_cast<int>(j);
+    }
+    if (!(up || down)) blk_ok = false;
+    b_lo = vmin(b_lo, vmin(r.x_a, r.x_c));
+    b_hi = vmax(b_hi, vmax(r.x_a, r.x_c));
+    t_last = L.tb;
+    if (!quiet) {
+      loud_acc = vmin(loud_acc, static_cast<int>(j));
       // (a schedule only: the hardware reciprocal instead of a correctly rounded division)
-      cost = quiet ? 0.0f : 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) * __builtin_amdgcn_rcpf(theta);
+      cost_acc += 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) * __builtin_amdgcn_rcpf(theta);
     }
   }
-  wave_segmented_cost_loud(work, first_loud, run, cost, loud);
-  if (blocks != nullptr) block_summary(blocks, q, total, run, joins, b_lo, b_hi, t_b);
+  wave_segmented_cost_loud(work, first_loud, acc_run, cost_acc, loud_acc);
+  if (blocks != nullptr) block_summary(blocks, qt, total, blk_ok, blk_run, b_lo, b_hi, t_last);
 }
 
 // Before phase A: zero each camera's counters, the order histogram and the queue counter (block 0),
@@ -885,11 +930,12 @@ hipError_t launch_chain_prepare(const ChainArgs& a, hipStream_t stream) {
   chain_init_kernel<<<run_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call, a.hist, a.queue);
   if (a.seg_capacity > 0) {
     const dim3 grid = grid_for(static_cast<size_t>(a.seg_capacity), 256);
+    const dim3 prep_grid = grid_for(static_cast<size_t>(a.seg_capacity), kSegmentAlign);
     if (!a.instant) chain_cost_kernel<<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
-    else if (a.groups <= 1) chain_prep_kernel<1><<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
-    else if (a.groups <= 2) chain_prep_kernel<2><<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
-    else if (a.groups <= 4) chain_prep_kernel<4><<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
-    else chain_prep_kernel<8><<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else if (a.groups <= 1) chain_prep_kernel<1><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else if (a.groups <= 2) chain_prep_kernel<2><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else if (a.groups <= 4) chain_prep_kernel<4><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else chain_prep_kernel<8><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
   }
   if (a.n_total > 0) {
     order_count_kernel<<<grid_for(a.n_total, 256), 256, 0, stream>>>(a.cams, a.n_cams, a.n_total, a.key, a.hist);
