@@ -163,9 +163,8 @@ struct InstantLane {
   //     brackets the crossing in the first node of constant dl that reaches it. (The generic
   //     bisection's bounds of I cannot drop nodes where Delta runs level with -theta_off, where the
   //     leak balances a slow fall, and visits most of their integers.) A segment where the upper
-  //     envelope allows an ON crossing takes the generic bisection.
-  // Returns false when the generic bisection must decide (mixed group slopes, n3_ref != 0, the ON
-  // level within reach of a falling segment under leak).
+  //     envelope allows an ON crossing takes the same bisection with both levels (two_sided_search).
+  // Returns false when the generic bisection must decide (mixed group slopes, n3_ref != 0).
 
   BSX_HD BSX_INLINE float s_at(int64_t t, float& x) const {
     x = log_ratio(photocurrent(t));
@@ -247,7 +246,7 @@ struct InstantLane {
     }
     // Falling I under leak: OFF only, unless the upper envelope reaches theta_on somewhere.
     const float dl_c = dl_at(c);
-    if (S_lo + dl_c >= theta_on) return false;
+    if (S_lo + dl_c >= theta_on) return two_sided_search(lo, c, out, stack, stride);
     const float L_c = S_c + dl_lo;
     if (!(L_c <= -theta_off)) return true;  // the lower envelope stays above: no crossing
     const int64_t tL = first_true(lo, D_lo, c, L_c, -theta_off, [&](int64_t m, float& v) {
@@ -309,6 +308,64 @@ struct InstantLane {
       e = stack[sp * stride];
     }
     return true;  // tU = c + 1: no crossing (a finite tU always holds one)
+  }
+
+  // Falling I under leak with both levels within reach: the first t of [lo, c] where Delta fires, by a
+  // time-ordered bisection on the exact bounds fl(S(b) + dl(a)) <= Delta <= fl(S(a) + dl(b)) of a
+  // node [a, b] (S does not increase, dl does not decrease). A node whose bounds stay inside both
+  // levels is dropped. On a node of constant dl, Delta does not increase: it fires at a, or first
+  // crosses -theta_off where the monotone search finds it, or not at all; on a node of constant S it
+  // does not decrease, and the same holds with theta_on. Other nodes are split. Delta(lo) is known
+  // not to fire.
+  BSX_HD BSX_INLINE bool two_sided_search(int64_t lo, int64_t c, Hit& out, uint32_t* stack, int stride) {
+    uint32_t a = 0, e = static_cast<uint32_t>(c - lo);
+    int sp = 0;
+    out.found = false;
+    while (true) {
+      float x_a, x_e;
+      const float S_a = s_at(lo + a, x_a), S_e = s_at(lo + e, x_e);
+      const float dl_a = dl_at(lo + a), dl_e = dl_at(lo + e);
+      n_leaves += 2;
+      const bool may = (S_a + dl_e >= theta_on) | (S_e + dl_a <= -theta_off);
+      if (may) {
+        const bool dl_flat = dl_a == dl_e, s_flat = S_a == S_e;
+        if (dl_flat || s_flat) {
+          const float D_a = S_a + dl_a, D_e = S_e + dl_e;
+          int64_t t = -1;
+          bool on = false;
+          if (D_a >= theta_on || D_a <= -theta_off) {
+            t = lo + a;
+            on = D_a >= theta_on;
+          } else if (dl_flat ? D_e <= -theta_off : D_e >= theta_on) {
+            on = !dl_flat;
+            const float level = on ? theta_on : -theta_off;
+            t = first_true(lo + a, D_a, lo + e, D_e, level, [&](int64_t m, float& v) {
+              float xm;
+              v = s_at(m, xm) + dl_at(m);
+              return on ? v >= theta_on : v <= -theta_off;
+            });
+          }
+          if (t >= 0) {
+            out.found = true;
+            out.on = on;
+            out.t = t;
+            out.x = log_ratio(photocurrent(t));
+            return true;
+          }
+        } else {
+          const uint32_t m = a + (e - a) / 2;  // e > a: dl differs at the ends
+          stack[sp * stride] = e;
+          ++sp;
+          e = m;
+          continue;
+        }
+      }
+      if (sp == 0) break;
+      --sp;
+      a = e + 1;
+      e = stack[sp * stride];
+    }
+    return true;
   }
 
   // First integer t in [lo, hi) of the current segment where Delta fires: time-ordered depth-first

@@ -1,6 +1,6 @@
 // The instantaneous lane's searches on the host: on falling segments under leak, with Delta started
-// just above -theta_off and the photocurrent's fall close to the leak's rise (Delta running level
-// with the OFF threshold over long stretches), transition_search must return exactly what the
+// just above -theta_off or just below theta_on and the photocurrent's fall close to the leak's rise
+// (Delta running level with a threshold over long stretches), transition_search must return exactly what the
 // generic time-ordered bisection (the reference's instant_search, src/sensor/pixel.h:1884-1916)
 // returns: found, polarity, crossing time and log value. Prints the counts and the evaluations of
 // both; exit status 1 on any mismatch. With a third argument "fast", checks the whole-segment fast
@@ -90,7 +90,7 @@ int main(int argc, char** argv) {
   std::mt19937_64 rng(seed);
   std::uniform_real_distribution<double> U(0.0, 1.0);
   std::vector<uint32_t> stack(InstantLane<4>::kSearchDepth);
-  long n_found = 0, n_generic = 0, bad = 0, leaves_transition = 0, leaves_generic = 0;
+  long n_found = 0, n_on = 0, n_generic = 0, bad = 0, leaves_transition = 0, leaves_generic = 0;
   for (int it = 0; it < trials; ++it) {
     InstantLane<4> L{};
     L.G = 1 + it % 3;
@@ -120,8 +120,11 @@ int main(int argc, char** argv) {
     float x0;
     L.s3_ref = 0.0f;
     (void)L.s_at(L.ta, x0);
+    // Delta(ta) just above -theta_off, or (odd trials) just below theta_on, where the leak may carry
+    // a slowly falling Delta up to the ON level (both levels within reach)
     const double delta = U(rng) * 2e-6 * (it % 3 == 0 ? 100.0 : 1.0);
-    L.s3_ref = static_cast<float>(static_cast<double>(x0) + L.dl_at(L.ta) + L.theta_off - delta);
+    const double level = it % 2 == 1 ? static_cast<double>(L.theta_on) - delta : -static_cast<double>(L.theta_off) + delta;
+    L.s3_ref = static_cast<float>(static_cast<double>(x0) + L.dl_at(L.ta) - level);
     const int64_t lo = L.ta + static_cast<int64_t>(U(rng) * 0.1 * static_cast<double>(span));
     const int64_t hi = L.tb;
     InstantLane<4>::Hit a{false, false, lo, 0.0f};
@@ -136,12 +139,13 @@ int main(int argc, char** argv) {
       continue;
     }
     n_found += b.found;
+    n_on += b.found && b.on;
     const bool same = a.found == b.found && (!a.found || (a.on == b.on && a.t == b.t && a.x == b.x));
     if (!same && ++bad <= 10)
       std::printf("mismatch at trial %d: transition found %d t %lld on %d, bisection found %d t %lld on %d\n", it,
                   a.found, static_cast<long long>(a.t), a.on, b.found, static_cast<long long>(b.t), b.on);
   }
-  std::printf("trials %d found %ld generic %ld mismatches %ld evaluations transition %ld bisection %ld\n", trials,
-              n_found, n_generic, bad, leaves_transition, leaves_generic);
+  std::printf("trials %d found %ld on %ld generic %ld mismatches %ld evaluations transition %ld bisection %ld\n",
+              trials, n_found, n_on, n_generic, bad, leaves_transition, leaves_generic);
   return bad != 0;
 }

@@ -1,9 +1,9 @@
 """The instantaneous lane's searches compiled for the host (g++, the same fp32 policy as the device
 build: -ffp-contract=off, no fast-math), against the generic time-ordered bisection (the
 reference's instant_search, src/sensor/pixel.h:1884-1916):
-  - on falling segments under leak where Delta runs level with the OFF threshold, the monotone
-    transition search must equal it in every field, with far fewer evaluations (the reason it
-    exists);
+  - on falling segments under leak where Delta runs level with the OFF threshold, or with the ON
+    threshold that the leak may reach, the monotone transition searches must equal it in every
+    field, with far fewer evaluations (the reason they exist);
   - phase A's whole-segment verdicts must agree with it wherever they decide;
   - the walk with phase A's records and block summaries must give exactly the events and final
     state of the segment-by-segment walk on exact searches alone."""
@@ -41,13 +41,14 @@ def walk_check(tmp_path_factory):
 def test_transition_search_equals_bisection(search_check, seed):
     r = subprocess.run([str(search_check), "20000", str(seed)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout
-    m = re.search(r"trials (\d+) found (\d+) generic (\d+) mismatches (\d+) evaluations transition (\d+) "
+    m = re.search(r"trials (\d+) found (\d+) on (\d+) generic (\d+) mismatches (\d+) evaluations transition (\d+) "
                   r"bisection (\d+)", r.stdout)
     assert m, r.stdout
-    trials, found, generic, bad, ev_t, ev_b = map(int, m.groups())
+    trials, found, on, generic, bad, ev_t, ev_b = map(int, m.groups())
     assert bad == 0
     assert generic == 0                  # every case here is decided by the transition search
     assert 0.1 * trials < found < 0.9 * trials   # both outcomes are exercised
+    assert 0.02 * trials < on < found    # ON crossings of the two-sided bisection too
     assert ev_t * 10 < ev_b
 
 
