@@ -100,7 +100,7 @@ struct InstantPrep {
 inline constexpr int kBlockSegments = 16;
 // Phase A: consecutive positions per thread, and the positions of a 256-thread block, to which every
 // camera's segment base is aligned.
-inline constexpr int kPrepPerThread = 4;
+inline constexpr int kPrepPerThread = 4;  // even: the records are stored in pairs
 inline constexpr int64_t kSegmentAlign = 256 * kPrepPerThread;
 struct InstantBlock {
   float x_min, x_max;

@@ -297,6 +297,9 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
   int64_t t_last = 0;
   InstantLane<kMaxG> L;
   int64_t loaded = -1;  // the run whose pixel constants L holds
+  // the thread's records, stored together after the loop (one contiguous 32-byte write per thread)
+  InstantPrep rec[kPrepPerThread];
+  for (int k = 0; k < kPrepPerThread; ++k) rec[k] = InstantPrep{0.0f, 0.0f};
   for (int k = 0; k < kPrepPerThread; ++k) {
     const int64_t q = qt + k;
     if (!(q < total)) {
@@ -340,7 +343,6 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
     const float* v = runs.irr + rr.row_start * kc.groups;
     if (j == 0) {
       if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
-        prep[q] = InstantPrep{0.0f, 0.0f};
         blk_ok = false;
         continue;
       }
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
     for (int g = 0; g < kMaxG; ++g)
       if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
     const InstantPrep r = L.prep_segment();
-    prep[q] = r;
+    rec[k] = r;
     const float theta = vmin(L.theta_on, L.theta_off);
     const bool quiet = L.quiet_segment(r);
     bool up = true, down = true;
@@ -375,6 +377,15 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
       // (a schedule only: the hardware reciprocal instead of a correctly rounded division)
       cost_acc += 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) * __builtin_amdgcn_rcpf(theta);
     }
+  }
+  if (qt + kPrepPerThread <= total) {
+    // 32-byte aligned: qt is a multiple of kPrepPerThread and the records are 8 bytes
+    float4* dst = reinterpret_cast<float4*>(prep + qt);
+    for (int k = 0; k < kPrepPerThread; k += 2)
+      dst[k / 2] = make_float4(rec[k].x_a, rec[k].x_c, rec[k + 1].x_a, rec[k + 1].x_c);
+  } else {
+    for (int k = 0; k < kPrepPerThread; ++k)
+      if (qt + k < total) prep[qt + k] = rec[k];
   }
   wave_segmented_cost_loud(work, first_loud, acc_run, cost_acc, loud_acc);
   if (blocks != nullptr) block_summary(blocks, qt, total, blk_ok, blk_run, b_lo, b_hi, t_last);
