@@ -39,13 +39,14 @@ hipError_t launch_to_bf16(const float* w, uint16_t* wb, int n, hipStream_t strea
 // Forward of n rows (features fp32, row-major [n x 12]) into y [n].
 hipError_t launch_forward(const uint16_t* wb, const float* b_fp32, const float* x, int64_t n, float* y,
                           hipStream_t stream);
-// One training batch of n rows: per-wave partial gradients (waves = ceil(n / 64) rows of kParams
-// values in `partial`) of the mean squared error over `batch` rows (batch >= n: the loss of a larger
-// batch split over launches), and per-wave partial sums of the squared errors in loss_partial.
+// One training batch of n rows: per-group partial gradients (ceil(n / 64) groups of 64 rows, one row
+// of kParams values each in `partial`) of the mean squared error over `batch` rows (batch >= n: the
+// loss of a larger batch split over launches), and per-group partial sums of the squared errors in
+// loss_partial.
 hipError_t launch_gradients(const uint16_t* wb, const float* b_fp32, const float* x, const float* target, int64_t n,
                             int64_t batch, float* partial, float* loss_partial, hipStream_t stream);
-// Sums the partial gradients of `waves` waves (in wave order) into grad, then one Adam step of the fp32
-// masters w (moments m, v) and their bf16 copies wb.
+// Sums the partial gradients of `waves` groups (in group order) into grad, then one Adam step of the
+// fp32 masters w (moments m, v) and their bf16 copies wb.
 hipError_t launch_adam(const float* partial, int waves, float* grad, float* w, float* m, float* v, uint16_t* wb,
                        const Adam& a, hipStream_t stream);
 // MFMA check: C [32 x 32] fp32 = A [32 x 8] . B [8 x 32], A and B bf16 row-major, through the kernels'
