@@ -846,14 +846,32 @@ __global__ void fill_u64_kernel(unsigned long long* dst, size_t n, unsigned long
 
 // Input check of device runs: non-finite samples (out[0]) and clamped samples (out[1]); one lane
 // per sample, coalesced, wave-reduced.
-__global__ void check_samples_kernel(const float* irr, size_t count, float vmax_, unsigned long long* out) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// Counts of the non-finite and of the clamped irradiance values (grid-stride, 16-byte loads when
+// the array is 16-byte aligned; one reduction per thread).
+template <bool kVec>
+__global__ __launch_bounds__(256) void check_samples_kernel(const float* irr, size_t count, float vmax_,
+                                                            unsigned long long* out) {
   uint32_t bad = 0, clamped = 0;
-  if (i < count) {
-    const float v = irr[i];
-    if (!(v - v == 0.0f)) bad = 1;
-    else if (clamp_irr(v, vmax_) != v) clamped = 1;
+  const auto one = [&](float v) {
+    if (!(v - v == 0.0f)) ++bad;
+    else if (clamp_irr(v, vmax_) != v) ++clamped;
+  };
+  const size_t tid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  size_t done = 0;
+  if (kVec) {
+    const float4* p = reinterpret_cast<const float4*>(irr);
+    const size_t n4 = count / 4;
+    for (size_t i = tid; i < n4; i += stride) {
+      const float4 v = p[i];
+      one(v.x);
+      one(v.y);
+      one(v.z);
+      one(v.w);
+    }
+    done = n4 * 4;
   }
+  for (size_t i = done + tid; i < count; i += stride) one(irr[i]);
   wave_add(out + 0, bad);
   wave_add(out + 1, clamped);
 }
@@ -1025,7 +1043,10 @@ hipError_t launch_fill_u64(unsigned long long* dst, size_t n, unsigned long long
 hipError_t launch_check_samples(const float* irr, size_t count, float irr_max, unsigned long long* out,
                                 hipStream_t stream) {
   if (count == 0) return hipSuccess;
-  check_samples_kernel<<<grid_for(count, 256), 256, 0, stream>>>(irr, count, irr_max, out);
+  const bool vec = reinterpret_cast<uintptr_t>(irr) % 16 == 0;
+  const dim3 grid(static_cast<unsigned>(vmin<size_t>((count + 1023) / 1024, 2048)));
+  if (vec) check_samples_kernel<true><<<grid, 256, 0, stream>>>(irr, count, irr_max, out);
+  else check_samples_kernel<false><<<grid, 256, 0, stream>>>(irr, count, irr_max, out);
   return hipGetLastError();
 }
 

@@ -163,6 +163,44 @@ def test_input_refusal_before_state_change():
         m.advance(np.zeros(0, np.uint32), np.zeros((0, 0), np.int64), np.zeros((0, 0, 1), np.float32), -5)
 
 
+@pytest.mark.parametrize("offset", [0, 1])
+def test_device_sample_checks_large_runs(offset):
+    """The device check of the runs' irradiance on arrays of 163,840 values, through 16-byte loads
+    (offset 0) and through 4-byte loads (offset 1: the array starts one float past an aligned
+    address): a NaN deep inside and an infinity in the last value are refused before any state
+    changes, and values above the largest accepted sample are counted as clamps, exactly."""
+    import torch
+
+    import ballasim_amd as bsx
+    W, H, n = 64, 64, 40
+    P = W * H
+    m = bsx.SensorModel("ideal", W, H, 1, device="cuda")
+    m.init(0, np.ones((P, 1), np.float32))
+    dev = "cuda"
+    pixel = torch.arange(P, dtype=torch.int32, device=dev)
+    count = torch.full((P,), n, dtype=torch.int32, device=dev)
+    start = torch.arange(P, dtype=torch.int64, device=dev) * n
+    t = (torch.arange(n, dtype=torch.int64, device=dev) * 1000 + 1000).repeat(P)
+    base = torch.full((P * n + offset,), 5.0, dtype=torch.float32, device=dev)
+    irr = base[offset:]
+    assert (irr.data_ptr() % 16 == 0) == (offset == 0)
+    t_end = int(n * 1000 + 1000)
+    for where, value in ((100_003, float("nan")), (P * n - 1, float("inf"))):
+        irr[where] = value
+        with pytest.raises(ValueError, match="non-finite"):
+            m.advance_device(pixel, start, count, t, irr, t_end)
+        assert m.initialized and m.time_ns == 0
+        irr[where] = 5.0
+    big = m.irr_max * 4.0
+    hot = torch.tensor([0, 7, 99_999, 150_001, P * n - 3, P * n - 1], device=dev)
+    irr[hot] = big
+    c0 = m.stats()["input_clamps"]
+    m.advance_device(pixel, start, count, t, irr, t_end)
+    torch.cuda.synchronize()
+    assert m.stats()["input_clamps"] - c0 == len(hot)
+    assert m.time_ns == t_end
+
+
 def test_event_buffer_overflow_reruns():
     """A forced tiny event capacity: the call re-runs from its input state with a larger buffer and
     gives the same events."""
