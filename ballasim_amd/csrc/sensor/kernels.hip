@@ -437,19 +437,47 @@ __device__ __forceinline__ int camera_of(const CameraDev* cams, int n_cams, uint
   return c;
 }
 
+// Pixels per thread of the order kernels (fewer blocks: fewer global atomics on the 64 counters).
+constexpr int kOrderPerThread = 16;
+
+// For the lanes of a wave with a bucket b >= 0: one LDS atomic per distinct bucket adds the lanes of
+// that bucket to cnt[b]; returns this lane's slot among them (the old count plus its rank among the
+// wave's lanes of the same bucket).
+__device__ __forceinline__ uint32_t wave_bucket_slot(uint32_t* cnt, int b) {
+  const int lane = __lane_id();
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long todo = __ballot(b >= 0);
+  uint32_t slot = 0;
+  while (todo != 0) {
+    const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+    const int bl = __shfl(b, leader);
+    const unsigned long long same = __ballot(b == bl) & todo;
+    uint32_t w = 0;
+    if (lane == leader) w = atomicAdd(&cnt[bl], static_cast<uint32_t>(__popcll(same)));
+    w = __shfl(w, leader);
+    if (b == bl) slot = w + static_cast<uint32_t>(__popcll(same & below));
+    todo &= ~same;
+  }
+  return slot;
+}
+
 __global__ __launch_bounds__(256) void order_count_kernel(const CameraDev* cams, int n_cams, uint32_t n, uint8_t* key,
                                                           uint32_t* hist) {
   __shared__ uint32_t cnt[64];
   if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < n) {
-    const CameraDev& c = cams[camera_of(cams, n_cams, g)];
-    const uint32_t p = g - c.base;
-    const int32_t r = c.runs.row_of != nullptr ? c.runs.row_of[p] : -1;
-    const int b = r >= 0 && c.work != nullptr ? cost_bucket(c.work[r]) : 0;
-    key[g] = static_cast<uint8_t>(b);
-    atomicAdd(&cnt[b], 1u);
+  const uint32_t base = blockIdx.x * (256u * kOrderPerThread);
+  for (int k = 0; k < kOrderPerThread; ++k) {
+    const uint32_t g = base + static_cast<uint32_t>(k) * 256u + threadIdx.x;
+    int b = -1;
+    if (g < n) {
+      const CameraDev& c = cams[camera_of(cams, n_cams, g)];
+      const uint32_t p = g - c.base;
+      const int32_t r = c.runs.row_of != nullptr ? c.runs.row_of[p] : -1;
+      b = r >= 0 && c.work != nullptr ? cost_bucket(c.work[r]) : 0;
+      key[g] = static_cast<uint8_t>(b);
+    }
+    (void)wave_bucket_slot(cnt, b);
   }
   __syncthreads();
   if (threadIdx.x < 64 && cnt[threadIdx.x] != 0) atomicAdd(hist + threadIdx.x, cnt[threadIdx.x]);
@@ -468,13 +496,21 @@ __global__ __launch_bounds__(256) void order_scatter_kernel(const uint8_t* key, 
   __shared__ uint32_t cnt[64], base[64];
   if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = p < n ? key[p] : 0;
-  const uint32_t local = p < n ? atomicAdd(&cnt[b], 1u) : 0u;
+  const uint32_t first = blockIdx.x * (256u * kOrderPerThread);
+  int bk[kOrderPerThread];
+  uint32_t slot[kOrderPerThread];
+  for (int k = 0; k < kOrderPerThread; ++k) {
+    const uint32_t p = first + static_cast<uint32_t>(k) * 256u + threadIdx.x;
+    bk[k] = p < n ? key[p] : -1;
+    slot[k] = wave_bucket_slot(cnt, bk[k]);
+  }
   __syncthreads();
   if (threadIdx.x < 64 && cnt[threadIdx.x] != 0) base[threadIdx.x] = atomicAdd(cursor + threadIdx.x, cnt[threadIdx.x]);
   __syncthreads();
-  if (p < n) perm[base[b] + local] = p;
+  for (int k = 0; k < kOrderPerThread; ++k) {
+    const uint32_t p = first + static_cast<uint32_t>(k) * 256u + threadIdx.x;
+    if (bk[k] >= 0) perm[base[bk[k]] + slot[k]] = p;
+  }
 }
 
 constexpr int kInstantBlock = 256;
@@ -967,9 +1003,10 @@ hipError_t launch_chain_prepare(const ChainArgs& a, hipStream_t stream) {
     else chain_prep_kernel<8><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
   }
   if (a.n_total > 0) {
-    order_count_kernel<<<grid_for(a.n_total, 256), 256, 0, stream>>>(a.cams, a.n_cams, a.n_total, a.key, a.hist);
+    const dim3 order_grid = grid_for(a.n_total, 256 * kOrderPerThread);
+    order_count_kernel<<<order_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.n_total, a.key, a.hist);
     order_offsets_kernel<<<1, 64, 0, stream>>>(a.hist, a.hist + 64);
-    order_scatter_kernel<<<grid_for(a.n_total, 256), 256, 0, stream>>>(a.key, a.n_total, a.hist + 64, a.perm);
+    order_scatter_kernel<<<order_grid, 256, 0, stream>>>(a.key, a.n_total, a.hist + 64, a.perm);
   }
   return hipGetLastError();
 }
