@@ -296,7 +296,8 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
   float b_lo = inf_<float>(), b_hi = -inf_<float>();
   int64_t t_last = 0;
   InstantLane<kMaxG> L;
-  int64_t loaded = -1;  // the run whose pixel constants L holds
+  int64_t loaded = -1;    // the run whose pixel constants L holds
+  int64_t prev_end = -1;  // the position whose segment's end sample L.tb, L.vb hold (-1: none)
   // the thread's records, stored together after the loop (one contiguous 32-byte write per thread)
   InstantPrep rec[kPrepPerThread];
   for (int k = 0; k < kPrepPerThread; ++k) rec[k] = InstantPrep{0.0f, 0.0f};
@@ -344,11 +345,16 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
     if (j == 0) {
       if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
         blk_ok = false;
+        prev_end = -1;
         continue;
       }
       L.ta = t_state;
       for (int g = 0; g < kMaxG; ++g)
         if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
+    } else if (prev_end == q - 1) {  // the previous position's segment ends where this one starts
+      L.ta = L.tb;
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < kc.groups) L.va[g] = L.vb[g];
     } else {
       L.ta = t[j - 1];
       for (int g = 0; g < kMaxG; ++g)
@@ -357,6 +363,7 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
     L.tb = t[j];
     for (int g = 0; g < kMaxG; ++g)
       if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
+    prev_end = q;
     const InstantPrep r = L.prep_segment();
     rec[k] = r;
     const float theta = vmin(L.theta_on, L.theta_off);
