@@ -43,7 +43,7 @@ ADVANCE_CHAIN = {
 # Measured HBM traffic of the instantaneous chain on this workload, bytes per sample: rocprofv3
 # FETCH_SIZE x 2 + WRITE_SIZE summed over the chain's kernels, averaged over the timed launches (the
 # factor 2 and 1 from a 1 GiB device copy read by the same counters: profiles/advance_chain_1280x720.md).
-TRAFFIC_BYTES_PER_SAMPLE = 58.4
+TRAFFIC_BYTES_PER_SAMPLE = 59.3
 
 
 def sensor_params(bsx, photoreceptor):
