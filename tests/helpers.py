@@ -40,9 +40,17 @@ def events_from_dict(ev, width):
 
 
 def first_difference(a, b):
+    """The first index where two event arrays of the same dtype differ in any field (value
+    comparison per field, as the elements' own comparison), with the two elements; (n, None, None)
+    when the common prefix agrees."""
     n = min(len(a), len(b))
-    for i in range(n):
-        if a[i] != b[i]:
+    if n:
+        ne = np.zeros(n, bool)
+        for f in a.dtype.names:
+            ne |= a[f][:n] != b[f][:n]
+        idx = np.flatnonzero(ne)
+        if len(idx):
+            i = int(idx[0])
             return i, a[i], b[i]
     return n, None, None
 
