@@ -419,7 +419,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py runs on one GPU: the sensor path has no multi-GPU form here "
-                                   "(pixels shard by tile or camera; see DESIGN.md)", "n_gpus": args.gpus}))
+                                   "(pixels shard by tile or camera)", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.steps is None:
         args.steps = 8 if args.workload == "rig" else 58

@@ -5,7 +5,7 @@
  * No reference counterpart: the reference's sampler has only the plain error-tolerance controller
  * (src/engine/sampling.h:90-122). The proposer proposes each pixel's next sample interval from its
  * recent samples (12 features, see bsx_scene_sample_learned in bsx_scene.h); the plain controller is
- * its fallback and the judge of its error.
+ * its fallback and the measure of its error.
  *
  * Parameter vector (BSX_MLP_PARAMS fp32 values): W1 [12 x 64] (input-major), b1 [64], W2 [64 x 64]
  * (row = layer-1 unit), b2 [64], w3 [64], b3. Forward: z1 = bf16(x) bf16(W1) + b1, h1 = bf16(silu(z1)),

@@ -1,7 +1,7 @@
 """The learned controller end to end on the GPU: training examples from dense traces of seeded
 config-1-style scenes, training by the library's kernels, and the learned sampler on a held-out seed
 against the plain error-tolerance controller and against the same sampler with the untrained network.
-The judge of event error is a dense-sampled reference (10 us): count mismatch and median timing error
+Event error is measured against a dense-sampled reference (10 us): count mismatch and median timing error
 (ballasim_amd.learned.event_error). The learned sampler reads the sensor's reference state at each window
 start (its band of no event start) and samples as the plain controller near the band edges.
 
