@@ -26,6 +26,9 @@ CASCADE_CASES = {
     "prophesee_zero_refractory": ("prophesee_gen3", dict(noise=False, refractory_s=0.0, eps_ci=0.003,
                                                         leak_rate_hz=0.5, theta_on=0.12, theta_off=0.14)),
     "samsung_fast_light_groups2": ("samsung_gen3", dict(noise=False, illuminance_scale=40.0)),
+    # leak-type hot pixels through the cascade's comparator (HOT labels and the hot-pixel flag)
+    "prophesee_hot": ("prophesee_gen3", dict(noise=False, hot_fraction=0.25, hot_rate_law=1, hot_rate_median_hz=300.0,
+                                             hot_rate_min_hz=20.0, hot_rate_max_hz=2000.0)),
 }
 
 
@@ -51,7 +54,7 @@ def test_cascade_matches_oracle(case):
     t = 3 * 2 ** 23 - 12345
     gpu.init(t, irr0)
     ref.init(t, irr0)
-    total = 0
+    total = n_hot = 0
     for call in range(3):
         t1 = t + int(rng.integers(1, 2 * 2 ** 23))
         ids, ts, ir, offs = random_runs(rng, W * H, t, t1, G=G, log_sigma=1.8)
@@ -62,8 +65,11 @@ def test_cascade_matches_oracle(case):
         assert len(got) == len(want) and i == len(got), f"call {call}: first difference at {i}: gpu {a} oracle {b}"
         assert np.array_equal(ev["t_reported_ns"], rep)
         total += len(got)
+        n_hot += int(np.count_nonzero(want["flags"] & (1 << 5)))
         t = t1
     assert total > 200
+    if "hot" in case:
+        assert n_hot > 50
     st = np.frombuffer(gpu.save_state()[56:], O.REF_STATE_DTYPE)
     os_ = ref.states()
     for f in ("z", "x1", "s2", "s3", "tv", "i_scale", "s3_ref", "dr_ref", "t_ref", "t_search", "blind_until",
