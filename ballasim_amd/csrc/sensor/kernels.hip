@@ -887,34 +887,26 @@ __global__ void fill_u64_kernel(unsigned long long* dst, size_t n, unsigned long
   if (i < n) dst[i] = v;
 }
 
-// Input check of device runs: non-finite samples (out[0]) and clamped samples (out[1]); one lane
-// per sample, coalesced, wave-reduced.
-// Counts of the non-finite and of the clamped irradiance values (grid-stride, 16-byte loads when
-// the array is 16-byte aligned; one reduction per thread).
-template <bool kVec>
-__global__ __launch_bounds__(256) void check_samples_kernel(const float* irr, size_t count, float vmax_,
-                                                            unsigned long long* out) {
+// Input check of device runs, as the reference's check_inputs_kernel (sensor_cuda.cu:415-440): one
+// wave per run over the c*G values that run references, counting the non-finite values (out[0]) and
+// the values the clamp changes (out[1]). Samples no run references are not read, and a run outside the
+// sample array is skipped (scatter_rows_kernel flags it).
+__global__ __launch_bounds__(256) void check_runs_kernel(const float* irr, const int64_t* row_start,
+                                                         const int32_t* row_count, size_t n_runs, size_t n_samples,
+                                                         int groups, float vmax_, unsigned long long* out) {
+  const size_t run = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  if (run >= n_runs) return;  // whole waves: the run is uniform across the wave
+  const int64_t s = row_start[run];
+  const int32_t c = row_count[run];
+  if (s < 0 || c < 0 || static_cast<uint64_t>(s) + static_cast<uint64_t>(c) > n_samples) return;
+  const float* v = irr + static_cast<size_t>(s) * static_cast<size_t>(groups);
+  const size_t n = static_cast<size_t>(c) * static_cast<size_t>(groups);
   uint32_t bad = 0, clamped = 0;
-  const auto one = [&](float v) {
-    if (!(v - v == 0.0f)) ++bad;
-    else if (clamp_irr(v, vmax_) != v) ++clamped;
-  };
-  const size_t tid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-  size_t done = 0;
-  if (kVec) {
-    const float4* p = reinterpret_cast<const float4*>(irr);
-    const size_t n4 = count / 4;
-    for (size_t i = tid; i < n4; i += stride) {
-      const float4 v = p[i];
-      one(v.x);
-      one(v.y);
-      one(v.z);
-      one(v.w);
-    }
-    done = n4 * 4;
+  for (size_t j = __lane_id(); j < n; j += 64) {
+    const float x = v[j];
+    if (!(x - x == 0.0f)) ++bad;
+    else if (clamp_irr(x, vmax_) != x) ++clamped;
   }
-  for (size_t i = done + tid; i < count; i += stride) one(irr[i]);
   wave_add(out + 0, bad);
   wave_add(out + 1, clamped);
 }
@@ -1084,13 +1076,12 @@ hipError_t launch_fill_u64(unsigned long long* dst, size_t n, unsigned long long
   return hipGetLastError();
 }
 
-hipError_t launch_check_samples(const float* irr, size_t count, float irr_max, unsigned long long* out,
-                                hipStream_t stream) {
-  if (count == 0) return hipSuccess;
-  const bool vec = reinterpret_cast<uintptr_t>(irr) % 16 == 0;
-  const dim3 grid(static_cast<unsigned>(vmin<size_t>((count + 1023) / 1024, 2048)));
-  if (vec) check_samples_kernel<true><<<grid, 256, 0, stream>>>(irr, count, irr_max, out);
-  else check_samples_kernel<false><<<grid, 256, 0, stream>>>(irr, count, irr_max, out);
+hipError_t launch_check_runs(const float* irr, const int64_t* row_start, const int32_t* row_count, size_t n_runs,
+                             size_t n_samples, int groups, float irr_max, unsigned long long* out,
+                             hipStream_t stream) {
+  if (n_runs == 0) return hipSuccess;
+  check_runs_kernel<<<grid_for(n_runs * 64, 256), 256, 0, stream>>>(irr, row_start, row_count, n_runs, n_samples,
+                                                                    groups, irr_max, out);
   return hipGetLastError();
 }
 

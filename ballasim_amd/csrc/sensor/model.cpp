@@ -466,7 +466,9 @@ void SensorModel::enqueue_device_checks(const bsx_device_runs& runs, int64_t t_e
             "scatter rows");
   unsigned long long* chk = reinterpret_cast<unsigned long long*>(d_flags_.as<char>() + 16);
   hip_check(launch_fill_u64(chk, 2, 0, stream), "fill");
-  hip_check(launch_check_samples(runs.irr, runs.n_samples * G_, irr_max_, chk, stream), "check samples");
+  hip_check(launch_check_runs(runs.irr, runs.row_start, runs.row_count, runs.n, runs.n_samples, G_, irr_max_, chk,
+                            stream),
+            "check samples");
   hip_check(hipMemcpyAsync(check_flags_, d_flags_.get(), 16, hipMemcpyDeviceToHost, stream), "read flags");
   hip_check(hipMemcpyAsync(check_samples_, chk, 16, hipMemcpyDeviceToHost, stream), "read checks");
   // The scan reads row_count of runs the scatter may have refused; its result is used only if every

@@ -192,17 +192,11 @@ class SensorModel:
         return self._events()
 
     def advance_device(self, pixel, row_start, row_count, t, irr, t_end_ns, stream=0):
-        """advance_device on runs already in device memory: torch tensors (uint32/int32 pixel, int64
-        row_start, int32 row_count, int64 t, float32 irr) or raw addresses. Returns (data_ptr, count)
-        of the unordered raw events left in device memory."""
-        def addr(x):
-            return x.data_ptr() if hasattr(x, "data_ptr") else int(x)
-
-        def numel(x):
-            return x.numel() if hasattr(x, "numel") else None
-
-        runs = _abi.DeviceRuns(addr(pixel), addr(row_start), addr(row_count), numel(pixel), addr(t), addr(irr),
-                               numel(t))
+        """advance_device on runs already in device memory, given as contiguous torch CUDA tensors:
+        pixel uint32/int32 (n,), row_start int64 (n,), row_count int32 (n,), t int64 (S,), irr
+        float32 with S * groups values. Returns (data_ptr, count) of the unordered raw events left in
+        device memory. Raw device addresses go through advance_device_runs with a DeviceRuns struct."""
+        runs = _checked_device_runs(pixel, row_start, row_count, t, irr, self._g, "SensorModel.advance_device")
         out = _abi.DeviceEvents()
         st = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
         _abi.check(self._lib.bsx_sensor_advance_device(self._h, C.byref(runs), int(t_end_ns), C.c_void_p(st),
@@ -295,20 +289,38 @@ class SensorModel:
     device = property(lambda self: "cuda")
 
 
-def _device_runs(r):
-    """A bsx_device_runs from a DeviceRuns struct or a (pixel, row_start, row_count, t, irr) tuple of
-    device tensors / raw addresses."""
+def _checked_device_runs(pixel, row_start, row_count, t, irr, groups, where):
+    """A bsx_device_runs over torch CUDA tensors, after the checks the C side cannot make: device,
+    contiguity, dtypes and lengths. A host tensor, a strided view, a wrong dtype or a short irr would
+    otherwise be read as raw device memory."""
+    import torch
+
+    named = (("pixel", pixel, (torch.uint32, torch.int32)), ("row_start", row_start, (torch.int64,)),
+             ("row_count", row_count, (torch.int32,)), ("t", t, (torch.int64,)), ("irr", irr, (torch.float32,)))
+    for name, x, dtypes in named:
+        if not isinstance(x, torch.Tensor):
+            raise ValueError(f"{where}: {name} must be a torch tensor in device memory")
+        if not x.is_cuda:
+            raise ValueError(f"{where}: {name} must be on the GPU, not {x.device}")
+        if not x.is_contiguous():
+            raise ValueError(f"{where}: {name} must be contiguous")
+        if x.dtype not in dtypes:
+            raise ValueError(f"{where}: {name} must have dtype {' or '.join(str(d) for d in dtypes)}, not {x.dtype}")
+    n = pixel.numel()
+    if row_start.numel() != n or row_count.numel() != n:
+        raise ValueError(f"{where}: pixel, row_start and row_count must have the same length")
+    if irr.numel() != t.numel() * groups:
+        raise ValueError(f"{where}: irr must hold t.numel() * groups = {t.numel() * groups} values, not {irr.numel()}")
+    return _abi.DeviceRuns(pixel.data_ptr(), row_start.data_ptr(), row_count.data_ptr(), n, t.data_ptr(),
+                           irr.data_ptr(), t.numel())
+
+
+def _device_runs(r, groups, where):
+    """A bsx_device_runs from a DeviceRuns struct (raw device addresses, taken as given) or a
+    (pixel, row_start, row_count, t, irr) tuple of torch CUDA tensors (checked)."""
     if isinstance(r, _abi.DeviceRuns):
         return r
-    pixel, row_start, row_count, t, irr = r
-
-    def addr(x):
-        return x.data_ptr() if hasattr(x, "data_ptr") else int(x)
-
-    def numel(x):
-        return x.numel() if hasattr(x, "numel") else None
-
-    return _abi.DeviceRuns(addr(pixel), addr(row_start), addr(row_count), numel(pixel), addr(t), addr(irr), numel(t))
+    return _checked_device_runs(*r, groups, where)
 
 
 class Rig:
@@ -348,7 +360,7 @@ class Rig:
             raise ValueError(f"Rig: need runs for each of the {len(self._cams)} cameras")
         arr = (_abi.DeviceRuns * len(runs))()
         for i, r in enumerate(runs):
-            arr[i] = _device_runs(r)
+            arr[i] = _device_runs(r, self._cams[i].groups, f"Rig.advance_device (camera {i})")
         return arr
 
     def advance_device(self, runs, t_end_ns, stream=0):

@@ -392,6 +392,13 @@ struct orc_model {
   orc_event* ev;
   size_t n_ev, cap_ev;
   orc_stats stats;
+  /* the noise tables (cascade with noise on): fp32 rows, node count, stride, grid origin and
+   * inverse spacing as the fp32 kernel reads them, and the constants the walk uses */
+  int noise;
+  float* tab32;
+  int tab_nodes, tab_stride;
+  float tab_ln_min, tab_inv_dln;
+  float a3f, margin_z;
 };
 
 static void set_err(char* err, size_t n, const char* msg) {
@@ -487,16 +494,21 @@ void orc_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t
   draw_maps(&m);
 }
 
+static void setup_noise(orc_model* m); /* orc_noise.inc */
+
 orc_model* orc_create(const bsx_params* p, uint32_t width, uint32_t height, uint64_t seed, uint32_t camera,
                       int32_t groups, int32_t num_threads, char* err, size_t err_n) {
   if (width == 0 || height == 0 || groups < 1 || groups > 8 || camera > 255) {
     set_err(err, err_n, "orc_create: bad size, groups or camera");
     return NULL;
   }
-  if (p->veiling_glare != 0 || p->tau_hp != 0 ||
-      (p->noise && p->photoreceptor == BSX_PHOTORECEPTOR_CASCADE) ||
-      (p->shot_rate_hz > 0 && p->photoreceptor == BSX_PHOTORECEPTOR_CASCADE)) {
-    set_err(err, err_n, "orc_create: glare, high-pass filter and cascade noise (in-path or shot) are not restated");
+  const int noise_on = p->noise && p->photoreceptor == BSX_PHOTORECEPTOR_CASCADE;
+  if (p->veiling_glare != 0 || p->tau_hp != 0 || (p->shot_rate_hz > 0 && p->photoreceptor == BSX_PHOTORECEPTOR_CASCADE) ||
+      (noise_on && (p->noise_diffusion != 0 || p->initial_state != 0 || p->noise_leaf_level < 0 ||
+                    p->noise_leaf_level > 10 || (p->hot_fraction != 0 && p->hot_leak_fraction < 1.0)))) {
+    set_err(err, err_n,
+            "orc_create: glare, the high-pass filter, the shot process on the cascade, transient diffusion, the "
+            "stationary start and threshold-type hot pixels are not restated");
     return NULL;
   }
   orc_model* m = (orc_model*)calloc(1, sizeof(orc_model));
@@ -529,6 +541,8 @@ orc_model* orc_create(const bsx_params* p, uint32_t width, uint32_t height, uint
     m->m32[i].hot = m->hot[i];
   }
   m->irr_max = (float)(1e9 / (p->illuminance_scale * 1.0));
+  m->noise = noise_on;
+  if (noise_on) setup_noise(m);
   m->refractory_ns = llround(p->refractory_s * 1e9);
   m->q_ns = llround(p->readout_quantum_s * 1e9);
   m->delay_ns = llround(p->readout_delay_s * 1e9);
@@ -542,6 +556,7 @@ orc_model* orc_create(const bsx_params* p, uint32_t width, uint32_t height, uint
 
 void orc_destroy(orc_model* m) {
   if (m == NULL) return;
+  free(m->tab32);
   free(m->theta_on);
   free(m->theta_off);
   free(m->leak);
@@ -591,6 +606,12 @@ typedef struct {
   uint16_t flags;
   int64_t shot_next;
   uint32_t shot_index;
+  /* the in-path noise (cascade, noise on): lattice nodes, the cell's rate, the cell and its level-0
+   * Hermite and margin; seg is the sample-segment cursor */
+  float nk[3], nk1[3], ln_a1, hc0[4], margin0;
+  int64_t cell;
+  int have_k1;
+  int seg;
   evvec* out;
   orc_stats* stats;
 } pix;
@@ -848,6 +869,11 @@ static void advance_instant(pix* k, int64_t t_to) {
   }
 }
 
+/* The cascade without noise, then its in-path noise. */
+static void advance_cascade(pix* k, int64_t t_end);
+#include "orc_cascade.inc"
+#include "orc_noise.inc"
+
 static void load(pix* k, const orc_model* m, size_t p) {
   const orc_state* s = &m->st[p];
   k->z = (float)s->z;
@@ -866,6 +892,20 @@ static void load(pix* k, const orc_model* m, size_t p) {
   k->shot_next = s->shot_next;
   k->shot_index = s->shot_index;
   for (int g = 0; g < m->G; ++g) k->held[g] = s->held[g];
+  for (int i = 0; i < 3; ++i) {
+    k->nk[i] = (float)s->nk[i];
+    k->nk1[i] = (float)s->nk1[i];
+  }
+  k->ln_a1 = (float)s->ln_a1;
+  k->cell = s->cell;
+  k->have_k1 = s->have_k1;
+  if (m->noise && k->have_k1) {
+    /* a call resumed inside a cell: its level-0 hull as draw_lattice left it */
+    float m0[23];
+    if (m->p.noise_leaf_level > 0)
+      noise_lookup(m, noise_pos(m, k->ln_a1), noise_margin_off(m->p.noise_leaf_level, 0), 23, m0);
+    level0(k, m0);
+  }
 }
 
 static void store(const pix* k, orc_model* m, size_t p) {
@@ -886,6 +926,13 @@ static void store(const pix* k, orc_model* m, size_t p) {
   s->shot_next = k->shot_next;
   s->shot_index = k->shot_index;
   for (int g = 0; g < m->G; ++g) s->held[g] = k->held[g];
+  for (int i = 0; i < 3; ++i) {
+    s->nk[i] = k->nk[i];
+    s->nk1[i] = k->nk1[i];
+  }
+  s->ln_a1 = k->ln_a1;
+  s->cell = k->cell;
+  s->have_k1 = (uint8_t)k->have_k1;
 }
 
 static void pix_setup(pix* k, const orc_model* m, size_t p) {
@@ -928,7 +975,8 @@ int orc_init(orc_model* m, int64_t t0, const float* irr0) {
     if (m->p.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS) {
       set_reference(&k, t0, log_ratio(&k, photocurrent(&k, 0, t0)), 0.0f, phi0);
     } else {
-      set_reference(&k, t0, k.s3, 0.0f, phi0);
+      const float n3_0 = m->noise ? init_noise(&k, t0, mean_I) : 0.0f;
+      set_reference(&k, t0, k.s3, n3_0, phi0);
     }
     k.blind_until = INT64_MIN;
     k.t_search = t0 + 1;
@@ -943,9 +991,6 @@ int orc_init(orc_model* m, int64_t t0, const float* irr0) {
   return 0;
 }
 
-/* The cascade without noise. */
-static void advance_cascade(pix* k, int64_t t_end);
-#include "orc_cascade.inc"
 
 typedef struct {
   orc_model* m;
@@ -988,6 +1033,7 @@ static void* worker(void* arg) {
       k.out = &j->outs[a->w];
       k.stats = &j->stats[a->w];
       if (m->p.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS) advance_instant(&k, j->t_end);
+      else if (m->noise) advance_cascade_noise(&k, j->t_end);
       else advance_cascade(&k, j->t_end);
       if (k.n > 0)
         for (int g = 0; g < m->G; ++g) k.held[g] = s_value(&k, n_aug(&k) - 1, g);
@@ -1089,6 +1135,8 @@ int orc_advance_runs(orc_model* m, const uint32_t* ids, size_t n, const int64_t*
     m->stats.fallbacks += s->fallbacks;
     m->stats.quarantines += s->quarantines;
     m->stats.pieces += s->pieces;
+    m->stats.cells += s->cells;
+    m->stats.refinements += s->refinements;
   }
   qsort(m->ev, m->n_ev, sizeof(orc_event), cmp_ev);
   free(j.outs);

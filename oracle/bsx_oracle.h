@@ -17,6 +17,8 @@
  *   - fire and labels                      src/sensor/pixel.h:288-307, 1531-1588
  *   - the CPU backend's pixel blocks       src/sensor/sensor_model.cpp:951-994
  *   - finish_events and the readout        src/sensor/sensor_model.cpp:996-1029, readout.h:22-26
+ *   - the in-path noise (frozen diffusion) src/sensor/noise.h, noise_tables.{h,cpp}, pixel.h:394-745,
+ *                                          821-845, 1270-1453, 1599-1652 (orc_noise.inc)
  *   - the rate-specified shot noise        (this library's extension; no reference counterpart)
  * How each part is pinned is stated in oracle/README.md.
  */
@@ -51,11 +53,14 @@ typedef struct orc_state {
   float held[8];
   int64_t shot_next;   /* the shot-noise process's next arrival (INT64_MAX: off) */
   uint32_t shot_index; /* its index */
+  double nk[3], nk1[3], ln_a1; /* the in-path noise: lattice nodes and the cell's rate */
+  int64_t cell;
+  uint8_t have_k1;
 } orc_state;
 
 typedef struct orc_stats {
   uint64_t substeps, substep_attempts, settled_substeps, leaf_intervals, fallbacks, input_clamps, quarantines,
-      pieces;
+      pieces, cells, refinements;
 } orc_stats;
 
 /* Returns null and fills err (if given) on invalid parameters. num_threads <= 0: all cores. */
@@ -79,6 +84,12 @@ void orc_maps(const orc_model* m, double* theta_on, double* theta_off, double* l
 void orc_states(const orc_model* m, orc_state* out);
 void orc_get_stats(const orc_model* m, orc_stats* out);
 int64_t orc_time(const orc_model* m);
+/* The noise tables of p (frozen diffusion): n_nodes rows of stride values, fp64 and fp32 (either
+ * output may be null); 0 ok, 1 bad leaf level. */
+int orc_noise_tables(const bsx_params* p, double* out64, float* out32, int32_t* n_nodes, int32_t* stride);
+/* normals3 of the noise purposes, fp64 (all 32 bits of each word) and fp32 (either may be null). */
+void orc_normals3(uint64_t seed, uint32_t purpose, uint32_t camera, uint32_t pixel, uint64_t index, double* out64,
+                  float* out32);
 
 /* The maths, exported for unit tests. */
 void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

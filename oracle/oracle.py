@@ -21,7 +21,9 @@ EVENT_DTYPE = np.dtype([("pixel", "<u4"), ("polarity", "u1"), ("label", "u1"), (
 STATE_DTYPE = np.dtype([("z", "<f8"), ("x1", "<f8"), ("s2", "<f8"), ("s3", "<f8"), ("i_scale", "<f8"),
                         ("tv", "<f8"), ("s3_ref", "<f8"), ("n3_ref", "<f8"), ("dr_ref", "<f8"),
                         ("t_ref", "<i8"), ("blind_until", "<i8"), ("t_search", "<i8"), ("flags", "<u2"),
-                        ("held", "<f4", (8,)), ("shot_next", "<i8"), ("shot_index", "<u4")], align=True)
+                        ("held", "<f4", (8,)), ("shot_next", "<i8"), ("shot_index", "<u4"),
+                        ("nk", "<f8", (3,)), ("nk1", "<f8", (3,)), ("ln_a1", "<f8"), ("cell", "<i8"),
+                        ("have_k1", "u1")], align=True)
 # The reference's PixelState (200 B; src/sensor/pixel.h:105-121).
 REF_STATE_DTYPE = np.dtype([("z", "<f8"), ("x1", "<f8"), ("s2", "<f8"), ("s3", "<f8"), ("i_scale", "<f8"),
                             ("tv", "<f8"), ("nk", "<f8", (3,)), ("nk1", "<f8", (3,)), ("ln_a1", "<f8"),
@@ -42,7 +44,7 @@ def _ptr(a):
 
 
 STAT_NAMES_ORACLE = ("substeps", "substep_attempts", "settled_substeps", "leaf_intervals", "fallbacks",
-                     "input_clamps", "quarantines", "pieces")
+                     "input_clamps", "quarantines", "pieces", "cells", "refinements")
 STAT_NAMES_REF = ("substeps", "substep_attempts", "settled_substeps", "cells", "refinements", "leaf_intervals",
                   "fallbacks", "input_clamps", "quarantines", "pieces")
 
@@ -75,6 +77,8 @@ def oracle_lib():
         lib.orc_time.restype = C.c_int64
         lib.orc_time.argtypes = [P]
         lib.orc_philox.argtypes = [P, P, P]
+        lib.orc_noise_tables.argtypes = [P, P, P, P, P]
+        lib.orc_normals3.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, P, P]
         for n in ("orc_expf", "orc_logf", "orc_expm1f", "orc_erfcxf", "orc_dawsonf"):
             getattr(lib, n).restype = C.c_float
             getattr(lib, n).argtypes = [C.c_float]
@@ -136,8 +140,34 @@ def ref_lib():
                                     C.c_int64, P]
         lib.ref_label_s1.argtypes = [P, C.c_size_t, P]
         lib.ref_stats.argtypes = [P, P]
+        lib.ref_noise_tables.argtypes = [P, P, P, P, P]
+        lib.ref_normals3.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, P, P]
         _ref = lib
     return _ref
+
+
+def noise_tables(params, lib="oracle"):
+    """The noise tables of params (frozen diffusion) from the oracle or the compiled reference:
+    (fp64 rows, fp32 rows), each (n_nodes, stride)."""
+    L = oracle_lib() if lib == "oracle" else ref_lib()
+    fn = L.orc_noise_tables if lib == "oracle" else L.ref_noise_tables
+    n, s = C.c_int32(), C.c_int32()
+    if fn(C.byref(params), None, None, C.byref(n), C.byref(s)) != 0:
+        raise ValueError("noise tables: bad leaf level")
+    d64 = np.zeros((n.value, s.value), np.float64)
+    d32 = np.zeros((n.value, s.value), np.float32)
+    fn(C.byref(params), _ptr(d64), _ptr(d32), C.byref(n), C.byref(s))
+    return d64, d32
+
+
+def normals3(seed, purpose, camera, pixel, index, lib="oracle"):
+    """normals3 of one key, (fp64 triple, fp32 triple), from the oracle or the compiled reference."""
+    L = oracle_lib() if lib == "oracle" else ref_lib()
+    fn = L.orc_normals3 if lib == "oracle" else L.ref_normals3
+    d = np.zeros(3, np.float64)
+    f = np.zeros(3, np.float32)
+    fn(seed, purpose, camera, pixel, index, _ptr(d), _ptr(f))
+    return d, f
 
 
 class _Base:
@@ -212,7 +242,7 @@ class OracleModel(_Base):
         return out
 
     def stats(self):
-        v = np.zeros(8, np.uint64)
+        v = np.zeros(len(STAT_NAMES_ORACLE), np.uint64)
         self.lib.orc_get_stats(self.h, _ptr(v))
         return dict(zip(STAT_NAMES_ORACLE, (int(x) for x in v)))
 

@@ -10,6 +10,8 @@
 #include <vector>
 
 #include "../include/bsx/bsx_sensor.h"
+#include "sensor/noise.h"
+#include "sensor/noise_tables.h"
 #include "sensor/output_order.h"
 #include "sensor/params.h"
 #include "sensor/readout.h"
@@ -190,6 +192,37 @@ int ref_advance_runs(void* h, const uint32_t* ids, size_t n, const int64_t* t, c
 }
 
 size_t ref_event_count(void* h) { return static_cast<RefModel*>(h)->ev.size(); }
+
+// The reference's noise tables for b's noise model (frozen diffusion): n_nodes rows of stride values, fp64 and
+// fp32 (either output may be null).
+int ref_noise_tables(const bsx_params* b, double* out64, float* out32, int32_t* n_nodes, int32_t* stride) {
+  try {
+    NoiseModel nm{};
+    nm.kq = b->k * 1.602176634e-19;
+    nm.a2 = 1.0 / b->tau_o;
+    nm.a3 = 1.0 / b->tau_sf;
+    nm.sigma_o = b->sigma_o;
+    nm.levels = b->noise_leaf_level;
+    nm.margin_z = b->noise_margin_z;
+    nm.split = false;
+    const NoiseTables t(nm);
+    *n_nodes = t.n_nodes();
+    *stride = t.layout().stride();
+    if (out64 != nullptr) std::memcpy(out64, t.data64().data(), t.data64().size() * sizeof(double));
+    if (out32 != nullptr) std::memcpy(out32, t.data32().data(), t.data32().size() * sizeof(float));
+    return 0;
+  } catch (const std::exception&) {
+    return 1;
+  }
+}
+
+// The reference's keyed normals (noise.h normals3) in fp64 and fp32 (either output may be null).
+void ref_normals3(uint64_t seed, uint32_t purpose, uint32_t camera, uint32_t pixel, uint64_t index, double* out64,
+                  float* out32) {
+  const auto pu = static_cast<balla::rng::Purpose>(purpose);
+  if (out64 != nullptr) normals3<double>(seed, pu, camera, pixel, index, out64);
+  if (out32 != nullptr) normals3<float>(seed, pu, camera, pixel, index, out32);
+}
 
 // The sorted raw events: the reference's RawEvent has the layout of bsx_raw_event.
 void ref_events(void* h, bsx_raw_event* out) {

@@ -201,6 +201,83 @@ def test_device_sample_checks_large_runs(offset):
     assert m.time_ns == t_end
 
 
+def test_device_runs_padding_not_checked():
+    """Device runs that leave samples unreferenced: the padding between runs holds NaN, infinities
+    and over-range values. Only the referenced samples are checked, as the reference's
+    check_inputs_kernel does (sensor_cuda.cu:415-440), so the call succeeds, and input_clamps counts
+    only the referenced values the clamp changes, equal to the oracle's count; events equal too."""
+    import torch
+
+    import ballasim_amd as bsx
+    W, H, n, pad = 32, 24, 9, 3
+    P = W * H
+    rng = np.random.default_rng(77)
+    p = _params("physical_leak")
+    gpu = bsx.SensorModel(p, W, H, 5, device="cuda")
+    ref = O.OracleModel(p.c, W, H, 5, 1, 0)
+    irr0 = np.exp(rng.normal(3, 1, size=(P, 1))).astype(np.float32)
+    gpu.init(0, irr0)
+    ref.init(0, irr0)
+    ids = rng.permutation(P).astype(np.uint32)[: P - 40]
+    m = len(ids)
+    start = np.arange(m, dtype=np.int64) * (n + pad) + pad
+    count = rng.integers(0, n + 1, size=m).astype(np.int32)
+    S = m * (n + pad) + pad
+    t = np.zeros(S, np.int64)
+    irr = np.full(S, np.nan, np.float32)
+    irr[1::7] = np.inf
+    irr[2::7] = gpu.irr_max * 8.0
+    for i in range(m):
+        s, c = int(start[i]), int(count[i])
+        t[s:s + c] = np.sort(rng.choice(np.arange(1, 2 ** 23), size=c, replace=False))
+        irr[s:s + c] = np.exp(rng.normal(3, 1.5, size=c))
+    big = rng.choice(np.flatnonzero(np.repeat(count > 0, 1)), size=25, replace=False)
+    irr[start[big]] = gpu.irr_max * 4.0
+    want_clamps = sum(int(np.count_nonzero(irr[s:s + c] > gpu.irr_max)) for s, c in zip(start, count))
+    assert want_clamps >= 25
+    want, rep = ref._advance(ids, t, irr.reshape(S, 1), start, count, 2 ** 23)
+    assert ref.stats()["input_clamps"] == want_clamps
+    dev = "cuda"
+    c0 = gpu.stats()["input_clamps"]
+    ptr, cnt = gpu.advance_device(torch.from_numpy(ids.astype(np.int32)).to(dev), torch.from_numpy(start).to(dev),
+                                  torch.from_numpy(count).to(dev), torch.from_numpy(t).to(dev),
+                                  torch.from_numpy(irr).to(dev), 2 ** 23)
+    got = events_from_dict(gpu.download_events(ptr, cnt), W)
+    assert gpu.stats()["input_clamps"] - c0 == want_clamps
+    assert len(got) == len(want) and len(want) > 0
+    i, a, b = first_difference(got, want)
+    assert a is None, f"event {i}: gpu {a} oracle {b}"
+
+
+def test_device_runs_tensor_checks():
+    """Tensors the C side cannot check are refused by the wrapper: host tensors, strided views,
+    wrong dtypes and mismatched lengths."""
+    import torch
+
+    import ballasim_amd as bsx
+    m = bsx.SensorModel("ideal", 8, 8, 1, device="cuda")
+    m.init(0, np.ones((64, 1), np.float32))
+    dev = "cuda"
+    good = dict(pixel=torch.arange(4, dtype=torch.int32, device=dev),
+                row_start=torch.arange(4, dtype=torch.int64, device=dev) * 2,
+                row_count=torch.full((4,), 2, dtype=torch.int32, device=dev),
+                t=torch.arange(1, 9, dtype=torch.int64, device=dev) * 100,
+                irr=torch.ones(8, dtype=torch.float32, device=dev))
+    bad = [("pixel", good["pixel"].cpu(), "on the GPU"),
+           ("row_start", good["row_start"].to(torch.int32), "dtype"),
+           ("row_count", good["row_count"].to(torch.int64), "dtype"),
+           ("t", torch.arange(1, 17, dtype=torch.int64, device=dev)[::2] * 100, "contiguous"),
+           ("irr", torch.ones(7, dtype=torch.float32, device=dev), "irr must hold"),
+           ("row_count", torch.full((3,), 2, dtype=torch.int32, device=dev), "same length")]
+    for name, value, msg in bad:
+        args = dict(good, **{name: value})
+        with pytest.raises(ValueError, match=msg):
+            m.advance_device(args["pixel"], args["row_start"], args["row_count"], args["t"], args["irr"], 10_000)
+        assert m.time_ns == 0
+    m.advance_device(good["pixel"], good["row_start"], good["row_count"], good["t"], good["irr"], 10_000)
+    assert m.time_ns == 10_000
+
+
 def test_event_buffer_overflow_reruns():
     """A forced tiny event capacity: the call re-runs from its input state with a larger buffer and
     gives the same events."""
