@@ -3,4 +3,4 @@
 The compute runs in ballasim_amd/lib/libbsx.so (hand-written HIP kernels behind the C ABI of
 include/bsx/bsx_sensor.h); this package holds the Python surface over it.
 """
-from .sensor import Rig, SensorModel, SensorParams, device_count, preset, preset_names  # noqa: F401
+from .sensor import Rig, SensorModel, SensorParams, device_count, noise_tables, preset, preset_names  # noqa: F401

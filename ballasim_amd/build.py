@@ -20,6 +20,7 @@ SOURCES = [
     "capi.cpp",
     "sensor/params.cpp",
     "sensor/maps.cpp",
+    "sensor/noise_tables.cpp",
     "sensor/model.cpp",
     "sensor/chain.cpp",
     "sensor/rig.cpp",

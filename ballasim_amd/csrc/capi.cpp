@@ -20,6 +20,7 @@
 #include "output/output.h"
 #include "sensor/kernels.h"
 #include "sensor/model.h"
+#include "sensor/noise_tables.h"
 #include "sensor/params.h"
 #include "sensor/rig.h"
 
@@ -162,6 +163,31 @@ int bsx_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t 
     if (leak != nullptr) std::memcpy(leak, m.leak.data(), n * 8);
     if (dark_fA != nullptr) std::memcpy(dark_fA, m.dark_fA.data(), n * 8);
     if (phi0 != nullptr) std::memcpy(phi0, m.phi0.data(), n * 8);
+  });
+}
+
+int bsx_noise_tables(const bsx_params* p, double* data64, float* data32, size_t cap, int32_t* n_nodes,
+                     int32_t* stride) {
+  return guarded([&] {
+    need_ptr(p, "params");
+    need_ptr(n_nodes, "n_nodes");
+    need_ptr(stride, "stride");
+    bsx::validate_ranges(*p);
+    bsx::NoiseModelParams nm{};
+    nm.kq = p->k * bsx::kElementaryCharge;
+    nm.a2 = 1.0 / p->tau_o;
+    nm.a3 = 1.0 / p->tau_sf;
+    nm.sigma_o = p->sigma_o;
+    nm.levels = p->noise_leaf_level;
+    const bsx::NoiseTableSet t = bsx::build_noise_tables(nm);
+    *n_nodes = t.n_nodes;
+    *stride = t.stride;
+    const size_t need = t.data64.size();
+    if ((data64 != nullptr || data32 != nullptr) && cap < need) {
+      throw std::invalid_argument("bsx_noise_tables: cap is smaller than n_nodes * stride");
+    }
+    if (data64 != nullptr) std::memcpy(data64, t.data64.data(), need * sizeof(double));
+    if (data32 != nullptr) std::memcpy(data32, t.data32.data(), need * sizeof(float));
   });
 }
 

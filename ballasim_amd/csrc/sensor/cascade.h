@@ -510,9 +510,10 @@ struct CascadeLane : InstantLane<kMaxG> {
     }
   }
 
-  // The sub-steps of one linear piece, each followed by the comparator.
-  template <class Sink>
-  BSX_HD BSX_INLINE void run_piece(const cascade::Piece& pc, Sink& sink) {
+  // The sub-steps of one linear piece, each followed by the comparator (cmp(sub): the plain one,
+  // or the noise lane's tree comparator).
+  template <class Cmp>
+  BSX_HD BSX_INLINE void run_piece(const cascade::Piece& pc, Cmp&& cmp) {
     int64_t t_a = pc.t_a, h_hint = 0;
     while (t_a < pc.t_b) {
       cascade::SubStep sub;
@@ -567,7 +568,7 @@ struct CascadeLane : InstantLane<kMaxG> {
       }
       sub.t_b = t_b;
       t_a = t_b;
-      compare(sub, sink);
+      cmp(sub);
     }
   }
 
@@ -588,7 +589,8 @@ struct CascadeLane : InstantLane<kMaxG> {
       while (t < stop) {
         const int64_t e = vmin(stop, ((t >> kPlainSegmentLog2) + 1) * kPlainSegmentNs);
         ++n_pieces;
-        run_piece(cascade::Piece{t, e, this->photocurrent(t), this->photocurrent(e)}, sink);
+        run_piece(cascade::Piece{t, e, this->photocurrent(t), this->photocurrent(e)},
+                  [&](const cascade::SubStep& sub) { compare(sub, sink); });
         t = e;
       }
       if (tb == kNoBreakpoint) break;

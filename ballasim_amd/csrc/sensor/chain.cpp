@@ -56,6 +56,7 @@ ChainArgs LaunchChain::args(const WalkTuning& tu) const {
   a.n_total = n_total_;
   a.groups = groups_;
   a.instant = instant_;
+  a.noise = noise_;
   a.call = d_call_.as<const CallDev>();
   a.seg_capacity = seg_capacity_;
   a.run_capacity = run_capacity_;
@@ -127,12 +128,21 @@ void LaunchChain::launch(std::vector<CameraDev>& cams, bool instant, int64_t t_e
     groups = std::max(groups, c.kc.groups);
   }
   if (base > 0xffffffffull) throw std::invalid_argument("launch chain: more than 2^32 - 1 pixels in one launch");
+  const bool noise = !instant && cams[0].kc.noise.tab != nullptr;
+  for (const CameraDev& c : cams) {
+    if ((!instant && c.kc.noise.tab != nullptr) != noise) {
+      throw std::invalid_argument("launch chain: the cameras of one launch must all have the in-path noise on, or all off");
+    }
+  }
   const int n = static_cast<int>(cams.size());
-  if (n != n_cams_ || base != n_total_ || groups != groups_ || instant != instant_) graph_valid_ = false;
+  if (n != n_cams_ || base != n_total_ || groups != groups_ || instant != instant_ || noise != noise_) {
+    graph_valid_ = false;
+  }
   n_cams_ = n;
   n_total_ = static_cast<uint32_t>(base);
   groups_ = groups;
   instant_ = instant;
+  noise_ = noise;
   if (!graph_on_) {
     run_capacity_ = rb;
     seg_capacity_ = sb;

@@ -70,6 +70,7 @@ class LaunchChain {
   uint32_t n_total_ = 0;
   int groups_ = 1;
   bool instant_ = true;
+  bool noise_ = false;
   int64_t seg_capacity_ = 0, run_capacity_ = 0;
   int64_t t_end_ = 0;
 

@@ -22,6 +22,15 @@ inline constexpr uint16_t kFlagFirstAfterInit = 1u << 12;
 // This library's flag for events of the rate-specified shot-noise process.
 inline constexpr uint16_t kFlagShot = 1u << 13;
 
+// The cascade's in-path noise (noise on): the fp32 tables over ln a1 and their grid.
+struct NoiseDev {
+  const BSX_GLOBAL float* tab;  // n_nodes x stride (null: noise off)
+  int32_t levels;               // leaf level L of the bridge tree (leaves of 2^(16 - L) ns)
+  int32_t n_nodes, stride;
+  float ln_min, inv_dln;        // the grid: node i at ln a1 = ln_min + i / inv_dln
+  float margin_z;               // the exclusion margin's z
+};
+
 // Constants of one call of the fp32 kernel (kernel units: fA, s, log-e).
 struct KConst {
   int32_t groups;
@@ -40,6 +49,7 @@ struct KConst {
   uint64_t seed;
   uint32_t camera;
   uint32_t width;
+  NoiseDev noise;
 };
 
 // Per-pixel constants in the kernel's precision (the reference casts its fp64 maps to T once per
@@ -74,6 +84,14 @@ struct StateDev {
   BSX_GLOBAL uint32_t* shot_index;
   BSX_GLOBAL uint16_t* flags;
   BSX_GLOBAL float* held;  // [g * n_pixels + p]
+  // The in-path noise (null when off): the noise (n1, n2, n3) at the current cell's start and end
+  // nodes ([i * n_pixels + p]), the cell's frozen rate ln a1, the cell index and whether the end
+  // node is drawn.
+  BSX_GLOBAL float* nk;
+  BSX_GLOBAL float* nk1;
+  BSX_GLOBAL float* ln_a1;
+  BSX_GLOBAL int64_t* cell;
+  BSX_GLOBAL uint8_t* have_k1;
 };
 
 // Sample runs of a call as the kernels read them: row_of[p] is pixel p's run or -1.

@@ -20,6 +20,7 @@ struct ChainArgs {
   uint32_t n_total;      // pixels of all cameras
   int groups;            // the largest light-group count of the cameras (the kernels' register tier)
   bool instant;          // instantaneous response (else the cascade)
+  bool noise;            // the cascade with its in-path noise
   const CallDev* call;
   int64_t seg_capacity;  // grid extents, at least the call's n_segments and n_runs
   int64_t run_capacity;

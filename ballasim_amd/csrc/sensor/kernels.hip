@@ -7,6 +7,7 @@
 #include "cascade.h"
 #include "kernels.h"
 #include "instant.h"
+#include "noise.h"
 
 namespace bsx {
 
@@ -25,9 +26,9 @@ template <int kMaxG>
 __global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateDev st, const float* irr0, int64_t t0,
                                                    unsigned long long* counters) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t clamps = 0;
+  uint32_t clamps = 0, cells = 0, refinements = 0;
   if (p < kc.n_pixels) {
-    InstantLane<kMaxG> L;
+    NoiseLane<kMaxG> L;
     L.G = kc.groups;
     L.dark = mp.dark[p];
     L.scale = mp.scale[p];
@@ -56,9 +57,36 @@ __global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateD
     st.s3[p] = 0.0f;
     st.tv[p] = 0.0f;
     const float s3_ref = kc.instant ? L.log_ratio(L.photocurrent(t0)) : 0.0f;
+    float n3_0 = 0.0f;
+    if (!kc.instant && kc.noise.tab != nullptr) {
+      // the cascade's in-path noise: the stationary noise at the cell's start node, extended to t0
+      L.pixel = static_cast<uint32_t>(p);
+      L.seed = kc.seed;
+      L.camera = kc.camera;
+      L.nd = kc.noise;
+      L.margin_z = kc.noise.margin_z;
+      L.cc = cascade::Consts{kc.k, kc.a2, kc.a3, kc.tau_o, kc.tau_sf, kc.eps_accept, kc.eps_drop};
+      L.z = 1.0f / mean_I;
+      L.x1 = 0.0f;
+      L.s2 = 0.0f;
+      L.s3 = 0.0f;
+      L.n_cells = L.n_refinements = 0;
+      L.mid = nullptr;
+      L.mid_stride = 0;
+      n3_0 = L.init(t0, mean_I);
+      for (int i = 0; i < 3; ++i) {
+        st.nk[static_cast<int64_t>(i) * kc.n_pixels + p] = L.nk[i];
+        st.nk1[static_cast<int64_t>(i) * kc.n_pixels + p] = L.nk1[i];
+      }
+      st.ln_a1[p] = L.ln_a1;
+      st.cell[p] = L.cell;
+      st.have_k1[p] = L.have_k1 ? 1 : 0;
+      cells = L.n_cells;
+      refinements = L.n_refinements;
+    }
     st.t_ref[p] = t0;
     st.s3_ref[p] = s3_ref;
-    st.n3_ref[p] = 0.0f;
+    st.n3_ref[p] = n3_0;
     st.dr_ref[p] = mp.phi0[p];
     st.blind_until[p] = kNotBlind;
     st.t_search[p] = t0 + 1;
@@ -69,6 +97,8 @@ __global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateD
     st.shot_index[p] = 0;
   }
   wave_add(counters + 1 + kStClamps, clamps);
+  wave_add(counters + 1 + kStCells, cells);
+  wave_add(counters + 1 + kStRefinements, refinements);
 }
 
 // Sum of v over the lanes of the wave that share `key` (keys non-decreasing along the lanes), added
@@ -855,6 +885,142 @@ __global__ __launch_bounds__(256) void cascade_walk_kernel(const CameraDev* cams
   walk_groups(n_total, queue, tu, [&](uint32_t i) { cascade_lane<kMaxG>(cams, n_cams, n_total, order, i, t_end); });
 }
 
+// The cascade with its in-path noise: lane i of the launch's pixel order takes pixel order[i] of its
+// camera and advances it to t_end; mid is the lane's column of the tree walk's path memory (LDS).
+template <int kMaxG>
+__device__ __forceinline__ void noise_lane(const CameraDev* cams, int n_cams, uint32_t n_total, const uint32_t* order,
+                                           uint32_t i, int64_t t_end, float* mid, int mid_stride) {
+  NoiseLane<kMaxG> L;
+  L.n_substeps = L.n_pieces = L.n_leaves = 0;
+  L.n_attempts = L.n_settled = L.n_fallbacks = L.n_quarantines = 0;
+  L.n_cells = L.n_refinements = 0;
+  int ci = -1;
+  const CameraDev* cam = cams;
+  int p = 0;
+  if (i < n_total) {
+    const uint32_t g = order != nullptr ? order[i] : i;
+    ci = camera_of(cams, n_cams, g);
+    cam = cams + ci;
+    p = static_cast<int>(g - cam->base);
+  }
+  RigSink sink{cam, ci};
+  if (ci >= 0) {
+    const KConst& kc = cam->kc;
+    const MapsDev& mp = cam->mp;
+    const StateDev& in = cam->in;
+    const StateDev& out = cam->out;
+    const RunsDev& runs = cam->runs;
+    const int64_t np = kc.n_pixels;
+    L.pixel = static_cast<uint32_t>(p);
+    L.theta_on = mp.theta_on[p];
+    L.theta_off = mp.theta_off[p];
+    L.leak = mp.leak[p];
+    L.dark = mp.dark[p];
+    L.scale = mp.scale[p];
+    L.irr_max = kc.irr_max;
+    L.eps_ci = kc.eps_ci;
+    L.label_cut = kc.label_cut;
+    L.refractory_ns = kc.refractory_ns;
+    L.G = kc.groups;
+    L.ideal_reset = kc.ideal_reset != 0;
+    L.hot = mp.hot[p] != 0;
+    L.seed = kc.seed;
+    L.camera = kc.camera;
+    L.cc = cascade::Consts{kc.k, kc.a2, kc.a3, kc.tau_o, kc.tau_sf, kc.eps_accept, kc.eps_drop};
+    L.nd = kc.noise;
+    L.margin_z = kc.noise.margin_z;
+    L.mid = mid;
+    L.mid_stride = mid_stride;
+    const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
+    L.prep = nullptr;
+    if (row >= 0) {
+      const int64_t start = runs.row_start[row];
+      L.ts = runs.t + start;
+      L.irr = runs.irr + start * kc.groups;
+      L.n = runs.row_count[row];
+    } else {
+      L.ts = nullptr;
+      L.irr = nullptr;
+      L.n = 0;
+    }
+    L.t_state = cam->t_state;
+    L.skip = L.n > 0 && L.ts[0] == L.t_state ? 1 : 0;
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) L.held[g] = in.held[static_cast<int64_t>(g) * np + p];
+    L.i_scale = in.i_scale[p];
+    L.z = in.z[p];
+    L.x1 = in.x1[p];
+    L.s2 = in.s2[p];
+    L.s3 = in.s3[p];
+    L.tv = in.tv[p];
+    L.s3_ref = in.s3_ref[p];
+    L.n3_ref = in.n3_ref[p];
+    L.dr_ref = in.dr_ref[p];
+    L.t_ref = in.t_ref[p];
+    L.blind_until = in.blind_until[p];
+    L.t_search = in.t_search[p];
+    L.flags = in.flags[p];
+    for (int c = 0; c < 3; ++c) {
+      L.nk[c] = in.nk[c * np + p];
+      L.nk1[c] = in.nk1[c * np + p];
+    }
+    L.ln_a1 = in.ln_a1[p];
+    L.cell = in.cell[p];
+    L.have_k1 = in.have_k1[p] != 0;
+    L.shot_rate = 0.0;
+    L.shot_next = INT64_MAX;
+    L.shot_index = 0;
+    L.resume();
+    L.advance(t_end, sink);
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) out.held[static_cast<int64_t>(g) * np + p] = L.held[g];
+    out.z[p] = L.z;
+    out.x1[p] = L.x1;
+    out.s2[p] = L.s2;
+    out.s3[p] = L.s3;
+    out.tv[p] = L.tv;
+    out.i_scale[p] = L.i_scale;
+    out.s3_ref[p] = L.s3_ref;
+    out.n3_ref[p] = L.n3_ref;
+    out.dr_ref[p] = L.dr_ref;
+    out.t_ref[p] = L.t_ref;
+    out.blind_until[p] = L.blind_until;
+    out.t_search[p] = L.t_search;
+    out.flags[p] = static_cast<uint16_t>(L.flags);
+    out.shot_next[p] = in.shot_next[p];
+    out.shot_index[p] = in.shot_index[p];
+    for (int c = 0; c < 3; ++c) {
+      out.nk[c * np + p] = L.nk[c];
+      out.nk1[c * np + p] = L.nk1[c];
+    }
+    out.ln_a1[p] = L.ln_a1;
+    out.cell[p] = L.cell;
+    out.have_k1[p] = L.have_k1 ? 1 : 0;
+  }
+  camera_add(cams, ci, 1 + kStSubsteps, L.n_substeps);
+  camera_add(cams, ci, 1 + kStAttempts, L.n_attempts);
+  camera_add(cams, ci, 1 + kStSettled, L.n_settled);
+  camera_add(cams, ci, 1 + kStLeaves, L.n_leaves);
+  camera_add(cams, ci, 1 + kStFallbacks, L.n_fallbacks);
+  camera_add(cams, ci, 1 + kStQuarantines, L.n_quarantines);
+  camera_add(cams, ci, 1 + kStPieces, L.n_pieces);
+  camera_add(cams, ci, 1 + kStCells, L.n_cells);
+  camera_add(cams, ci, 1 + kStRefinements, L.n_refinements);
+}
+
+template <int kMaxG>
+__global__ __launch_bounds__(256) void noise_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
+                                                         const uint32_t* order, unsigned int* queue, WalkTuning tu,
+                                                         const CallDev* call) {
+  // The lanes' path memory of the tree walk, one column per lane (blockDim.x <= 256 by the launch
+  // bound).
+  __shared__ float mid[noise::kMidSlots * 256];
+  const int64_t t_end = call->t_end;
+  walk_groups(n_total, queue, tu, [&](uint32_t i) {
+    noise_lane<kMaxG>(cams, n_cams, n_total, order, i, t_end, mid + threadIdx.x, blockDim.x);
+  });
+}
+
 // row_of[pixel[i]] = i, flagging out-of-range ids (flags[0]), duplicates (flags[1]) and runs outside
 // the sample array (flags[2]), so that no kernel reads past the runs it was given.
 __global__ void scatter_rows_kernel(const uint32_t* pixel, const int64_t* row_start, const int32_t* row_count,
@@ -957,19 +1123,21 @@ namespace {
 
 // Blocks of the walk kernel resident on the device at once (a persistent grid's size), per kernel,
 // light-group tier and block size.
-int resident_blocks(bool instant, int tier, int block) {
-  static int cache[2][4][4] = {};
-  int& r = cache[instant ? 1 : 0][tier][block / 64 - 1];
+int resident_blocks(int kind, int tier, int block) {
+  static int cache[3][4][4] = {};
+  int& r = cache[kind][tier][block / 64 - 1];
   if (r == 0) {
     int dev = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    static const void* const kernels[2][4] = {
+    static const void* const kernels[3][4] = {
         {reinterpret_cast<const void*>(cascade_walk_kernel<1>), reinterpret_cast<const void*>(cascade_walk_kernel<2>),
          reinterpret_cast<const void*>(cascade_walk_kernel<4>), reinterpret_cast<const void*>(cascade_walk_kernel<8>)},
         {reinterpret_cast<const void*>(instant_walk_kernel<1>), reinterpret_cast<const void*>(instant_walk_kernel<2>),
-         reinterpret_cast<const void*>(instant_walk_kernel<4>), reinterpret_cast<const void*>(instant_walk_kernel<8>)}};
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[instant ? 1 : 0][tier], block, 0) != hipSuccess)
+         reinterpret_cast<const void*>(instant_walk_kernel<4>), reinterpret_cast<const void*>(instant_walk_kernel<8>)},
+        {reinterpret_cast<const void*>(noise_walk_kernel<1>), reinterpret_cast<const void*>(noise_walk_kernel<2>),
+         reinterpret_cast<const void*>(noise_walk_kernel<4>), reinterpret_cast<const void*>(noise_walk_kernel<8>)}};
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[kind][tier], block, 0) != hipSuccess)
       per_cu = 1;
     r = vmax(1, cus) * vmax(1, per_cu);
   }
@@ -1014,7 +1182,8 @@ int chain_walk_blocks(const ChainArgs& a) {
   const int block = a.tu.block;
   const unsigned full = static_cast<unsigned>((static_cast<uint64_t>(a.n_total) + block - 1) / block);
   if (a.tu.chunk <= 0 || a.queue == nullptr) return static_cast<int>(full);
-  return static_cast<int>(vmin(full, static_cast<unsigned>(resident_blocks(a.instant, group_tier(a.groups), block))));
+  const int kind = a.instant ? 1 : a.noise ? 2 : 0;
+  return static_cast<int>(vmin(full, static_cast<unsigned>(resident_blocks(kind, group_tier(a.groups), block))));
 }
 
 hipError_t launch_chain_walk(const ChainArgs& a, hipStream_t stream) {
@@ -1029,6 +1198,11 @@ hipError_t launch_chain_walk(const ChainArgs& a, hipStream_t stream) {
     else if (tier == 1) BSX_WALK(instant_walk_kernel, 2);
     else if (tier == 2) BSX_WALK(instant_walk_kernel, 4);
     else BSX_WALK(instant_walk_kernel, 8);
+  } else if (a.noise) {
+    if (tier == 0) BSX_WALK(noise_walk_kernel, 1);
+    else if (tier == 1) BSX_WALK(noise_walk_kernel, 2);
+    else if (tier == 2) BSX_WALK(noise_walk_kernel, 4);
+    else BSX_WALK(noise_walk_kernel, 8);
   } else {
     if (tier == 0) BSX_WALK(cascade_walk_kernel, 1);
     else if (tier == 1) BSX_WALK(cascade_walk_kernel, 2);

@@ -9,6 +9,7 @@
 
 #include "chain.h"
 #include "kernels.h"
+#include "noise_tables.h"
 
 namespace bsx {
 
@@ -117,6 +118,28 @@ SensorModel::SensorModel(const SensorParams& params, uint32_t width, uint32_t he
   size_t state_bytes = 0;
   state_bytes += 9 * align256(P_ * 4) + 4 * align256(P_ * 8) + align256(P_ * 4) + align256(P_ * 2);
   state_bytes += align256(P_ * 4 * G_);
+  if (noise_on(params_)) {
+    // nk[3], nk1[3], ln_a1 (fp32), cell (int64), have_k1 (byte)
+    state_bytes += 7 * align256(P_ * 4) + align256(P_ * 8) + align256(P_);
+    NoiseModelParams nm{};
+    nm.kq = params_.k * kElementaryCharge;
+    nm.a2 = 1.0 / params_.tau_o;
+    nm.a3 = 1.0 / params_.tau_sf;
+    nm.sigma_o = params_.sigma_o;
+    nm.levels = params_.noise_leaf_level;
+    const NoiseTableSet tabs = build_noise_tables(nm);
+    d_noise_tab_.ensure(tabs.data32.size() * sizeof(float));
+    hip_check(hipMemcpy(d_noise_tab_.get(), tabs.data32.data(), tabs.data32.size() * sizeof(float),
+                        hipMemcpyHostToDevice),
+              "upload noise tables");
+    noise_dev_.tab = d_noise_tab_.as<const float>();
+    noise_dev_.levels = tabs.levels;
+    noise_dev_.n_nodes = tabs.n_nodes;
+    noise_dev_.stride = tabs.stride;
+    noise_dev_.ln_min = static_cast<float>(kNoiseLnA1Min);
+    noise_dev_.inv_dln = static_cast<float>(tabs.inv_dln);
+    noise_dev_.margin_z = static_cast<float>(params_.noise_margin_z);
+  }
   d_state_[0].ensure(state_bytes);
   d_state_[1].ensure(state_bytes);
   upload_maps();
@@ -156,6 +179,7 @@ KConst SensorModel::kconst() const {
   kc.seed = seed_;
   kc.camera = camera_;
   kc.width = width_;
+  kc.noise = noise_dev_;
   return kc;
 }
 
@@ -201,6 +225,18 @@ StateDev SensorModel::state_view(int which) const {
   s.flags = reinterpret_cast<uint16_t*>(b);
   b += align256(P_ * 2);
   s.held = reinterpret_cast<float*>(b);
+  b += align256(P_ * 4 * G_);
+  if (noise_on(params_)) {
+    s.nk = reinterpret_cast<float*>(b);
+    b += 3 * f4;
+    s.nk1 = reinterpret_cast<float*>(b);
+    b += 3 * f4;
+    s.ln_a1 = reinterpret_cast<float*>(b);
+    b += f4;
+    s.cell = reinterpret_cast<int64_t*>(b);
+    b += f8;
+    s.have_k1 = reinterpret_cast<uint8_t*>(b);
+  }
   return s;
 }
 
@@ -254,6 +290,8 @@ void SensorModel::init(int64_t t0_ns, const float* irradiance0) {
   unsigned long long c[1 + kStCount];
   hip_check(hipMemcpy(c, d_counters_.get(), sizeof(c), hipMemcpyDeviceToHost), "read counters");
   stats_.input_clamps += c[1 + kStClamps];
+  stats_.cells += c[1 + kStCells];
+  stats_.refinements += c[1 + kStRefinements];
   t_state_ = t0_ns;
   initialized_ = true;
 }
@@ -581,11 +619,28 @@ void SensorModel::save_state(uint8_t* out, size_t n) const {
   geti(s.t_ref, &SavedPixelState::t_ref);
   geti(s.blind_until, &SavedPixelState::blind_until);
   geti(s.t_search, &SavedPixelState::t_search);
-  // The shot-noise process's next arrival and index ride in fields the noise-free kernel leaves
-  // unused (cell and nk[0]).
-  geti(s.shot_next, &SavedPixelState::cell);
-  hip_check(hipMemcpy(u.data(), s.shot_index, P_ * 4, hipMemcpyDeviceToHost), "download state");
-  for (size_t i = 0; i < P_; ++i) ps[i].nk[0] = static_cast<double>(u[i]);
+  if (noise_on(params_)) {
+    // the in-path noise: the nodes, the cell's rate, the cell and whether its end node is drawn
+    for (int c = 0; c < 3; ++c) {
+      hip_check(hipMemcpy(f.data(), s.nk + static_cast<size_t>(c) * P_, P_ * 4, hipMemcpyDeviceToHost),
+                "download state");
+      for (size_t i = 0; i < P_; ++i) ps[i].nk[c] = f[i];
+      hip_check(hipMemcpy(f.data(), s.nk1 + static_cast<size_t>(c) * P_, P_ * 4, hipMemcpyDeviceToHost),
+                "download state");
+      for (size_t i = 0; i < P_; ++i) ps[i].nk1[c] = f[i];
+    }
+    getf(s.ln_a1, &SavedPixelState::ln_a1);
+    geti(s.cell, &SavedPixelState::cell);
+    std::vector<uint8_t> hk(P_);
+    hip_check(hipMemcpy(hk.data(), s.have_k1, P_, hipMemcpyDeviceToHost), "download state");
+    for (size_t i = 0; i < P_; ++i) ps[i].have_k1 = hk[i];
+  } else {
+    // The shot-noise process's next arrival and index ride in fields the noise-free kernel leaves
+    // unused (cell and nk[0]).
+    geti(s.shot_next, &SavedPixelState::cell);
+    hip_check(hipMemcpy(u.data(), s.shot_index, P_ * 4, hipMemcpyDeviceToHost), "download state");
+    for (size_t i = 0; i < P_; ++i) ps[i].nk[0] = static_cast<double>(u[i]);
+  }
   hip_check(hipMemcpy(fl.data(), s.flags, P_ * 2, hipMemcpyDeviceToHost), "download state");
   for (size_t i = 0; i < P_; ++i) ps[i].flags = fl[i];
   for (int g = 0; g < G_; ++g) {
@@ -635,8 +690,28 @@ void SensorModel::load_state(const uint8_t* data, size_t n) {
   puti(s.t_ref, &SavedPixelState::t_ref);
   puti(s.blind_until, &SavedPixelState::blind_until);
   puti(s.t_search, &SavedPixelState::t_search);
-  puti(s.shot_next, &SavedPixelState::cell);
-  for (size_t i = 0; i < P_; ++i) u[i] = static_cast<uint32_t>(ps[i].nk[0]);
+  if (noise_on(params_)) {
+    for (int c = 0; c < 3; ++c) {
+      for (size_t i = 0; i < P_; ++i) f[i] = static_cast<float>(ps[i].nk[c]);
+      hip_check(hipMemcpy(s.nk + static_cast<size_t>(c) * P_, f.data(), P_ * 4, hipMemcpyHostToDevice),
+                "upload state");
+      for (size_t i = 0; i < P_; ++i) f[i] = static_cast<float>(ps[i].nk1[c]);
+      hip_check(hipMemcpy(s.nk1 + static_cast<size_t>(c) * P_, f.data(), P_ * 4, hipMemcpyHostToDevice),
+                "upload state");
+    }
+    putf(s.ln_a1, &SavedPixelState::ln_a1);
+    puti(s.cell, &SavedPixelState::cell);
+    std::vector<uint8_t> hk(P_);
+    for (size_t i = 0; i < P_; ++i) hk[i] = ps[i].have_k1;
+    hip_check(hipMemcpy(s.have_k1, hk.data(), P_, hipMemcpyHostToDevice), "upload state");
+    // no shot process with the in-path noise
+    for (size_t i = 0; i < P_; ++i) q[i] = INT64_MAX;
+    hip_check(hipMemcpy(s.shot_next, q.data(), P_ * 8, hipMemcpyHostToDevice), "upload state");
+    std::fill(u.begin(), u.end(), 0u);
+  } else {
+    puti(s.shot_next, &SavedPixelState::cell);
+    for (size_t i = 0; i < P_; ++i) u[i] = static_cast<uint32_t>(ps[i].nk[0]);
+  }
   hip_check(hipMemcpy(s.shot_index, u.data(), P_ * 4, hipMemcpyHostToDevice), "upload state");
   for (size_t i = 0; i < P_; ++i) fl[i] = ps[i].flags;
   hip_check(hipMemcpy(s.flags, fl.data(), P_ * 2, hipMemcpyHostToDevice), "upload state");

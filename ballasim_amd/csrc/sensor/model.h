@@ -157,6 +157,8 @@ class SensorModel {
   float irr_max_ = 0.0f;
 
   DeviceBuffer d_maps_;       // 6 fp32 arrays of P
+  DeviceBuffer d_noise_tab_;  // the in-path noise's fp32 tables (noise on)
+  NoiseDev noise_dev_{};      // their view (tab null: noise off)
   DeviceBuffer d_state_[2];   // two state copies
   int cur_ = 0;
   DeviceBuffer d_events_;

@@ -191,7 +191,10 @@ void validate(const SensorParams& p) {
   need(p.tau_hp == 0, "the high-pass filter (tau_hp > 0) is not supported by the bsx backend");
   need(p.limiter == 0, "the rate limiter is not supported by the bsx backend");
   need(p.initial_state == 0, "initial_state stationary is not supported by the bsx backend");
-  need(!noise_on(p), "the cascade's in-path noise (noise on) is not supported by the bsx backend yet");
+  need(!noise_on(p) || p.noise_diffusion == 0,
+       "noise_diffusion transient (the in-path noise's transient rule) is not supported by the bsx backend");
+  need(!noise_on(p) || p.hot_fraction == 0 || p.hot_leak_fraction >= 1.0,
+       "threshold-type hot pixels (hot_leak_fraction < 1 with the in-path noise) are not supported by the bsx backend");
   need(p.shot_rate_hz == 0 || p.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS,
        "shot_rate_hz > 0 (the shot-noise process) needs the instantaneous photoreceptor");
 }

@@ -76,6 +76,19 @@ def preset_names():
     return buf.value.decode().split(",")
 
 
+def noise_tables(params):
+    """The in-path noise's tables of a parameter set (fp64 and the fp32 copy the kernels read), each
+    (n_nodes, stride), built on the host (reference: NoiseTables, src/sensor/noise_tables.cpp:341-411)."""
+    lib = _abi.load_library()
+    p = params._p if isinstance(params, SensorParams) else params
+    n, s = C.c_int32(0), C.c_int32(0)
+    _abi.check(lib.bsx_noise_tables(C.byref(p), None, None, 0, C.byref(n), C.byref(s)))
+    d64 = np.zeros((n.value, s.value), np.float64)
+    d32 = np.zeros((n.value, s.value), np.float32)
+    _abi.check(lib.bsx_noise_tables(C.byref(p), _ptr(d64), _ptr(d32), d64.size, C.byref(n), C.byref(s)))
+    return d64, d32
+
+
 def device_count():
     lib = _abi.load_library()
     n = C.c_int32(0)

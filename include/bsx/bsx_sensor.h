@@ -191,6 +191,14 @@ int bsx_params_validate(const bsx_params* p);
 int bsx_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t n_pixels, double* theta_on,
                   double* theta_off, double* leak, double* dark_fA, double* phi0);
 
+/* The in-path noise's tables of a cascade parameter set, built on the host in long double (no
+ * device needed). Reference: NoiseTables(NoiseModel{k q, 1/tau_o, 1/tau_sf, sigma_o,
+ * noise_leaf_level}), src/sensor/noise_tables.h:86-108 / noise_tables.cpp:341-411 (frozen
+ * diffusion). *n_nodes and *stride receive the shape; data64 / data32 (either may be null) receive
+ * n_nodes * stride values when cap holds that many, else BSX_E_INVALID. */
+int bsx_noise_tables(const bsx_params* p, double* data64, float* data32, size_t cap, int32_t* n_nodes,
+                     int32_t* stride);
+
 /* Reference: SensorModel::SensorModel(params, width, height, seed, light_groups, camera, ...,
  * Precision::kFp32, Device::kCuda), src/sensor/sensor_model.h:107-110 (num_threads, precision and
  * roi have no meaning here: one GPU, fp32, full frame). */

@@ -38,7 +38,22 @@ CASES = {
                                    "camera": 6, "t0": -2 * WINDOW, "calls": 3, "input_seed": 5,
                                    "overrides": {"noise": 0, "reset_mode": 1, "refractory_s": 2e-5,
                                                  "leak_rate_hz": 1.5}},
+    # the reference's default sensor: the cascade with its in-path noise, unmodified
+    "cascade_prophesee_noise": {"preset": "prophesee_gen3", "width": 16, "height": 12, "groups": 1, "seed": 23,
+                                "camera": 1, "t0": 2 * WINDOW - 40_000, "calls": 3, "input_seed": 6},
+    "cascade_noise_ideal_levels3": {"preset": "prophesee_gen3", "width": 16, "height": 12, "groups": 1, "seed": 24,
+                                    "camera": 4, "t0": -3 * 65536, "calls": 3, "input_seed": 7,
+                                    "overrides": {"reset_mode": 1, "refractory_s": 1e-5, "noise_leaf_level": 3,
+                                                  "sigma_o": 0.12, "leak_rate_hz": 0.5}},
 }
+# Fields of the saved state each case stores (the noise fields only where the noise is on).
+STATE_FIELDS = ("z", "x1", "s2", "s3", "i_scale", "tv", "s3_ref", "n3_ref", "dr_ref", "t_ref", "blind_until",
+                "t_search", "flags")
+NOISE_STATE_FIELDS = ("nk", "nk1", "ln_a1", "cell", "have_k1")
+
+
+def noise_case(spec):
+    return spec["preset"] == "prophesee_gen3" and spec.get("overrides", {}).get("noise", 1) != 0
 
 
 def case_params(spec):

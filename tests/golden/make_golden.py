@@ -18,7 +18,8 @@ sys.path.insert(0, ROOT)
 
 from ballasim_amd import _abi  # noqa: E402
 from oracle import oracle as O  # noqa: E402
-from tests.golden.cases import CASES, MAP_CASES, case_params, case_inputs  # noqa: E402
+from tests.golden.cases import (CASES, MAP_CASES, NOISE_STATE_FIELDS, STATE_FIELDS, case_inputs,  # noqa: E402
+                                case_params, noise_case)
 
 
 def params_dict(p):
@@ -57,9 +58,10 @@ def main():
             arrays[f"events_{k}"] = ev
             arrays[f"reported_{k}"] = rep
         st = m.states()
-        for f in ("z", "x1", "s2", "s3", "i_scale", "tv", "s3_ref", "n3_ref", "dr_ref", "t_ref", "blind_until",
-                  "t_search", "flags"):
+        for f in STATE_FIELDS + (NOISE_STATE_FIELDS if noise_case(spec) else ()):
             arrays[f"state_{f}"] = st[f]
+        arrays["stats"] = np.array([m.stats()[k] for k in sorted(m.stats())], np.uint64)
+        arrays["stat_names"] = np.array(sorted(m.stats()))
         np.savez_compressed(os.path.join(HERE, f"case_{name}.npz"), **arrays)
         manifest["cases"][name] = {"params": params_dict(p), **{k: v for k, v in spec.items() if k != "overrides"},
                                    "overrides": spec.get("overrides", {}),

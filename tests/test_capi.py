@@ -10,6 +10,7 @@ import pytest
 
 import ballasim_amd as bsx
 from ballasim_amd import _abi
+from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -72,9 +73,36 @@ def test_validate_refuses_by_name(field, value, msg):
         p.validate()
 
 
-def test_noise_on_cascade_is_refused_not_approximated():
-    with pytest.raises(ValueError, match="noise"):
-        bsx.preset("prophesee_gen3").validate()
+def test_noise_on_cascade_is_accepted_and_its_unbuilt_variants_refused():
+    """The reference's default sensor (the cascade with its in-path noise) validates unmodified; the
+    noise variants this backend does not run are refused by name, never approximated."""
+    bsx.preset("prophesee_gen3").validate()
+    p = bsx.preset("prophesee_gen3")
+    p.noise_diffusion = 1
+    with pytest.raises(ValueError, match="transient"):
+        p.validate()
+    p = bsx.preset("prophesee_gen3")
+    p.hot_fraction = 0.01
+    p.hot_leak_fraction = 0.5
+    with pytest.raises(ValueError, match="threshold-type hot pixels"):
+        p.validate()
+    p = bsx.preset("prophesee_gen3")
+    p.shot_rate_hz = 0.1
+    with pytest.raises(ValueError, match="instantaneous"):
+        p.validate()
+
+
+def test_noise_tables_equal_oracle():
+    """The product's host tables (bsx_noise_tables) bit for bit against the oracle's, which the
+    reference pins (test_noise_oracle.py), at the default and the extreme tree depths."""
+    for levels in (5, 0, 10):
+        p = bsx.preset("prophesee_gen3")
+        p.noise_leaf_level = levels
+        a64, a32 = bsx.noise_tables(p)
+        b64, b32 = O.noise_tables(p.c)
+        assert a64.shape == (705, 18 + 47 * levels)
+        assert np.array_equal(a64.view(np.uint64), b64.view(np.uint64))
+        assert np.array_equal(a32.view(np.uint32), b32.view(np.uint32))
 
 
 def test_no_device_is_a_runtime_error():

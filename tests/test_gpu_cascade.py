@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.golden.cases import CASES
+from tests.golden.cases import CASES, NOISE_STATE_FIELDS, STATE_FIELDS, noise_case
 from tests.helpers import events_from_dict, first_difference, random_runs
 
 pytestmark = pytest.mark.gpu
@@ -103,6 +103,11 @@ def test_golden_fixture_replayed_on_gpu(name):
         assert len(got) == len(want) and i == len(got), f"call {k}: first difference at {i}: gpu {a} reference {b}"
         assert np.array_equal(ev["t_reported_ns"], g[f"reported_{k}"])
     st = np.frombuffer(gpu.save_state()[56:], O.REF_STATE_DTYPE)
-    for f in ("z", "x1", "s2", "s3", "i_scale", "tv", "s3_ref", "n3_ref", "dr_ref", "t_ref", "blind_until",
-              "t_search", "flags"):
+    for f in STATE_FIELDS + (NOISE_STATE_FIELDS if noise_case(CASES[name]) else ()):
         assert np.array_equal(st[f], g[f"state_{f}"]), f
+    got = gpu.stats()
+    instant = p.photoreceptor == "instantaneous"
+    for k, v in zip(g["stat_names"], g["stats"]):
+        if instant and str(k) == "leaf_intervals":
+            continue  # the instantaneous path counts its own searches' evaluations
+        assert got[str(k)] == int(v), (k, got[str(k)], int(v))

@@ -266,7 +266,7 @@ def test_device_runs_tensor_checks():
     bad = [("pixel", good["pixel"].cpu(), "on the GPU"),
            ("row_start", good["row_start"].to(torch.int32), "dtype"),
            ("row_count", good["row_count"].to(torch.int64), "dtype"),
-           ("t", torch.arange(1, 17, dtype=torch.int64, device=dev)[::2] * 100, "contiguous"),
+           ("t", (torch.arange(1, 17, dtype=torch.int64, device=dev) * 100)[::2], "contiguous"),
            ("irr", torch.ones(7, dtype=torch.float32, device=dev), "irr must hold"),
            ("row_count", torch.full((3,), 2, dtype=torch.int32, device=dev), "same length")]
     for name, value, msg in bad:

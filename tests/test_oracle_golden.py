@@ -10,7 +10,7 @@ import pytest
 
 from ballasim_amd import _abi
 from oracle import oracle as O
-from tests.golden.cases import CASES, MAP_CASES
+from tests.golden.cases import CASES, MAP_CASES, NOISE_STATE_FIELDS, STATE_FIELDS, noise_case
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MANIFEST = json.load(open(os.path.join(GOLDEN, "manifest.json")))
@@ -38,9 +38,11 @@ def test_oracle_matches_reference_fixture(name):
         assert np.array_equal(ev, want), f"call {k}"
         assert np.array_equal(rep, g[f"reported_{k}"])
     st = m.states()
-    for f in ("z", "x1", "s2", "s3", "i_scale", "tv", "s3_ref", "n3_ref", "dr_ref", "t_ref", "blind_until",
-              "t_search", "flags"):
+    for f in STATE_FIELDS + (NOISE_STATE_FIELDS if noise_case(CASES[name]) else ()):
         assert np.array_equal(st[f], g[f"state_{f}"]), f
+    got = m.stats()
+    for k, v in zip(g["stat_names"], g["stats"]):
+        assert got[str(k)] == int(v), (k, got[str(k)], int(v))
 
 
 @pytest.mark.parametrize("name", sorted(MAP_CASES))
