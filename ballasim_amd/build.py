@@ -29,6 +29,7 @@ SOURCES = [
     "output/output.hip",
     "mlp/mlp.hip",
     "sensor/kernels.hip",
+    "sensor/scratch.hip",
     "scene/scene_host.cpp",
     "scene/scene.hip",
     "scene/learned.hip",

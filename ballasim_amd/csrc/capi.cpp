@@ -262,6 +262,16 @@ int bsx_sensor_advance_device(bsx_sensor* s, const bsx_device_runs* runs, int64_
   });
 }
 
+int bsx_sensor_advance_device_noalloc(bsx_sensor* s, const bsx_device_runs* runs, int64_t t_end_ns, void* stream,
+                                      void* events, uint64_t events_cap, uint64_t* n_events) {
+  return guarded([&] {
+    need_ptr(runs, "runs");
+    need_ptr(n_events, "n_events");
+    *n_events = model_of(s).advance_device_noalloc(*runs, t_end_ns, static_cast<hipStream_t>(stream),
+                                                   static_cast<bsx::RawEventDev*>(events), events_cap);
+  });
+}
+
 int bsx_sensor_download_events(bsx_sensor* s, const bsx_device_events* events) {
   return guarded([&] {
     need_ptr(events, "events");

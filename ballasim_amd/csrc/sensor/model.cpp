@@ -10,6 +10,7 @@
 #include "chain.h"
 #include "kernels.h"
 #include "noise_tables.h"
+#include "scratch.h"
 
 namespace bsx {
 
@@ -541,6 +542,89 @@ bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64
   initialized_ = true;
   return bsx_device_events{reinterpret_cast<const bsx_raw_event*>(d_events_.get()),
                            static_cast<size_t>(counters_host_[0])};
+}
+
+uint64_t SensorModel::advance_device_noalloc(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream,
+                                             RawEventDev* events, uint64_t cap) {
+  static_assert(kScratchCounters + (1 + kStCount) * sizeof(unsigned long long) <= kScratchCamera, "scratch layout");
+  static_assert(kScratchCamera + sizeof(CameraDev) <= kScratchCall, "scratch layout");
+  static_assert(kScratchCall + sizeof(CallDev) <= kScratchRowOf, "scratch layout");
+  if (!initialized_) {
+    throw std::logic_error(
+        "SensorModel.advance: call init() or load_state() first (a call that failed part-way leaves the model "
+        "uninitialized)");
+  }
+  if (t_end_ns < t_state_) throw std::invalid_argument("SensorModel.advance: t_end_ns is before the model time");
+  if (P_ > kScratchMaxPixels) {
+    throw std::invalid_argument("advance_device_noalloc: the camera has " + std::to_string(P_) +
+                                " pixels; the static scratch serves at most " + std::to_string(kScratchMaxPixels));
+  }
+  if (events == nullptr && cap > 0) throw std::invalid_argument("advance_device_noalloc: events is null");
+  char* base = static_cast<char*>(scratch_address());
+  unsigned int* flags = reinterpret_cast<unsigned int*>(base + kScratchFlags);
+  unsigned long long* chk = reinterpret_cast<unsigned long long*>(base + kScratchChecks);
+  unsigned long long* counters = reinterpret_cast<unsigned long long*>(base + kScratchCounters);
+  CameraDev* d_cam = reinterpret_cast<CameraDev*>(base + kScratchCamera);
+  CallDev* d_call = reinterpret_cast<CallDev*>(base + kScratchCall);
+  int32_t* row_of = reinterpret_cast<int32_t*>(base + kScratchRowOf);
+  // the checks of the runs, before any state change
+  hip_check(launch_scatter_rows(runs.pixel, runs.row_start, runs.row_count, runs.n, runs.n_samples, row_of,
+                                static_cast<uint32_t>(P_), flags, stream),
+            "scatter rows");
+  hip_check(launch_fill_u64(chk, 2, 0, stream), "fill");
+  hip_check(launch_check_runs(runs.irr, runs.row_start, runs.row_count, runs.n, runs.n_samples, G_, irr_max_, chk,
+                              stream),
+            "check samples");
+  hip_check(hipMemcpyAsync(check_flags_, flags, 16, hipMemcpyDeviceToHost, stream), "read flags");
+  hip_check(hipMemcpyAsync(check_samples_, chk, 16, hipMemcpyDeviceToHost, stream), "read checks");
+  hip_check(hipStreamSynchronize(stream), "checks");
+  finish_device_checks();
+  stats_.input_clamps += check_samples_[1];
+  // one walk over every pixel in pixel order, exact searches only
+  CameraDev& cd = noalloc_cam_;
+  cd = CameraDev{};
+  cd.kc = kconst();
+  cd.mp = maps_view(d_maps_, P_);
+  cd.in = state_view(cur_);
+  cd.out = state_view(1 - cur_);
+  cd.runs.row_of = row_of;
+  cd.runs.row_start = runs.row_start;
+  cd.runs.row_count = runs.row_count;
+  cd.runs.t = runs.t;
+  cd.runs.irr = runs.irr;
+  cd.pixel = runs.pixel;
+  cd.ev = events;
+  cd.cap = cap;
+  cd.counters = counters;
+  cd.t_state = t_state_;
+  cd.n_runs = static_cast<int64_t>(runs.n);
+  noalloc_call_ = CallDev{t_end_ns, 0, 0};
+  hip_check(hipMemcpyAsync(d_cam, &cd, sizeof(CameraDev), hipMemcpyHostToDevice, stream), "upload camera");
+  hip_check(hipMemcpyAsync(d_call, &noalloc_call_, sizeof(CallDev), hipMemcpyHostToDevice, stream), "upload call");
+  hip_check(launch_fill_u64(counters, 1 + kStCount, 0, stream), "fill");
+  ChainArgs a{};
+  a.cams = d_cam;
+  a.n_cams = 1;
+  a.n_total = static_cast<uint32_t>(P_);
+  a.groups = G_;
+  a.instant = instant();
+  a.noise = noise_dev_.tab != nullptr && !instant();
+  a.call = d_call;
+  a.tu = chain_->tuning();
+  a.tu.chunk = 0;
+  hip_check(launch_chain_walk(a, stream), "walk");
+  hip_check(hipMemcpyAsync(counters_host_, counters, sizeof(counters_host_), hipMemcpyDeviceToHost, stream),
+            "read counters");
+  hip_check(hipStreamSynchronize(stream), "advance");
+  const unsigned long long n_events = counters_host_[0];
+  if (n_events > cap) {
+    throw std::invalid_argument("advance_device_noalloc: the call makes " + std::to_string(n_events) +
+                                " events, more than the buffer's " + std::to_string(cap) +
+                                " (the state is unchanged)");
+  }
+  ++stats_.launches;
+  commit(counters_host_, t_end_ns);
+  return n_events;
 }
 
 void SensorModel::download_events(const bsx_device_events& ev) {

@@ -96,6 +96,13 @@ class SensorModel {
   void advance_runs(const uint32_t* ids, size_t n, const int64_t* t, const float* irr, size_t n_samples,
                     const int64_t* row_start, const int32_t* row_count, int64_t t_end_ns);
   bsx_device_events advance_device(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream);
+  // advance_device for callers that cannot allocate: the call's scratch (run index, checks,
+  // counters, descriptors) lives in the library's static device scratch, the raw events go to the
+  // caller's device buffer of cap events, and nothing is allocated. The walk runs without phase A and
+  // without the heavy-first order (same results). More events than cap: std::invalid_argument, the
+  // state unchanged. Returns the event count.
+  uint64_t advance_device_noalloc(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream,
+                                  RawEventDev* events, uint64_t cap);
   void download_events(const bsx_device_events& ev);
   const HostEvents& events() const { return events_; }
 
@@ -177,6 +184,8 @@ class SensorModel {
   unsigned long long check_samples_[2] = {0, 0};
   int64_t scan_total_ = 0;
   unsigned long long counters_host_[1 + kStCount] = {};
+  CameraDev noalloc_cam_{};  // the no-allocation call's descriptor and header, copied into the scratch
+  CallDev noalloc_call_{};
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 
   int64_t t_state_ = 0;

@@ -229,6 +229,18 @@ int bsx_sensor_advance_device(bsx_sensor* s, const bsx_device_runs* runs, int64_
 /* Reference: SensorModel::download_events(events), src/sensor/sensor_model.h:167. Sorts by
  * (t_cross, pixel) and fills the host view of bsx_sensor_events. */
 int bsx_sensor_download_events(bsx_sensor* s, const bsx_device_events* events);
+/* advance_device that allocates nothing. It has no counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The call's scratch (run index, check words, counters, launch descriptors)
+ * lives in the library's static device scratch of 16 MiB + 4 KiB per GPU (held until the process
+ * ends); cameras of more than 2^22 pixels are refused (BSX_E_INVALID). The raw events (unordered, the
+ * 32-byte bsx_raw_event layout) go to the caller's device buffer `events` of `events_cap` entries;
+ * *n_events receives their count. A call that would make more than events_cap events fails with
+ * BSX_E_INVALID and leaves the model's state unchanged. Results equal bsx_sensor_advance_device's.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU (from several
+ * threads or streams at once) race. */
+int bsx_sensor_advance_device_noalloc(bsx_sensor* s, const bsx_device_runs* runs, int64_t t_end_ns, void* stream,
+                                      void* events, uint64_t events_cap, uint64_t* n_events);
 /* The host events of the last advance / advance_runs / download_events. */
 int bsx_sensor_events(bsx_sensor* s, bsx_events* out);
 
