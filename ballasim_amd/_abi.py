@@ -180,6 +180,12 @@ SIGNATURES = {
     "bsx_output_flush": (C.c_int, [P, P, P]),
     "bsx_scene_render_requests": (C.c_int, [P, C.POINTER(View), C.c_int64, P, C.c_size_t, P, C.c_int32, P, P, P, P]),
     "bsx_draw_maps": (C.c_int, [C.POINTER(Params), C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P]),
+    "bsx_sensor_create_from_constants": (C.c_int, [P, P, C.c_size_t, P, C.c_size_t, C.POINTER(P)]),
+    "bsx_sensor_upload_states": (C.c_int, [P, P, C.c_size_t]),
+    "bsx_sensor_download_states": (C.c_int, [P, P, C.c_size_t]),
+    "bsx_sensor_set_time": (C.c_int, [P, C.c_int64]),
+    "bsx_sensor_set_kernel_photometry": (C.c_int, [P, C.c_float, C.c_float, P, C.c_size_t]),
+    "bsx_sensor_raw_events": (C.c_int, [P, C.POINTER(P), C.POINTER(C.c_size_t)]),
     "bsx_noise_tables": (C.c_int, [C.POINTER(Params), P, P, C.c_size_t, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "bsx_scene_create": (C.c_int, [C.POINTER(Shape), C.c_int32, C.c_float, C.POINTER(P)]),
     "bsx_scene_destroy": (C.c_int, [P]),

@@ -206,6 +206,53 @@ int bsx_sensor_create(const bsx_params* params, uint32_t width, uint32_t height,
   });
 }
 
+int bsx_sensor_create_from_constants(const bsx_kernel_constants* kc, const bsx_pixel_map* maps, size_t n_pixels,
+                                     const float* tables, size_t n_table_values, bsx_sensor** out) {
+  static_assert(sizeof(bsx_pixel_map) == 48, "bsx_pixel_map is the reference's 48-byte PixelMap");
+  return guarded([&] {
+    need_ptr(kc, "kc");
+    need_ptr(out, "out");
+    *out = nullptr;
+    auto* m = new bsx::SensorModel(*kc, maps, n_pixels, tables, n_table_values);
+    *out = new bsx_sensor{m};
+  });
+}
+
+int bsx_sensor_upload_states(bsx_sensor* s, const void* states, size_t n_states) {
+  static_assert(sizeof(bsx::SavedPixelState) == 200, "the reference's 200-byte PixelState");
+  return guarded([&] {
+    need_ptr(states, "states");
+    model_of(s).upload_states(static_cast<const bsx::SavedPixelState*>(states), n_states);
+  });
+}
+
+int bsx_sensor_download_states(bsx_sensor* s, void* states, size_t n_states) {
+  return guarded([&] {
+    need_ptr(states, "states");
+    model_of(s).download_states(static_cast<bsx::SavedPixelState*>(states), n_states);
+  });
+}
+
+int bsx_sensor_set_time(bsx_sensor* s, int64_t t_state_ns) {
+  return guarded([&] { model_of(s).set_time(t_state_ns); });
+}
+
+int bsx_sensor_set_kernel_photometry(bsx_sensor* s, float scale, float irr_max, const bsx_pixel_map* maps,
+                                     size_t n_pixels) {
+  return guarded([&] { model_of(s).set_kernel_photometry(scale, irr_max, maps, n_pixels); });
+}
+
+int bsx_sensor_raw_events(bsx_sensor* s, const bsx_raw_event** out, size_t* count) {
+  static_assert(sizeof(bsx_raw_event) == sizeof(bsx::RawEventDev), "raw event layout");
+  return guarded([&] {
+    need_ptr(out, "out");
+    need_ptr(count, "count");
+    const std::vector<bsx::RawEventDev>& r = model_of(s).raw_events();
+    *out = reinterpret_cast<const bsx_raw_event*>(r.data());
+    *count = r.size();
+  });
+}
+
 int bsx_sensor_destroy(bsx_sensor* s) {
   return guarded([&] {
     if (s == nullptr) return;

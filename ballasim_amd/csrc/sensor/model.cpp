@@ -99,10 +99,7 @@ SensorModel::SensorModel(const SensorParams& params, uint32_t width, uint32_t he
                                   "backend");
     }
   }
-  int n_dev = 0;
-  if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= 0) {
-    throw std::runtime_error("SensorModel: no HIP device available");
-  }
+  require_device();
   G_ = static_cast<int>(groups_.size());
   P_ = static_cast<size_t>(width) * height;
   maps_ = draw_sensor_maps(params_, seed_, camera_, static_cast<uint32_t>(P_));
@@ -116,12 +113,10 @@ SensorModel::SensorModel(const SensorParams& params, uint32_t width, uint32_t he
   }
   readout_ = readout_of(params_, seed_, camera_);
   irr_max_ = static_cast<float>(kMaxSamplePhotocurrent_fA / (params_.illuminance_scale * gain_max_));
-  size_t state_bytes = 0;
-  state_bytes += 9 * align256(P_ * 4) + 4 * align256(P_ * 8) + align256(P_ * 4) + align256(P_ * 2);
-  state_bytes += align256(P_ * 4 * G_);
-  if (noise_on(params_)) {
-    // nk[3], nk1[3], ln_a1 (fp32), cell (int64), have_k1 (byte)
-    state_bytes += 7 * align256(P_ * 4) + align256(P_ * 8) + align256(P_);
+  scale32_ = static_cast<float>(params_.illuminance_scale);
+  kc_ = constants_of_params();
+  noise_ = noise_on(params_);
+  if (noise_) {
     NoiseModelParams nm{};
     nm.kq = params_.k * kElementaryCharge;
     nm.a2 = 1.0 / params_.tau_o;
@@ -129,18 +124,98 @@ SensorModel::SensorModel(const SensorParams& params, uint32_t width, uint32_t he
     nm.sigma_o = params_.sigma_o;
     nm.levels = params_.noise_leaf_level;
     const NoiseTableSet tabs = build_noise_tables(nm);
-    d_noise_tab_.ensure(tabs.data32.size() * sizeof(float));
-    hip_check(hipMemcpy(d_noise_tab_.get(), tabs.data32.data(), tabs.data32.size() * sizeof(float),
-                        hipMemcpyHostToDevice),
-              "upload noise tables");
-    noise_dev_.tab = d_noise_tab_.as<const float>();
-    noise_dev_.levels = tabs.levels;
-    noise_dev_.n_nodes = tabs.n_nodes;
-    noise_dev_.stride = tabs.stride;
-    noise_dev_.ln_min = static_cast<float>(kNoiseLnA1Min);
-    noise_dev_.inv_dln = static_cast<float>(tabs.inv_dln);
-    noise_dev_.margin_z = static_cast<float>(params_.noise_margin_z);
+    upload_noise_tables(tabs.data32.data(), tabs.data32.size(), tabs.levels, tabs.n_nodes, tabs.stride,
+                        static_cast<float>(kNoiseLnA1Min), static_cast<float>(tabs.inv_dln),
+                        static_cast<float>(params_.noise_margin_z));
   }
+  allocate();
+}
+
+SensorModel::SensorModel(const bsx_kernel_constants& kc, const bsx_pixel_map* maps, size_t n_pixels,
+                         const float* tables, size_t n_table_values)
+    : params_(preset("prophesee_gen3")), width_(0), height_(1), seed_(kc.seed), camera_(kc.camera) {
+  // The reference's SensorModel built these from its parameters and checked them; here the features
+  // this backend does not run are refused by name.
+  if (kc.modulated || kc.breakpoints) {
+    throw std::invalid_argument("CudaSensorKernel (bsx): modulated light groups are not supported by the bsx backend");
+  }
+  if (kc.transient) {
+    throw std::invalid_argument("CudaSensorKernel (bsx): noise_diffusion transient is not supported by the bsx backend");
+  }
+  if (kc.hpf) throw std::invalid_argument("CudaSensorKernel (bsx): the high-pass filter is not supported by the bsx backend");
+  if (kc.groups < 1 || kc.groups > kMaxGroups) throw std::invalid_argument("CudaSensorKernel (bsx): need 1 to 8 light groups");
+  if (kc.camera > 255) throw std::invalid_argument("CudaSensorKernel (bsx): camera index must be < 256");
+  if (n_pixels == 0 || n_pixels > 0xffffffffull || (n_pixels > 0 && maps == nullptr)) {
+    throw std::invalid_argument("CudaSensorKernel (bsx): need one PixelMap per pixel");
+  }
+  require_device();
+  from_constants_ = true;
+  G_ = kc.groups;
+  groups_.assign(static_cast<size_t>(G_), bsx_waveform{});
+  P_ = n_pixels;
+  width_ = static_cast<uint32_t>(n_pixels);
+  set_kernel_maps(maps, n_pixels);
+  irr_max_ = kc.irr_max;
+  scale32_ = kc.scale;
+  KConst k{};
+  k.groups = kc.groups;
+  k.instant = kc.instant ? 1 : 0;
+  k.ideal_reset = kc.ideal_reset ? 1 : 0;
+  k.n_pixels = static_cast<int32_t>(P_);
+  k.refractory_ns = kc.refractory_ns;
+  k.eps_ci = kc.eps_ci;
+  k.label_cut = kc.label_cut;
+  k.k = kc.k;
+  k.a2 = kc.a2;
+  k.a3 = kc.a3;
+  k.tau_o = kc.tau_o;
+  k.tau_sf = kc.tau_sf;
+  k.eps_accept = kc.eps_accept;
+  k.eps_drop = kc.eps_drop;
+  k.eps_pl_base = kc.eps_pl_base;
+  k.shot_rate_hz = 0.0;
+  k.seed = kc.seed;
+  k.camera = kc.camera;
+  k.width = width_;
+  kc_ = k;
+  noise_ = kc.noise != 0 && !kc.instant;
+  if (noise_) {
+    if (kc.levels < 0 || kc.levels > kNoiseMaxLevels || kc.n_nodes < 2 || kc.stride != noise_stride(kc.levels) ||
+        tables == nullptr || n_table_values != static_cast<size_t>(kc.n_nodes) * static_cast<size_t>(kc.stride)) {
+      throw std::invalid_argument("CudaSensorKernel (bsx): the noise tables do not match their layout");
+    }
+    upload_noise_tables(tables, n_table_values, kc.levels, kc.n_nodes, kc.stride, kc.ln_a1_min, kc.inv_dln,
+                        kc.margin_z);
+  }
+  allocate();
+}
+
+void SensorModel::require_device() {
+  int n_dev = 0;
+  if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= 0) {
+    throw std::runtime_error("SensorModel: no HIP device available");
+  }
+}
+
+void SensorModel::upload_noise_tables(const float* data, size_t n, int levels, int n_nodes, int stride, float ln_min,
+                                      float inv_dln, float margin_z) {
+  d_noise_tab_.ensure(n * sizeof(float));
+  hip_check(hipMemcpy(d_noise_tab_.get(), data, n * sizeof(float), hipMemcpyHostToDevice), "upload noise tables");
+  noise_dev_.tab = d_noise_tab_.as<const float>();
+  noise_dev_.levels = levels;
+  noise_dev_.n_nodes = n_nodes;
+  noise_dev_.stride = stride;
+  noise_dev_.ln_min = ln_min;
+  noise_dev_.inv_dln = inv_dln;
+  noise_dev_.margin_z = margin_z;
+}
+
+void SensorModel::allocate() {
+  size_t state_bytes = 0;
+  state_bytes += 9 * align256(P_ * 4) + 4 * align256(P_ * 8) + align256(P_ * 4) + align256(P_ * 2);
+  state_bytes += align256(P_ * 4 * G_);
+  // the in-path noise: nk[3], nk1[3], ln_a1 (fp32), cell (int64), have_k1 (byte)
+  if (noise_) state_bytes += 7 * align256(P_ * 4) + align256(P_ * 8) + align256(P_);
   d_state_[0].ensure(state_bytes);
   d_state_[1].ensure(state_bytes);
   upload_maps();
@@ -151,12 +226,60 @@ SensorModel::SensorModel(const SensorParams& params, uint32_t width, uint32_t he
   chain_ = std::make_unique<LaunchChain>();
 }
 
+void SensorModel::set_kernel_maps(const bsx_pixel_map* maps, size_t n) {
+  maps_ = SensorMaps{};
+  maps_.theta_on.resize(n);
+  maps_.theta_off.resize(n);
+  maps_.leak.resize(n);
+  maps_.dark_fA.resize(n);
+  maps_.phi0.resize(n);
+  maps_.hot.resize(n);
+  maps_.hot_rate_hz.assign(n, 0.0);
+  maps_.gain.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const bsx_pixel_map& m = maps[i];
+    if (m.hot > 1) {
+      throw std::invalid_argument("CudaSensorKernel (bsx): threshold-type hot pixels are not supported by the bsx "
+                                  "backend");
+    }
+    maps_.theta_on[i] = m.theta_on;
+    maps_.theta_off[i] = m.theta_off;
+    maps_.leak[i] = m.leak;
+    maps_.dark_fA[i] = m.dark_fA;
+    maps_.phi0[i] = m.phi0;
+    maps_.hot[i] = m.hot;
+    maps_.gain[i] = m.gain;
+  }
+}
+
+void SensorModel::set_kernel_photometry(float scale, float irr_max, const bsx_pixel_map* maps, size_t n) {
+  if (!from_constants_) throw std::logic_error("set_kernel_photometry: the model was built from parameters");
+  if (n != P_ || maps == nullptr) throw std::invalid_argument("set_kernel_photometry: one PixelMap per pixel");
+  set_kernel_maps(maps, n);
+  scale32_ = scale;
+  irr_max_ = irr_max;
+  upload_maps();
+}
+
+void SensorModel::set_time(int64_t t_state_ns) {
+  if (!states_set_) throw std::logic_error("SensorModel.set_time: upload the states first");
+  t_state_ = t_state_ns;
+  initialized_ = true;
+}
+
 SensorModel::~SensorModel() {
   if (ev0_ != nullptr) (void)hipEventDestroy(ev0_);
   if (ev1_ != nullptr) (void)hipEventDestroy(ev1_);
 }
 
 KConst SensorModel::kconst() const {
+  KConst kc = kc_;
+  kc.irr_max = irr_max_;
+  kc.noise = noise_dev_;
+  return kc;
+}
+
+KConst SensorModel::constants_of_params() const {
   KConst kc{};
   kc.groups = G_;
   kc.instant = params_.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS ? 1 : 0;
@@ -180,7 +303,6 @@ KConst SensorModel::kconst() const {
   kc.seed = seed_;
   kc.camera = camera_;
   kc.width = width_;
-  kc.noise = noise_dev_;
   return kc;
 }
 
@@ -188,7 +310,7 @@ void SensorModel::upload_maps() {
   const size_t a = align256(P_ * 4);
   d_maps_.ensure(7 * a);
   std::vector<float> buf(P_);
-  const float scale32 = static_cast<float>(params_.illuminance_scale);
+  const float scale32 = scale32_;
   const auto put = [&](int slot, const auto& fn) {
     for (size_t i = 0; i < P_; ++i) buf[i] = fn(i);
     hip_check(hipMemcpy(static_cast<char*>(d_maps_.get()) + slot * a, buf.data(), P_ * 4, hipMemcpyHostToDevice),
@@ -227,7 +349,7 @@ StateDev SensorModel::state_view(int which) const {
   b += align256(P_ * 2);
   s.held = reinterpret_cast<float*>(b);
   b += align256(P_ * 4 * G_);
-  if (noise_on(params_)) {
+  if (noise_) {
     s.nk = reinterpret_cast<float*>(b);
     b += 3 * f4;
     s.nk1 = reinterpret_cast<float*>(b);
@@ -256,6 +378,7 @@ static MapsDev maps_view(const DeviceBuffer& d, size_t P) {
 }
 
 void SensorModel::set_photometry(double illuminance_scale, const float* gain, size_t n_gain) {
+  if (from_constants_) throw std::logic_error("SensorModel::set_photometry: the model was built from kernel constants");
   if (gain != nullptr && n_gain != P_) {
     throw std::invalid_argument("SensorModel::set_photometry: one gain per pixel of the frame (or none)");
   }
@@ -273,6 +396,7 @@ void SensorModel::set_photometry(double illuminance_scale, const float* gain, si
   gain_max_ = g_max;
   for (size_t i = 0; i < P_; ++i) maps_.gain[i] = gain == nullptr ? 1.0f : gain[i];
   irr_max_ = static_cast<float>(kMaxSamplePhotocurrent_fA / (params_.illuminance_scale * gain_max_));
+  scale32_ = static_cast<float>(params_.illuminance_scale);
   upload_maps();
 }
 
@@ -638,7 +762,8 @@ void SensorModel::download_events(const bsx_device_events& ev) {
 
 void SensorModel::finish_host_events(const RawEventDev* raw, size_t n) {
   // (t_cross, pixel) is unique: a pixel never fires twice in one ns.
-  std::vector<RawEventDev> all(raw, raw + n);
+  std::vector<RawEventDev>& all = raw_sorted_;
+  all.assign(raw, raw + n);
   std::sort(all.begin(), all.end(), [](const RawEventDev& a, const RawEventDev& b) {
     return a.t_cross != b.t_cross ? a.t_cross < b.t_cross : a.pixel < b.pixel;
   });
@@ -661,28 +786,15 @@ void SensorModel::finish_host_events(const RawEventDev* raw, size_t n) {
 
 size_t SensorModel::state_bytes() const { return sizeof(StateHeader) + P_ * sizeof(SavedPixelState); }
 
-void SensorModel::save_state(uint8_t* out, size_t n) const {
-  if (n < state_bytes()) throw std::invalid_argument("SensorModel.save_state: buffer too small");
-  StateHeader h{};
-  h.magic = kStateMagic;
-  h.version = kStateVersion;
-  h.width = width_;
-  h.height = height_;
-  h.groups = static_cast<uint32_t>(G_);
-  h.camera = camera_;
-  h.seed = seed_;
-  h.fingerprint = fingerprint(params_, groups_, maps_.gain);
-  h.t_state = t_state_;
-  h.initialized = initialized_ ? 1 : 0;
-  h.state_size = sizeof(SavedPixelState);
-  std::memcpy(out, &h, sizeof(h));
+void SensorModel::download_states(SavedPixelState* out, size_t n) const {
+  if (n != P_) throw std::invalid_argument("SensorModel.download_states: one state per pixel");
   const StateDev s = state_view(cur_);
   std::vector<float> f(P_);
   std::vector<int64_t> q(P_);
   std::vector<uint32_t> u(P_);
   std::vector<uint16_t> fl(P_);
-  std::vector<SavedPixelState> ps(P_);
-  std::memset(ps.data(), 0, P_ * sizeof(SavedPixelState));
+  SavedPixelState* ps = out;
+  std::memset(static_cast<void*>(ps), 0, P_ * sizeof(SavedPixelState));
   const auto getf = [&](const float* src, double SavedPixelState::*dst) {
     hip_check(hipMemcpy(f.data(), src, P_ * 4, hipMemcpyDeviceToHost), "download state");
     for (size_t i = 0; i < P_; ++i) ps[i].*dst = static_cast<double>(f[i]);
@@ -703,7 +815,7 @@ void SensorModel::save_state(uint8_t* out, size_t n) const {
   geti(s.t_ref, &SavedPixelState::t_ref);
   geti(s.blind_until, &SavedPixelState::blind_until);
   geti(s.t_search, &SavedPixelState::t_search);
-  if (noise_on(params_)) {
+  if (noise_) {
     // the in-path noise: the nodes, the cell's rate, the cell and whether its end node is drawn
     for (int c = 0; c < 3; ++c) {
       hip_check(hipMemcpy(f.data(), s.nk + static_cast<size_t>(c) * P_, P_ * 4, hipMemcpyDeviceToHost),
@@ -718,9 +830,9 @@ void SensorModel::save_state(uint8_t* out, size_t n) const {
     std::vector<uint8_t> hk(P_);
     hip_check(hipMemcpy(hk.data(), s.have_k1, P_, hipMemcpyDeviceToHost), "download state");
     for (size_t i = 0; i < P_; ++i) ps[i].have_k1 = hk[i];
-  } else {
+  } else if (kc_.shot_rate_hz > 0.0) {
     // The shot-noise process's next arrival and index ride in fields the noise-free kernel leaves
-    // unused (cell and nk[0]).
+    // unused (cell and nk[0]); without the process they stay 0, as the reference leaves them.
     geti(s.shot_next, &SavedPixelState::cell);
     hip_check(hipMemcpy(u.data(), s.shot_index, P_ * 4, hipMemcpyDeviceToHost), "download state");
     for (size_t i = 0; i < P_; ++i) ps[i].nk[0] = static_cast<double>(u[i]);
@@ -732,23 +844,33 @@ void SensorModel::save_state(uint8_t* out, size_t n) const {
               "download state");
     for (size_t i = 0; i < P_; ++i) ps[i].held[g] = f[i];
   }
+}
+
+void SensorModel::save_state(uint8_t* out, size_t n) const {
+  if (from_constants_) throw std::logic_error("SensorModel.save_state: a model built from kernel constants moves its "
+                                              "states with download_states");
+  if (n < state_bytes()) throw std::invalid_argument("SensorModel.save_state: buffer too small");
+  StateHeader h{};
+  h.magic = kStateMagic;
+  h.version = kStateVersion;
+  h.width = width_;
+  h.height = height_;
+  h.groups = static_cast<uint32_t>(G_);
+  h.camera = camera_;
+  h.seed = seed_;
+  h.fingerprint = fingerprint(params_, groups_, maps_.gain);
+  h.t_state = t_state_;
+  h.initialized = initialized_ ? 1 : 0;
+  h.state_size = sizeof(SavedPixelState);
+  std::memcpy(out, &h, sizeof(h));
+  std::vector<SavedPixelState> ps(P_);
+  download_states(ps.data(), P_);
   std::memcpy(out + sizeof(h), ps.data(), P_ * sizeof(SavedPixelState));
 }
 
-void SensorModel::load_state(const uint8_t* data, size_t n) {
-  if (n != state_bytes()) throw std::invalid_argument("SensorModel.load_state: size does not match this model");
-  StateHeader h{};
-  std::memcpy(&h, data, sizeof(h));
-  if (h.magic != kStateMagic || h.version != kStateVersion || h.state_size != sizeof(SavedPixelState)) {
-    throw std::invalid_argument("SensorModel.load_state: not a sensor state of this version");
-  }
-  if (h.width != width_ || h.height != height_ || h.groups != static_cast<uint32_t>(G_) || h.camera != camera_ ||
-      h.seed != seed_ || h.fingerprint != fingerprint(params_, groups_, maps_.gain)) {
-    throw std::invalid_argument("SensorModel.load_state: state was saved by a model with other parameters");
-  }
+void SensorModel::upload_states(const SavedPixelState* ps, size_t n) {
+  if (n != P_) throw std::invalid_argument("SensorModel.upload_states: one state per pixel");
   initialized_ = false;
-  std::vector<SavedPixelState> ps(P_);
-  std::memcpy(ps.data(), data + sizeof(h), P_ * sizeof(SavedPixelState));
   const StateDev s = state_view(cur_);
   std::vector<float> f(P_);
   std::vector<int64_t> q(P_);
@@ -774,7 +896,7 @@ void SensorModel::load_state(const uint8_t* data, size_t n) {
   puti(s.t_ref, &SavedPixelState::t_ref);
   puti(s.blind_until, &SavedPixelState::blind_until);
   puti(s.t_search, &SavedPixelState::t_search);
-  if (noise_on(params_)) {
+  if (noise_) {
     for (int c = 0; c < 3; ++c) {
       for (size_t i = 0; i < P_; ++i) f[i] = static_cast<float>(ps[i].nk[c]);
       hip_check(hipMemcpy(s.nk + static_cast<size_t>(c) * P_, f.data(), P_ * 4, hipMemcpyHostToDevice),
@@ -788,13 +910,15 @@ void SensorModel::load_state(const uint8_t* data, size_t n) {
     std::vector<uint8_t> hk(P_);
     for (size_t i = 0; i < P_; ++i) hk[i] = ps[i].have_k1;
     hip_check(hipMemcpy(s.have_k1, hk.data(), P_, hipMemcpyHostToDevice), "upload state");
-    // no shot process with the in-path noise
+  }
+  if (kc_.shot_rate_hz > 0.0 && !noise_) {
+    puti(s.shot_next, &SavedPixelState::cell);
+    for (size_t i = 0; i < P_; ++i) u[i] = static_cast<uint32_t>(ps[i].nk[0]);
+  } else {
+    // no shot process: no arrival
     for (size_t i = 0; i < P_; ++i) q[i] = INT64_MAX;
     hip_check(hipMemcpy(s.shot_next, q.data(), P_ * 8, hipMemcpyHostToDevice), "upload state");
     std::fill(u.begin(), u.end(), 0u);
-  } else {
-    puti(s.shot_next, &SavedPixelState::cell);
-    for (size_t i = 0; i < P_; ++i) u[i] = static_cast<uint32_t>(ps[i].nk[0]);
   }
   hip_check(hipMemcpy(s.shot_index, u.data(), P_ * 4, hipMemcpyHostToDevice), "upload state");
   for (size_t i = 0; i < P_; ++i) fl[i] = ps[i].flags;
@@ -804,6 +928,26 @@ void SensorModel::load_state(const uint8_t* data, size_t n) {
     hip_check(hipMemcpy(s.held + static_cast<size_t>(g) * P_, f.data(), P_ * 4, hipMemcpyHostToDevice),
               "upload state");
   }
+  states_set_ = true;
+}
+
+void SensorModel::load_state(const uint8_t* data, size_t n) {
+  if (from_constants_) throw std::logic_error("SensorModel.load_state: a model built from kernel constants moves its "
+                                              "states with upload_states");
+  if (n != state_bytes()) throw std::invalid_argument("SensorModel.load_state: size does not match this model");
+  StateHeader h{};
+  std::memcpy(&h, data, sizeof(h));
+  if (h.magic != kStateMagic || h.version != kStateVersion || h.state_size != sizeof(SavedPixelState)) {
+    throw std::invalid_argument("SensorModel.load_state: not a sensor state of this version");
+  }
+  if (h.width != width_ || h.height != height_ || h.groups != static_cast<uint32_t>(G_) || h.camera != camera_ ||
+      h.seed != seed_ || h.fingerprint != fingerprint(params_, groups_, maps_.gain)) {
+    throw std::invalid_argument("SensorModel.load_state: state was saved by a model with other parameters");
+  }
+  initialized_ = false;
+  std::vector<SavedPixelState> ps(P_);
+  std::memcpy(ps.data(), data + sizeof(h), P_ * sizeof(SavedPixelState));
+  upload_states(ps.data(), P_);
   t_state_ = h.t_state;
   initialized_ = h.initialized != 0;
 }

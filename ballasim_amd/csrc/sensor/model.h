@@ -85,6 +85,11 @@ class SensorModel {
  public:
   SensorModel(const SensorParams& params, uint32_t width, uint32_t height, uint64_t seed,
               std::vector<bsx_waveform> groups, uint32_t camera);
+  // The reference's backend seam (CudaSensorKernel(kc, maps, tables), src/sensor/sensor_cuda.h:22-23):
+  // a model of n_pixels pixels from the reference's fp32 kernel constants, its per-pixel maps and its
+  // fp32 noise tables. Its states come from upload_states; set_time gives each call's model time.
+  SensorModel(const bsx_kernel_constants& kc, const bsx_pixel_map* maps, size_t n_pixels, const float* tables,
+              size_t n_table_values);
   ~SensorModel();
   SensorModel(const SensorModel&) = delete;
   SensorModel& operator=(const SensorModel&) = delete;
@@ -109,6 +114,14 @@ class SensorModel {
   size_t state_bytes() const;
   void save_state(uint8_t* out, size_t n) const;
   void load_state(const uint8_t* data, size_t n);
+  // The reference's per-pixel states (200 B each) to and from the device's working state.
+  void upload_states(const SavedPixelState* states, size_t n);
+  void download_states(SavedPixelState* out, size_t n) const;
+  // The seam's calls: the model time of the next call (after upload_states), and new photometry.
+  void set_time(int64_t t_state_ns);
+  void set_kernel_photometry(float scale, float irr_max, const bsx_pixel_map* maps, size_t n);
+  // The last host call's raw events, sorted by (t_cross, pixel).
+  const std::vector<RawEventDev>& raw_events() const { return raw_sorted_; }
 
   const bsx_stats& stats() const { return stats_; }
   void reset_stats();
@@ -149,7 +162,13 @@ class SensorModel {
                    bool scanned);
   void finish_host_events(const RawEventDev* raw, size_t n);
   KConst kconst() const;
-  bool instant() const { return params_.photoreceptor == BSX_PHOTORECEPTOR_INSTANTANEOUS; }
+  KConst constants_of_params() const;
+  static void require_device();
+  void upload_noise_tables(const float* data, size_t n, int levels, int n_nodes, int stride, float ln_min,
+                           float inv_dln, float margin_z);
+  void allocate();
+  void set_kernel_maps(const bsx_pixel_map* maps, size_t n);
+  bool instant() const { return kc_.instant != 0; }
 
   SensorParams params_;
   uint32_t width_, height_;
@@ -162,6 +181,11 @@ class SensorModel {
   Readout readout_{};
   double gain_max_ = 1.0;
   float irr_max_ = 0.0f;
+  float scale32_ = 1.0f;       // illuminance_scale in fp32 (the maps' scale is scale32_ x gain)
+  KConst kc_{};                // the call constants (irr_max and the noise view added by kconst())
+  bool noise_ = false;         // the cascade's in-path noise
+  bool from_constants_ = false;  // built from the reference's kernel constants (the backend seam)
+  bool states_set_ = false;
 
   DeviceBuffer d_maps_;       // 6 fp32 arrays of P
   DeviceBuffer d_noise_tab_;  // the in-path noise's fp32 tables (noise on)
@@ -193,6 +217,7 @@ class SensorModel {
   bsx_stats stats_{};
   HostEvents events_;
   std::vector<RawEventDev> raw_host_;
+  std::vector<RawEventDev> raw_sorted_;
 };
 
 }  // namespace bsx

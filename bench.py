@@ -5,12 +5,17 @@ with affine motion (velocity U(+-2000 px/s), rotation U(+-3 rad/s), contrast U[0
 log-uniform 10-1000 lux background, for 500 ms, sampled by the plain error-tolerance controller
 (refine_tol_log 0.01) on the GPU. Thresholds 0.15 +- 0.03, refractory 50 us, leak 0.05 Hz/px, shot
 noise 0.1 Hz/px (Philox-keyed Poisson arrivals per pixel).
-A step is one advance of every pixel over one 2^23 ns window; all windows' runs are generated on the
-device and resident in HBM before the timed region. value = pixel-samples advanced per second.
+A step is three advances of every pixel over consecutive 2^23 ns windows (--windows-per-step), so the
+default 20 steps cover the whole 500-ms run from t = 0 (60 windows, 503 ms); warm-up and the
+autotuner run on a separate sensor first. All windows' runs are generated on the device and resident
+in HBM before the timed region. value = pixel-samples advanced per second. The same line carries the
+cascade photoreceptor's figures on the first windows of the same runs, without noise and with the
+reference's in-path noise ("cascade", "cascade_noise").
 
 The JSON line also carries the roofline of the advance kernel (the fixed algorithmic bytes per
-launch over its HIP-event time, against 8 TB/s) and the reference's CPU path timed on this host's
-cores on a bounded sample of the same runs.
+launch over its HIP-event time, against 8 TB/s; the measured HBM traffic comes from separate
+rocprofv3 counter runs, so "traffic" is null here) and the reference's CPU path timed on every core
+this process may use, on a bounded sample of the same runs.
 
 --workload rig: the ragged batch. A 4-camera rig (1280x720, 640x480, 346x260, 1920x1080, each with
 its own offset in the scene plane) watching the 346x260 scene of one textured disk (100 lux
@@ -40,26 +45,11 @@ ADVANCE_CHAIN = {
     "instantaneous": "advance chain: chain_init_kernel + chain_prep_kernel + order_count/offsets/scatter + instant_walk_kernel",
     "cascade": "advance chain: chain_init_kernel + chain_cost_kernel + order_count/offsets/scatter + cascade_walk_kernel",
 }
-# Measured HBM traffic of the instantaneous chain on this workload, bytes per sample: rocprofv3
-# FETCH_SIZE x 2 + WRITE_SIZE summed over the chain's kernels, averaged over the timed launches (the
-# factor 2 and 1 from a 1 GiB device copy read by the same counters: profiles/advance_chain_1280x720.md).
-TRAFFIC_BYTES_PER_SAMPLE = 59.3
 
 
-def sensor_params(bsx, photoreceptor):
-    p = bsx.preset("prophesee_gen3")
-    p.photoreceptor = photoreceptor
-    p.noise = False
-    p.theta_on = p.theta_off = 0.15
-    p.sigma_theta_on = p.sigma_theta_off = 0.03
-    p.refractory_s = 50e-6
-    p.leak_rate_hz = 0.05
-    # shot noise 0.1 Hz/px: the library's rate-specified Philox process (the instantaneous response
-    # only; the cascade runs noise-free)
-    p.shot_rate_hz = 0.1 if photoreceptor == "instantaneous" else 0.0
-    from ballasim_amd.scene import LUX_TO_FA
-    p.illuminance_scale = LUX_TO_FA
-    return p
+def sensor_params(bsx, photoreceptor, noise=False):
+    from ballasim_amd.configs import config2_params
+    return config2_params(photoreceptor, noise)
 
 
 def algorithmic_bytes(samples, n_listed, n_pixels, events, groups=1):
@@ -72,7 +62,7 @@ def cpu_baseline(p, window_runs, frame, width, height, budget_s):
     (window_runs yields each window's runs copied to the host)."""
     import numpy as np
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = len(os.sched_getaffinity(0))
     kind = "reference" if O.ref_available() else "port"
     m = (O.RefModel(p.c, width, height, 1, 1, 0, threads, True) if kind == "reference"
          else O.OracleModel(p.c, width, height, 1, 1, 0, threads))
@@ -88,23 +78,15 @@ def cpu_baseline(p, window_runs, frame, width, height, budget_s):
         if elapsed >= budget_s:
             break
     return {"value": samples / elapsed, "unit": "pixel-samples/s", "cores": threads, "kind": kind,
+            "host_cpus": os.cpu_count(),
             "sample": f"the first {windows} windows ({windows * WINDOW / 1e6:.1f} ms) of the same runs, "
                       f"{samples} samples, {events} events, {elapsed:.2f} s",
             "events_per_s": events / elapsed}
 
 
 def rig_params(bsx, photoreceptor):
-    p = bsx.preset("prophesee_gen3")
-    p.photoreceptor = photoreceptor
-    p.noise = False
-    p.theta_on = p.theta_off = 0.2
-    p.sigma_theta_on = p.sigma_theta_off = 0.02
-    p.refractory_s = 100e-6
-    p.leak_rate_hz = 0.0
-    p.shot_rate_hz = 0.0
-    from ballasim_amd.scene import LUX_TO_FA
-    p.illuminance_scale = LUX_TO_FA
-    return p
+    from ballasim_amd.configs import config3_params
+    return config3_params(photoreceptor)
 
 
 def bench_rig(args):
@@ -219,7 +201,7 @@ def cpu_baseline_rig(p, runs, views, scene, budget_s):
     """The CPU path on the rig's cameras in turn (window by window) until budget_s of CPU work."""
     from ballasim_amd import scene as S
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = len(os.sched_getaffinity(0))
     kind = "reference" if O.ref_available() else "port"
     models = []
     for c, v in enumerate(views):
@@ -391,9 +373,16 @@ def X_rows_per_s(ms, rows):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed windows (default 58; rig: 8)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; rig: 8 windows)")
+    ap.add_argument("--windows-per-step", type=int, default=3,
+                    help="plain: 2^23-ns windows per step (default 3: 20 steps cover the 500-ms run)")
+    ap.add_argument("--cascade-windows", type=int, default=3,
+                    help="plain: windows timed with the cascade photoreceptor, without and with its noise (0: skip)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--photoreceptor", default="instantaneous", choices=["instantaneous", "cascade"])
+    ap.add_argument("--noise", action="store_true",
+                    help="plain, cascade: the reference's in-path noise on (the instantaneous response carries the "
+                         "shot process instead)")
     ap.add_argument("--workload", default="plain", choices=["plain", "rig", "learned"],
                     help="plain: the 1280x720 sensor (default); rig: the 4-camera ragged batch; learned: the "
                          "proposer's training and the learned controller against the plain one")
@@ -422,27 +411,52 @@ def main():
                                    "(pixels shard by tile or camera)", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.steps is None:
-        args.steps = 8 if args.workload == "rig" else 58
+        args.steps = 8 if args.workload == "rig" else 20
     if args.workload == "rig":
         return bench_rig(args)
     if args.workload == "learned":
         return bench_learned(args)
+    return bench_plain(args)
+
+
+def time_windows(sensor, runs, totals, first, count, tuning=None):
+    """Advances a fresh-initialised sensor over windows [first, first + count) (a rig of one camera: one
+    graph-captured launch chain per window); (elapsed s, samples, events per window, stats with kernel_ms)."""
+    from ballasim_amd.sensor import Rig
+    rig = Rig([sensor])
+    if tuning is not None:
+        rig.set_tuning(**tuning)
+    sensor.reset_stats()
+    k0 = rig.stats()["kernel_ms"]
+    ev_counts = []
+    t0 = time.perf_counter()
+    for i in range(first, first + count):
+        (_, n_ev), = rig.advance_device([runs[i]], (i + 1) * WINDOW)
+        ev_counts.append(n_ev)
+    elapsed = time.perf_counter() - t0  # a rig call returns after its counters are read back (synchronized)
+    st = dict(sensor.stats())
+    st["kernel_ms"] = rig.stats()["kernel_ms"] - k0
+    rig.close()
+    return elapsed, sum(totals[first:first + count]), ev_counts, st
+
+
+def bench_plain(args):
     import numpy as np
 
     import ballasim_amd as bsx
     from ballasim_amd import scene as S
+    from ballasim_amd.sensor import Rig, SensorModel
 
     scene, view = S.config2_scene(args.seed)
     W, H = view.width, view.height
     P = W * H
-    p = sensor_params(bsx, args.photoreceptor)
-    from ballasim_amd.sensor import SensorModel
-    sensor = SensorModel(p, W, H, 1, ["constant"], 0, device="cuda")
+    p = sensor_params(bsx, args.photoreceptor, args.noise)
     frame = scene.render(view, 0)
-    sensor.init(0, frame.reshape(-1, 1))
     dark = float(p.dark_current_fA) / float(p.illuminance_scale)
     rules = S.rules(dark, tol=0.01)
-    n_win = args.warmup + args.steps
+    wps = args.windows_per_step
+    n_timed = args.steps * wps
+    n_win = max(n_timed, args.warmup * wps, 1)
     t_gen = time.perf_counter()
     runs, totals = [], []
     for i in range(n_win):
@@ -451,30 +465,29 @@ def main():
         totals.append(total)
     gen_s = time.perf_counter() - t_gen
     print(f"# generated {n_win} windows, {sum(totals)} samples in {gen_s:.1f} s", file=sys.stderr, flush=True)
-    # The walk's launch shape from the autotuner, timed on the first timed window's input after the warm-up
-    # windows (the camera as a rig of one; the autotuner changes no state).
-    from ballasim_amd.sensor import Rig
-    rig = Rig([sensor])
-    for i in range(args.warmup):
+
+    def fresh(params):
+        m = SensorModel(params, W, H, 1, ["constant"], 0, device="cuda")
+        m.init(0, frame.reshape(-1, 1))
+        return m
+
+    # Warm-up and the autotuner on a separate sensor over the first windows; the timed sensor then runs
+    # windows 0 .. n_timed - 1 from its own init, so the timed steps cover the run from its start.
+    warm = fresh(p)
+    rig = Rig([warm])
+    for i in range(args.warmup * wps):
         rig.advance_device([runs[i]], (i + 1) * WINDOW)
-    trials = rig.autotune([runs[args.warmup]], (args.warmup + 1) * WINDOW, reps=2) if args.autotune else []
+    k = args.warmup * wps
+    trials = rig.autotune([runs[k]], (k + 1) * WINDOW, reps=2) if args.autotune and k < n_win else []
     tuning = rig.tuning()
-    sensor.reset_stats()
-    k0 = rig.stats()["kernel_ms"]
-    ev_counts = []
-    t0 = time.perf_counter()
-    for i in range(args.warmup, n_win):
-        (_, count), = rig.advance_device([runs[i]], (i + 1) * WINDOW)
-        ev_counts.append(count)
-    t1 = time.perf_counter()  # a rig call returns after its graph launch and counter read-back (synchronized)
-    st = dict(sensor.stats())
-    st["kernel_ms"] = rig.stats()["kernel_ms"] - k0
-    elapsed = t1 - t0
-    samples = sum(totals[args.warmup:])
+    rig.close()
+    del warm
+    sensor = fresh(p)
+    elapsed, samples, ev_counts, st = time_windows(sensor, runs, totals, 0, n_timed, tuning)
+    print(f"# timed {n_timed} windows in {elapsed:.3f} s", file=sys.stderr, flush=True)
     events = sum(ev_counts)
     kernel_s = st["kernel_ms"] * 1e-3
-    bytes_total = sum(algorithmic_bytes(totals[i], P, P, ev_counts[i - args.warmup])
-                      for i in range(args.warmup, n_win))
+    bytes_total = sum(algorithmic_bytes(totals[i], P, P, ev_counts[i]) for i in range(n_timed))
     achieved = bytes_total / kernel_s / 1e9
     line = {
         "metric": "pixel-samples/s + events/s on 1 MI355X; irradiance samples/event at fixed error",
@@ -489,23 +502,42 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (procedural scene, seed %d)" % args.seed,
-        "config": {"workload": "1280x720 sensor, 64 textured affine rectangles, %d windows of 2^23 ns "
-                               "(%.1f ms), plain controller tol 0.01, %s response, leak 0.05 Hz/px, %s"
-                               % (args.steps, args.steps * WINDOW / 1e6, args.photoreceptor,
-                                  "shot noise 0.1 Hz/px" if p.shot_rate_hz > 0 else "noise off"),
-                   "pixels": P, "samples": samples, "events": events},
+        "config": {"workload": "1280x720 sensor, 64 textured affine rectangles, %d windows of 2^23 ns from t = 0 "
+                               "(%.1f ms; %d windows per step), plain controller tol 0.01, %s response, leak "
+                               "0.05 Hz/px, %s"
+                               % (n_timed, n_timed * WINDOW / 1e6, wps, args.photoreceptor,
+                                  "shot noise 0.1 Hz/px" if p.shot_rate_hz > 0 else
+                                  "the in-path noise" if p.noise else "noise off"),
+                   "pixels": P, "samples": samples, "events": events, "windows": n_timed,
+                   "windows_per_step": wps},
         "events_per_s": events / elapsed,
         "samples_per_event": samples / max(events, 1),
         "tuning": tuning,
         "autotune_trials": len(trials),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (TRAFFIC_BYTES_PER_SAMPLE * samples / args.steps
-                                 if args.photoreceptor == "instantaneous" and TRAFFIC_BYTES_PER_SAMPLE else None),
-                     "kernel": ADVANCE_CHAIN[args.photoreceptor],
-                     "kernel_ms_per_launch": st["kernel_ms"] / args.steps,
-                     "bytes_per_launch": bytes_total / args.steps},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": (ADVANCE_CHAIN[args.photoreceptor].replace("cascade_walk_kernel", "noise_walk_kernel")
+                                if p.noise else ADVANCE_CHAIN[args.photoreceptor]),
+                     "kernel_ms_per_launch": st["kernel_ms"] / n_timed,
+                     "bytes_per_launch": bytes_total / n_timed},
     }
+    # The cascade photoreceptor on the first windows of the same runs: without noise, and with the
+    # reference's in-path noise (its default sensor model) in place of the shot process.
+    if args.cascade_windows > 0:
+        extra = {}
+        for name, noise in (("cascade", False), ("cascade_noise", True)):
+            m = fresh(sensor_params(bsx, "cascade", noise))
+            nw = min(args.cascade_windows, n_win)
+            el, sm, evc, stc = time_windows(m, runs, totals, 0, nw, tuning)
+            extra[name] = {"value": sm / el, "unit": "pixel-samples/s", "events_per_s": sum(evc) / el,
+                           "windows": nw, "ms_per_window": el / nw * 1e3,
+                           "kernel_ms_per_launch": stc["kernel_ms"] / nw, "samples": sm, "events": sum(evc),
+                           "noise": "in-path LNA noise (prophesee_gen3 model)" if noise else "off",
+                           "cells": stc["cells"], "refinements": stc["refinements"]}
+            print(f"# {name}: {nw} windows in {el:.3f} s", file=sys.stderr, flush=True)
+            del m
+        line["cascade"] = extra["cascade"]
+        line["cascade_noise"] = extra["cascade_noise"]
     if args.cpu_budget > 0:
         window_runs = (S.runs_to_host(runs[i]) for i in range(n_win))
         line["cpu_baseline"] = cpu_baseline(p, window_runs, frame, W, H, args.cpu_budget)

@@ -116,6 +116,35 @@ typedef struct bsx_device_events {
   size_t count;
 } bsx_device_events;
 
+/* One pixel's frozen maps as the reference hands them to its GPU backend: PixelMap
+ * (src/sensor/maps.h:17-27), 48 B. hot: 0 none, 1 leak-type, 2 threshold-type. */
+typedef struct bsx_pixel_map {
+  double theta_on, theta_off, leak, dark_fA, phi0;
+  uint8_t hot;
+  uint8_t pad[3];
+  float gain;
+} bsx_pixel_map;
+
+/* The fp32 kernel constants the reference hands to its GPU backend (KernelConstants<float>,
+ * src/sensor/pixel.h:216-252), as plain fields: the cascade's constants (CascadeConstants,
+ * src/sensor/substep.h:26-34), the photometry, the noise tables' view and the reset. The last four
+ * flags name features of the reference this backend refuses (BSX_E_INVALID when set). */
+typedef struct bsx_kernel_constants {
+  float k, a2, a3, tau_o, tau_sf, eps_accept, eps_drop;
+  float eps_pl_base, scale, irr_max;
+  int32_t groups;
+  int32_t instant, noise, levels;
+  float margin_z;
+  int32_t n_nodes, stride; /* the noise tables' shape (noise on) */
+  float ln_a1_min, inv_dln;
+  int32_t ideal_reset;
+  int64_t refractory_ns;
+  float eps_ci, label_cut;
+  uint64_t seed;
+  uint32_t camera;
+  int32_t modulated, breakpoints, transient, hpf;
+} bsx_kernel_constants;
+
 /* Host view of the events of the last advance/download, sorted by (t_cross, pixel), as the
  * reference's Events (src/sensor/sensor_model.h:66-75). Pointers into model-owned memory, valid
  * until the next call on the model. */
@@ -198,6 +227,28 @@ int bsx_draw_maps(const bsx_params* p, uint64_t seed, uint32_t camera, uint32_t 
  * n_nodes * stride values when cap holds that many, else BSX_E_INVALID. */
 int bsx_noise_tables(const bsx_params* p, double* data64, float* data32, size_t cap, int32_t* n_nodes,
                      int32_t* stride);
+
+/* The reference's GPU backend seam (CudaSensorKernel, src/sensor/sensor_cuda.h:20-68), for its own
+ * SensorModel to run on this backend: a model of n_pixels pixels built from the reference's fp32
+ * kernel constants, its PixelMaps and its fp32 noise tables (n_table_values = n_nodes * stride, or 0
+ * with the noise off). Reference: CudaSensorKernel(kc, maps, tables), sensor_cuda.h:22-23. */
+int bsx_sensor_create_from_constants(const bsx_kernel_constants* kc, const bsx_pixel_map* maps, size_t n_pixels,
+                                     const float* tables, size_t n_table_values, bsx_sensor** out);
+/* The reference's 200-B PixelStates (src/sensor/pixel.h:105-121) in and out of the device's working
+ * state. Reference: upload_states / download_states, sensor_cuda.h:30-31. */
+int bsx_sensor_upload_states(bsx_sensor* s, const void* states, size_t n_states);
+int bsx_sensor_download_states(bsx_sensor* s, void* states, size_t n_states);
+/* The model time of the next call (the reference passes t_state to each advance:
+ * sensor_cuda.h:50-51, 59-61), after upload_states. */
+int bsx_sensor_set_time(bsx_sensor* s, int64_t t_state_ns);
+/* New photometry of a model built from constants: kc.scale, kc.irr_max and the maps (their gains).
+ * Reference: CudaSensorKernel::set_photometry(kc, maps), sensor_cuda.h:40. */
+int bsx_sensor_set_kernel_photometry(bsx_sensor* s, float scale, float irr_max, const bsx_pixel_map* maps,
+                                     size_t n_pixels);
+/* The raw events of the last host advance or download, sorted by (t_cross, pixel): *out points into
+ * model-owned memory, valid until the next call on the model. Reference: the RawEvents that
+ * CudaSensorKernel::advance and download return (sensor_cuda.h:54, 59-61). */
+int bsx_sensor_raw_events(bsx_sensor* s, const bsx_raw_event** out, size_t* count);
 
 /* Reference: SensorModel::SensorModel(params, width, height, seed, light_groups, camera, ...,
  * Precision::kFp32, Device::kCuda), src/sensor/sensor_model.h:107-110 (num_threads, precision and

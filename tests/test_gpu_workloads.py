@@ -1,0 +1,140 @@
+"""GPU parity at the benchmark workloads' own shapes, bit for bit against the oracle (itself pinned to
+the compiled reference): events in every field, reported times, the saved state of every pixel and
+the work counters, window by window.
+
+- config 2: the 1280x720 sensor over the 64 textured affine rectangles, the first four windows of the
+  plain controller's runs (about 49 M samples a window), with leak 0.05 Hz/px and the shot process at
+  0.1 Hz/px; then two windows with the cascade carrying the reference's in-path noise;
+- config 1: the 346x260 textured disk through the cascade at the 200-Hz mapping, all twelve windows
+  of the 100-ms run;
+- config 3: one window of the four rig cameras at their real shapes (3.39 M pixels, log-normal(ln 20,
+  1.5) sample counts) in one rig launch, each camera against its own oracle model.
+The instantaneous path's leaf_intervals counter counts this backend's own search evaluations, so it
+is the one counter not compared there."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import events_from_dict, first_difference
+
+pytestmark = pytest.mark.gpu
+
+WINDOW = 1 << 23
+THREADS = len(os.sched_getaffinity(0))
+STATE = ("z", "x1", "s2", "s3", "tv", "i_scale", "s3_ref", "n3_ref", "dr_ref", "t_ref", "t_search", "blind_until",
+         "flags", "held")
+NOISE_STATE = ("nk", "nk1", "ln_a1", "cell", "have_k1")
+COUNTERS = ("substeps", "substep_attempts", "settled_substeps", "cells", "refinements", "leaf_intervals", "fallbacks",
+            "quarantines", "pieces", "input_clamps")
+
+
+def _compare_events(ev, want, rep, width, where):
+    got = events_from_dict(ev, width)
+    i, a, b = first_difference(got, want)
+    assert len(got) == len(want) and i == len(got), f"{where}: first difference at {i} of {len(want)}: gpu {a} oracle {b}"
+    assert np.array_equal(ev["t_reported_ns"], rep), where
+
+
+def _compare_state(gpu, ref, where, instant, noise=False, shot=False):
+    st = np.frombuffer(gpu.save_state()[56:], O.REF_STATE_DTYPE)
+    so = ref.states()
+    for f in STATE + (NOISE_STATE if noise else ()):
+        assert np.array_equal(st[f], so[f]), f"{where}: state {f}"
+    if shot:  # the shot process's next arrival and index ride in cell and nk[0]
+        assert np.array_equal(st["cell"], so["shot_next"]), f"{where}: shot_next"
+        assert np.array_equal(st["nk"][:, 0], so["shot_index"].astype(np.float64)), f"{where}: shot_index"
+    ks, ko = gpu.stats(), ref.stats()
+    for f in COUNTERS:
+        if instant and f == "leaf_intervals":
+            continue
+        assert ks[f] == ko[f], (where, f, ks[f], ko[f])
+
+
+def _run_windows(scene, view, p, rules, windows, seed=1, camera=0):
+    import ballasim_amd as bsx
+    from ballasim_amd import scene as S
+    W, H = view.width, view.height
+    frame = scene.render(view, 0).reshape(-1, 1)
+    gpu = bsx.SensorModel(p, W, H, seed, ["constant"], camera, device="cuda")
+    ref = O.OracleModel(p.c, W, H, seed, 1, camera, THREADS)
+    gpu.init(0, frame)
+    ref.init(0, frame)
+    instant = p.photoreceptor == "instantaneous"
+    total_events = total_samples = 0
+    for i in range(windows):
+        runs, n_samples = scene.sample_window(view, rules, i * WINDOW, slot=0)
+        print(f"window {i}: {n_samples} samples", flush=True)
+        ptr, count = gpu.advance_device_runs(runs, (i + 1) * WINDOW)
+        ev = gpu.download_events(ptr, count)
+        ids, t, irr, offs = S.runs_to_host(runs)
+        want, rep = ref.advance_runs(ids, t, irr, offs, (i + 1) * WINDOW)
+        _compare_events(ev, want, rep, W, f"window {i}")
+        _compare_state(gpu, ref, f"window {i}", instant, noise=p.noise and not instant, shot=p.shot_rate_hz > 0)
+        total_events += len(want)
+        total_samples += n_samples
+    return total_samples, total_events, ref.stats()
+
+
+def test_config2_windows_equal_oracle():
+    from ballasim_amd import configs
+    from ballasim_amd import scene as S
+    scene, view = S.config2_scene(0)
+    p = configs.config2_params("instantaneous")
+    rules = S.rules(float(p.dark_current_fA) / float(p.illuminance_scale), tol=0.01)
+    samples, events, st = _run_windows(scene, view, p, rules, 4)
+    assert samples > 4 * 20_000_000 and events > 100_000
+
+
+def test_config2_cascade_with_noise_windows_equal_oracle():
+    from ballasim_amd import configs
+    from ballasim_amd import scene as S
+    scene, view = S.config2_scene(0)
+    p = configs.config2_params("cascade", noise=True)
+    rules = S.rules(float(p.dark_current_fA) / float(p.illuminance_scale), tol=0.01)
+    samples, events, st = _run_windows(scene, view, p, rules, 2)
+    assert events > 10_000 and st["cells"] > 921_600 and st["refinements"] > 0
+
+
+def test_config1_all_windows_cascade_200hz():
+    from ballasim_amd import configs
+    from ballasim_amd import scene as S
+    scene, view = S.config1_scene(0)
+    p = configs.config1_params("cascade")
+    rules = S.rules(float(p.dark_current_fA) / float(p.illuminance_scale), tol=0.01)
+    samples, events, st = _run_windows(scene, view, p, rules, 12)
+    assert events > 1_000 and st["substep_attempts"] > st["substeps"]
+
+
+def test_config3_rig_window_each_camera_equals_oracle():
+    import ballasim_amd as bsx
+    from ballasim_amd import configs
+    from ballasim_amd import scene as S
+    from ballasim_amd.sensor import Rig
+    scene, _ = S.config1_scene(0)
+    views = S.rig_views()
+    p = configs.config3_params("instantaneous")
+    rng = np.random.default_rng(1)
+    cams, refs, runs, totals = [], [], [], []
+    for c, v in enumerate(views):
+        frame = scene.render(v, 0).reshape(-1, 1)
+        m = bsx.SensorModel(p, v.width, v.height, 1 + c, ["constant"], c, device="cuda")
+        m.init(0, frame)
+        o = O.OracleModel(p.c, v.width, v.height, 1 + c, 1, c, THREADS)
+        o.init(0, frame)
+        cams.append(m)
+        refs.append(o)
+        r, tot = scene.sample_counts(v, S.lognormal_counts(rng, v.width * v.height), 0, slot=c)
+        runs.append(r)
+        totals.append(tot)
+    print(f"rig window: {sum(totals)} samples over {sum(v.width * v.height for v in views)} pixels", flush=True)
+    rig = Rig(cams)
+    out = rig.advance_device(runs, WINDOW)
+    for c, v in enumerate(views):
+        ev = cams[c].download_events(*out[c])
+        ids, t, irr, offs = S.runs_to_host(runs[c])
+        want, rep = refs[c].advance_runs(ids, t, irr, offs, WINDOW)
+        _compare_events(ev, want, rep, v.width, f"camera {c}")
+        _compare_state(cams[c], refs[c], f"camera {c}", True)
+    assert sum(totals) > 150_000_000
