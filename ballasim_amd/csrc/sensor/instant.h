@@ -415,6 +415,47 @@ struct InstantLane {
     s3_ref = s3;
     n3_ref = n3;
     dr_ref = offset;
+    scr_valid = false;
+  }
+
+  // ---- The photocurrent screen ------------------------------------------------------------------
+  // Between two reference changes, Delta(t) = fl(fl(fl(x(t) - s3_ref) + (0 - n3_ref)) + dl(t)) with
+  // x(t) = pm::log(fl(I(t) / i_scale)) is non-decreasing in the photocurrent I(t) (every step is a
+  // correctly rounded monotone operation, and pm::log is monotone on every positive float) and, with
+  // leak >= 0, in t through dl. Its fp32 value lies within 2^-13 of the real-valued
+  // ln(I / i_scale) - s3_ref + dl(t) (|x|, |s3_ref| and |dl| stay below 64: photocurrents lie in
+  // [dark, 1e9 fA] and a reference is an x value; a handful of roundings of 2^-18 at most, and the log's
+  // own error). So on [t_from, t_end) no integer fires while I(t) stays above
+  //   scr_lo = i_scale exp(-theta_off + s3_ref - dl(t_from)) (1 + 2^-9)
+  // and below
+  //   scr_hi = i_scale exp(theta_on + s3_ref - dl(t_end - 1)) (1 - 2^-9),
+  // a margin of 2^-9 in log space against an error below 2^-13 (the exp's own error is far smaller).
+  // A segment's computed I(t) lies within a relative 2^-20 of the range of its end values, so the
+  // test widens that range by 2^-12. It needs no logarithm: a segment that passes it is decided
+  // without the exact search, whose result (no crossing) it is.
+  float scr_lo = 0.0f, scr_hi = 0.0f;
+  bool scr_valid = false;
+  bool screen = true;  // the screen on (off: every piece takes the exact search, for the tests)
+
+  BSX_HD BSX_INLINE void screen_bounds(int64_t t_from, int64_t t_last) {
+    const float dl_lo = dl_at(t_from), dl_hi = dl_at(t_last);
+    scr_hi = i_scale * pm::exp(theta_on + s3_ref - dl_hi) * (1.0f - 0x1p-9f);
+    scr_lo = i_scale * pm::exp(-theta_off + s3_ref - dl_lo) * (1.0f + 0x1p-9f);
+    scr_valid = true;
+  }
+
+  // True when I(t) over the current segment stays strictly inside (scr_lo, scr_hi).
+  BSX_HD BSX_INLINE bool screened() const {
+    float lo = dark, hi = dark;
+    const bool held = tb == kNoBreakpoint;
+    for (int g = 0; g < kMaxG; ++g) {
+      if (g < G) {
+        const float b = held ? va[g] : vb[g];
+        lo = lo + scale * vmin(va[g], b);
+        hi = hi + scale * vmax(va[g], b);
+      }
+    }
+    return (hi * (1.0f + 0x1p-12f) < scr_hi) & (lo * (1.0f - 0x1p-12f) > scr_lo);
   }
 
   BSX_HD BSX_INLINE static uint8_t split_label(float ds, float dl, float D, float cut) {
@@ -543,6 +584,13 @@ struct InstantLane {
       }
       const int64_t lo = vmax(t_search, t);
       if (lo >= e) return true;
+      if (screen && n3_ref == 0.0f && leak >= 0.0f) {
+        if (!scr_valid) screen_bounds(lo, t_end_ - 1);
+        if (screened()) {
+          t_search = vmax(t_search, e);
+          return true;
+        }
+      }
       if (q != nullptr && lo == ta && e == tb) {
         Hit h{false, false, lo, 0.0f};
         const int r = fast_search(*q, h);

@@ -428,6 +428,117 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
   if (blocks != nullptr) block_summary(blocks, qt, total, blk_ok, blk_run, b_lo, b_hi, t_last);
 }
 
+// Phase A of the instantaneous response as a screen of every segment of every camera's runs, without
+// logarithms: one block of kSegmentAlign consecutive positions of one camera, each thread
+// kPrepPerThread consecutive positions (a pixel's constants and photocurrent bounds formed once for its
+// consecutive segments, a segment's end sample carried into the next). A segment is quiet under the
+// call's input state when no release or shot arrival is pending before its end, it holds no 2^23-ns
+// cut, and its photocurrent stays inside the pixel's screen bounds over the whole call
+// (InstantLane::screen_bounds from t_state to t_end - 1, InstantLane::screened): no integer of it fires
+// under that state. Each run's first segment that is not quiet (the walk starts there) and its walk
+// cost (one unit per loud segment plus kSearchWeight per threshold's worth of log-intensity change, fast
+// logs: a schedule only) are folded per run as in chain_prep_kernel. Nothing is written per segment.
+template <int kMaxG>
+__global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams, int n_cams, const CallDev* call) {
+  const int64_t g0 = static_cast<int64_t>(blockIdx.x) * kSegmentAlign;
+  if (g0 >= call->n_segments) return;  // beyond this call's segments (a grid sized for a larger one)
+  const CameraDev& cd = cams[camera_at<Space::kSegments>(cams, n_cams, g0)];
+  const KConst kc = cd.kc;
+  const MapsDev mp = cd.mp;
+  const StateDev in = cd.in;
+  const RunsDev runs = cd.runs;
+  const uint32_t* const pixel = cd.pixel;
+  const int64_t* const pos = cd.pos;
+  float* const work = cd.work;
+  int* const first_loud = cd.first_loud;
+  const int64_t t_state = cd.t_state, total = cd.total, n_runs = cd.n_runs, t_end = call->t_end;
+  const int64_t q0 = g0 - cd.seg_base, qt = q0 + static_cast<int64_t>(threadIdx.x) * kPrepPerThread;
+  __shared__ BlockRuns br;
+  RunRef rr = stage_block_runs(br, pos, runs, pixel, n_runs, q0, qt, total);
+  int acc_run = -1;
+  float cost_acc = 0.0f;
+  int loud_acc = INT32_MAX;
+  InstantLane<kMaxG> L;
+  bool usable = false;    // the pixel's screen applies (leak >= 0, n3_ref = 0, not blind)
+  int64_t loaded = -1;    // the run whose pixel constants L holds
+  int64_t prev_end = -1;  // the position whose segment's end sample L.tb, L.vb hold (-1: none)
+  for (int k = 0; k < kPrepPerThread; ++k) {
+    const int64_t q = qt + k;
+    if (!(q < total)) break;
+    advance_run(rr, br, pos, runs, pixel, n_runs, q);
+    const int run = static_cast<int>(rr.i);
+    if (run != acc_run) {
+      if (acc_run >= 0) {
+        if (work != nullptr) atomicAdd(work + acc_run, cost_acc);
+        if (first_loud != nullptr && loud_acc != INT32_MAX) atomicMin(first_loud + acc_run, loud_acc);
+      }
+      acc_run = run;
+      cost_acc = 0.0f;
+      loud_acc = INT32_MAX;
+    }
+    const uint32_t p = rr.pixel;
+    if (rr.i != loaded) {
+      loaded = rr.i;
+      L.G = kc.groups;
+      L.irr_max = kc.irr_max;
+      L.dark = mp.dark[p];
+      L.scale = mp.scale[p];
+      L.leak = mp.leak[p];
+      L.theta_on = mp.theta_on[p];
+      L.theta_off = mp.theta_off[p];
+      L.i_scale = in.i_scale[p];
+      L.s3_ref = in.s3_ref[p];
+      L.n3_ref = in.n3_ref[p];
+      L.dr_ref = in.dr_ref[p];
+      L.t_ref = in.t_ref[p];
+      L.blind_until = in.blind_until[p];
+      L.shot_next = in.shot_next[p];
+      usable = L.blind_until == kNotBlind && L.leak >= 0.0f && L.n3_ref == 0.0f;
+      if (usable) L.screen_bounds(t_state, t_end - 1);
+    }
+    const int64_t j = q - rr.pos;
+    const int64_t* t = runs.t + rr.row_start;
+    const float* v = runs.irr + rr.row_start * kc.groups;
+    if (j == 0) {
+      if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
+        prev_end = -1;
+        continue;
+      }
+      L.ta = t_state;
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
+    } else if (prev_end == q - 1) {  // the previous position's segment ends where this one starts
+      L.ta = L.tb;
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < kc.groups) L.va[g] = L.vb[g];
+    } else {
+      L.ta = t[j - 1];
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < kc.groups) L.va[g] = L.clampv(v[(j - 1) * kc.groups + g]);
+    }
+    L.tb = t[j];
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
+    prev_end = q;
+    const bool quiet = usable & (L.shot_next >= L.tb) &
+                       (((L.ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= L.tb) & L.screened();
+    if (!quiet) {
+      loud_acc = vmin(loud_acc, static_cast<int>(j));
+      float Ia = L.dark, Ib = L.dark;
+      for (int g = 0; g < kMaxG; ++g) {
+        if (g < kc.groups) {
+          Ia += L.scale * L.va[g];
+          Ib += L.scale * L.vb[g];
+        }
+      }
+      // (a schedule only: hardware logarithm and reciprocal)
+      cost_acc += 1.0f + kSearchWeight * fabs_(__logf(Ib) - __logf(Ia)) *
+                             __builtin_amdgcn_rcpf(vmin(L.theta_on, L.theta_off));
+    }
+  }
+  wave_segmented_cost_loud(work, first_loud, acc_run, cost_acc, loud_acc);
+}
+
 // Before phase A: zero each camera's counters, the order histogram and the queue counter (block 0),
 // and set every run's cost to 0 and first loud segment to none (one thread per run, cameras' run
 // bases multiples of 256).
@@ -1164,10 +1275,10 @@ hipError_t launch_chain_prepare(const ChainArgs& a, hipStream_t stream) {
     const dim3 grid = grid_for(static_cast<size_t>(a.seg_capacity), 256);
     const dim3 prep_grid = grid_for(static_cast<size_t>(a.seg_capacity), kSegmentAlign);
     if (!a.instant) chain_cost_kernel<<<grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
-    else if (a.groups <= 1) chain_prep_kernel<1><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
-    else if (a.groups <= 2) chain_prep_kernel<2><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
-    else if (a.groups <= 4) chain_prep_kernel<4><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
-    else chain_prep_kernel<8><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else if (a.groups <= 1) chain_screen_kernel<1><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else if (a.groups <= 2) chain_screen_kernel<2><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else if (a.groups <= 4) chain_screen_kernel<4><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
+    else chain_screen_kernel<8><<<prep_grid, 256, 0, stream>>>(a.cams, a.n_cams, a.call);
   }
   if (a.n_total > 0) {
     const dim3 order_grid = grid_for(a.n_total, 256 * kOrderPerThread);

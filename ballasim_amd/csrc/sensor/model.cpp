@@ -529,10 +529,6 @@ CameraDev SensorModel::stage(const RunsDev& runs, const uint32_t* pixel, size_t 
   const bool inst = instant();
   if (cap_ == 0) cap_ = std::max<unsigned long long>(1ull << 20, 2 * P_);
   d_events_.ensure(cap_ * sizeof(RawEventDev));
-  if (inst) {
-    d_prep_.ensure(std::max<int64_t>(total, 1) * sizeof(InstantPrep));
-    d_blocks_.ensure(std::max<int64_t>((total + kBlockSegments - 1) / kBlockSegments, 1) * sizeof(InstantBlock));
-  }
   d_work_.ensure(std::max<size_t>(n_runs, 1) * sizeof(float));
   d_loud_.ensure(std::max<size_t>(n_runs, 1) * sizeof(int));
   CameraDev cd{};
@@ -543,8 +539,8 @@ CameraDev SensorModel::stage(const RunsDev& runs, const uint32_t* pixel, size_t 
   cd.runs = runs;
   cd.pixel = pixel;
   cd.pos = d_pos_.as<const int64_t>();
-  cd.prep = inst ? d_prep_.as<InstantPrep>() : nullptr;
-  cd.blocks = inst ? d_blocks_.as<InstantBlock>() : nullptr;
+  cd.prep = nullptr;  // phase A screens the segments and writes no per-segment records
+  cd.blocks = nullptr;
   cd.first_loud = inst ? d_loud_.as<int>() : nullptr;
   cd.work = d_work_.as<float>();
   cd.ev = d_events_.as<RawEventDev>();

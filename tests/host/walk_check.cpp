@@ -32,7 +32,7 @@ int main(int argc, char** argv) {
   std::uniform_real_distribution<double> U(0.0, 1.0);
   std::normal_distribution<double> N01(0.0, 1.0);
   std::vector<uint32_t> stack(InstantLane<kG>::kSearchDepth);
-  long n_events = 0, n_blocks_skipped = 0, bad = 0;
+  long n_events = 0, n_blocks_skipped = 0, n_screened = 0, bad = 0;
   for (int it = 0; it < trials; ++it) {
     const int G = 1 + it % 2;
     const int n = 20 + static_cast<int>(U(rng) * 300);
@@ -132,10 +132,11 @@ int main(int argc, char** argv) {
       }
       if (ok) blocks[b] = InstantBlock{mn, mx, ts[j0 + 15]};
     }
-    VecSink s0, s1;
-    InstantLane<kG> A = base, B = base;
+    VecSink s0, s1, s2;
+    InstantLane<kG> A = base, B = base, C = base;
     A.prep = nullptr;
     A.blocks = nullptr;
+    A.screen = false;  // the reference: every piece by the exact search
     B.prep = prep.data();
     B.blocks = blocks.data();
     B.pos0 = pos0;
@@ -151,7 +152,49 @@ int main(int argc, char** argv) {
     for (int m = step(); m != InstantLane<kG>::kDone; m = step())
       if (m == InstantLane<kG>::kSearch) B.search_step(s1, stack.data(), 1);
     B.end();
-    bool ok = s0.ev.size() == s1.ev.size();
+    // the kernel's path: phase A's log-free screen of the segments under the input state (the rule of
+    // chain_screen_kernel) gives the first loud segment, and the walk screens each piece from there
+    C.prep = nullptr;
+    C.blocks = nullptr;
+    int first_loud = n;
+    {
+      InstantLane<kG> S = base;
+      const bool usable = S.blind_until == bsx::kNotBlind && S.leak >= 0.0f && S.n3_ref == 0.0f;
+      if (usable) S.screen_bounds(t_state, t_end - 1);
+      for (int j = 0; j < n; ++j) {
+        if (j == 0 && ts[0] == t_state) continue;
+        if (j == 0) {
+          S.ta = t_state;
+          for (int g = 0; g < G; ++g) S.va[g] = S.held[g];
+        } else {
+          S.ta = ts[j - 1];
+          for (int g = 0; g < G; ++g) S.va[g] = S.clampv(irr[static_cast<size_t>(j - 1) * G + g]);
+        }
+        S.tb = ts[j];
+        for (int g = 0; g < G; ++g) S.vb[g] = S.clampv(irr[static_cast<size_t>(j) * G + g]);
+        const bool quiet = usable && S.shot_next >= S.tb &&
+                           ((S.ta >> bsx::kPlainSegmentLog2) + 1) * bsx::kPlainSegmentNs >= S.tb && S.screened();
+        if (!quiet) {
+          first_loud = j;
+          break;
+        }
+      }
+    }
+    n_screened += first_loud - C.skip;
+    {
+      uint32_t stack2[InstantLane<kG>::kSearchDepth];
+      C.begin(t_end, first_loud);
+      const int budget = 1 + it % 4;
+      for (int m = C.walk(s2, budget); m != InstantLane<kG>::kDone; m = C.walk(s2, budget))
+        if (m == InstantLane<kG>::kSearch) C.search_step(s2, stack2, 1);
+      C.end();
+    }
+    bool ok = s0.ev.size() == s1.ev.size() && s0.ev.size() == s2.ev.size();
+    for (size_t k = 0; ok && k < s0.ev.size(); ++k) ok = same_event(s0.ev[k], s2.ev[k]);
+    ok = ok && A.t_search == C.t_search && A.t_ref == C.t_ref && A.blind_until == C.blind_until &&
+         std::memcmp(&A.s3_ref, &C.s3_ref, 4) == 0 && std::memcmp(&A.dr_ref, &C.dr_ref, 4) == 0 &&
+         A.flags == C.flags && A.shot_next == C.shot_next && A.shot_index == C.shot_index &&
+         A.n_substeps == C.n_substeps && A.n_pieces == C.n_pieces && std::memcmp(A.held, C.held, sizeof(A.held)) == 0;
     for (size_t k = 0; ok && k < s0.ev.size(); ++k) ok = same_event(s0.ev[k], s1.ev[k]);
     ok = ok && A.t_search == B.t_search && A.t_ref == B.t_ref && A.blind_until == B.blind_until &&
          std::memcmp(&A.s3_ref, &B.s3_ref, 4) == 0 && std::memcmp(&A.dr_ref, &B.dr_ref, 4) == 0 &&
@@ -163,7 +206,7 @@ int main(int argc, char** argv) {
                   s1.ev.size(), static_cast<long long>(A.t_search), static_cast<long long>(B.t_search), A.n_pieces,
                   B.n_pieces);
   }
-  std::printf("walk trials %d events %ld blocks_skipped %ld differences %ld\n", trials, n_events, n_blocks_skipped,
-              bad);
+  std::printf("walk trials %d events %ld blocks_skipped %ld screened %ld differences %ld\n", trials, n_events,
+              n_blocks_skipped, n_screened, bad);
   return bad != 0;
 }

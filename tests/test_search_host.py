@@ -69,9 +69,11 @@ def test_fast_search_agrees_with_bisection(search_check, seed):
 def test_walk_with_block_skips_equals_exact_walk(walk_check, seed):
     r = subprocess.run([str(walk_check), "3000", str(seed)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout
-    m = re.search(r"walk trials (\d+) events (\d+) blocks_skipped (\d+) differences (\d+)", r.stdout)
+    m = re.search(r"walk trials (\d+) events (\d+) blocks_skipped (\d+) screened (\d+) differences (\d+)",
+                  r.stdout)
     assert m, r.stdout
-    trials, events, skipped, bad = map(int, m.groups())
+    trials, events, skipped, screened, bad = map(int, m.groups())
     assert bad == 0
     assert events > 10 * trials      # runs that fire
     assert skipped > trials // 4     # and quiet blocks that the walk passes at once
+    assert screened > trials         # and quiet prefixes that phase A's screen decides
