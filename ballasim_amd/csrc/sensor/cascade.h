@@ -573,18 +573,13 @@ struct CascadeLane : InstantLane<kMaxG> {
   }
 
   // The call: the segments of the augmented samples in order, cut at multiples of 2^23 ns, each a
-  // linear piece of photocurrent (constant waveforms); the next sample is fetched one segment ahead.
+  // linear piece of photocurrent (constant waveforms); the segment ends are fetched kAhead samples ahead.
   template <class Sink>
   BSX_HD BSX_INLINE void advance(int64_t t_end, Sink& sink) {
     this->begin(t_end);
     int64_t t = ta;
     while (t < t_end) {
-      tb = this->nt;
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < G) vb[g] = this->nv[g];
-      ++this->next_i;
-      if (this->next_i < n) this->load_sample(this->next_i, this->nt, this->nv);
-      else this->nt = kNoBreakpoint;
+      this->open_next();
       const int64_t stop = vmin(t_end, tb);
       while (t < stop) {
         const int64_t e = vmin(stop, ((t >> kPlainSegmentLog2) + 1) * kPlainSegmentNs);

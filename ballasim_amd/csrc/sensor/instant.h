@@ -36,15 +36,9 @@ struct InstantLane {
   int G;
   bool ideal_reset;
   bool hot = false;  // a hot pixel: its events carry kFlagHotPixel, and HOT for any label but SIGNAL
-  // The call's input run, and its segments' prepared values (prep[j]: the segment ending at input
-  // sample j; null: every search is exact).
+  // The call's input run.
   const BSX_GLOBAL int64_t* ts;
   const BSX_GLOBAL float* irr;
-  const BSX_GLOBAL InstantPrep* prep;
-  // Phase A's block summaries of the camera's segment space (null: none) and the run's first
-  // position in it.
-  const BSX_GLOBAL InstantBlock* blocks = nullptr;
-  int64_t pos0 = 0;
   int n, skip;
   int64_t t_state;
   // State.
@@ -508,74 +502,11 @@ struct InstantLane {
     }
   }
 
-  using Prep = InstantPrep;
-
-  // The segment's values as phase A forms them (also the check of the fast path on the host).
-  BSX_HD BSX_INLINE Prep prep_segment() const {
-    Prep q;
-    q.x_a = log_ratio(photocurrent_start());
-    q.x_c = log_ratio(photocurrent(tb - 1));
-    return q;
-  }
-
-  // The whole-segment search from t_a on the prepared values: 0 no crossing in [t_a, t_b), 1 a
-  // crossing at t_a itself (the hit is filled), 2 undecided (the exact search must run).
-  //   - Without leak, or with I(t_b - 1) > I(t_a) (known when x_c > x_a, or when no group falls),
-  //     this is the root test of instant_may_fire: Delta at t_b - 1 against both levels, or against
-  //     theta_on.
-  //   - A falling segment under leak (no group rises): S(t) = ln(I(t) / i_scale) - s3_ref - n3_ref
-  //     does not increase and dl(t) does not decrease on [t_a, t_b - 1] (the monotone steps of the
-  //     transition searches), so fl(S(c) + dl(t_a)) <= Delta(t) <= fl(S(t_a) + dl(c)) there, c =
-  //     t_b - 1. When both bounds stay inside the levels, no integer of the segment fires, and the
-  //     reference's bisection (which reports only integers that fire) finds none either. These
-  //     bounds need no logarithms beyond x_a and x_c, and are tighter than instant_may_fire's.
-  //   - Groups moving both ways with x_c <= x_a: undecided.
-  BSX_HD BSX_INLINE int fast_search(const Prep& q, Hit& h) const {
-    h.x = q.x_a;
-    if (fires(ta, h)) return 1;
-    if (tb - ta == 1) return 0;
-    if (tb - ta == 2) return 2;  // the root is a leaf: the exact search tests it
-    const int64_t c = tb - 1;
-    const float D_c = delta_at(c, q.x_c);
-    if (leak == 0.0f) return D_c >= theta_on || D_c <= -theta_off ? 2 : 0;
-    if (!(leak > 0.0f)) return 2;
-    bool up = true, down = true;
-    for (int g = 0; g < kMaxG; ++g) {
-      if (g < G) {
-        if (vb[g] < va[g]) up = false;
-        if (vb[g] > va[g]) down = false;
-      }
-    }
-    if (up || q.x_c > q.x_a) return D_c >= theta_on ? 2 : 0;
-    if (!down) return 2;
-    const float S_a = (q.x_a - s3_ref) + (0.0f - n3_ref);
-    const float S_c = (q.x_c - s3_ref) + (0.0f - n3_ref);
-    if (S_a + dl_at(c) >= theta_on) return 2;
-    return S_c + dl_at(ta) <= -theta_off ? 2 : 0;
-  }
-
-  // Does the walk, reaching the start of the current segment [ta, tb) in the state the lane holds
-  // (the call's input state, with no event, release or arrival before it), finish the segment
-  // without an event? True when no release or arrival is pending before tb, the segment is one piece
-  // (no 2^23 cut) and fast_search = 0: then no integer of [ta, tb) fires under that state, so none
-  // of [max(t_search, ta), tb) does either, and the walk's one piece of the segment (decided by
-  // piece_fast, by an exact search after dead time, or skipped inside it) only advances t_search to
-  // max(t_search, tb) and counts one piece.
-  // (Evaluated without early exits: every condition is computed and combined at the end.)
-  BSX_HD BSX_INLINE bool quiet_segment(const Prep& q) const {
-    const bool eligible = (blind_until == kNotBlind) & (shot_next >= tb) &
-                          (((ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= tb);
-    Hit h{false, false, ta, 0.0f};
-    const int r = fast_search(q, h);
-    return eligible & (r == 0);
-  }
-
   // The fast part of one piece [t, e) of the current segment (the body of advance_instant's piece
-  // loop up to its exact search). With the segment's prepared values (q non-null), a search that
-  // starts at the segment's start and spans all of it is decided on them. Returns true when the
-  // piece is finished, false when the exact search of [s_lo, e) must run next (search_step).
+  // loop up to its exact search): releases, dead time, and the photocurrent screen. Returns true when
+  // the piece is finished, false when the exact search of [s_lo, e) must run next (search_step).
   template <class Sink>
-  BSX_HD BSX_INLINE bool piece_fast(int64_t t, int64_t e, Sink& sink, const Prep* q) {
+  BSX_HD BSX_INLINE bool piece_fast(int64_t t, int64_t e, Sink& sink) {
     while (true) {
       if (blind_until != kNotBlind) {
         if (blind_until >= e) return true;
@@ -591,20 +522,8 @@ struct InstantLane {
           return true;
         }
       }
-      if (q != nullptr && lo == ta && e == tb) {
-        Hit h{false, false, lo, 0.0f};
-        const int r = fast_search(*q, h);
-        if (r == 1) {
-          fire(h.t, h.on, h.x, 0.0f, sink);
-          continue;
-        }
-        if (r == 0) {
-          if (tb - ta > 2) ++n_leaves;  // the root node the exact search would have counted
-          t_search = vmax(t_search, e);
-          return true;
-        }
-      }
       s_lo = lo;
+      (void)sink;
       return false;
     }
   }
@@ -652,18 +571,43 @@ struct InstantLane {
   // lanes of a wave freely (the kernel gathers the searches of many lanes into one pass).
   static constexpr int kWalk = 0, kSearch = 1, kDone = 2;
 
-  int next_i;          // the next input sample to become a segment end
-  InstantBlock blk;    // the summary of the block whose first segment ends at next_i (x_min NaN: none)
-  int64_t nt;          // the prefetched segment end (kNoBreakpoint: none)
-  float nv[kMaxG];
-  Prep q_next, q_cur;  // the prefetched and the current segment's prepared values
-  bool has_prep, seg_open, in_piece;
+  // The segment ends are loaded kAhead samples ahead of the walk (raw: clamped when a segment opens),
+  // so that a lane's walk through quiet segments does not wait on one load at a time.
+  static constexpr int kAhead = 4;
+  int next_i;              // the input sample that ends the next segment to open
+  int64_t ft[kAhead];      // ts[next_i + k] (kNoBreakpoint past the run)
+  float fv[kAhead][kMaxG];
+  bool seg_open, in_piece;
   int64_t t_cur, stop, t_end_, s_lo, e_cur;
 
-  // first_loud: the index of the first input sample ending a segment that is not quiet_segment
+  BSX_HD BSX_INLINE void fetch(int k, int i) {
+    ft[k] = kNoBreakpoint;
+    if (i < n) {
+      ft[k] = ts[i];
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < G) fv[k][g] = irr[static_cast<int64_t>(i) * G + g];
+    }
+  }
+
+  // The next segment [ta, tb): its end from the head of the prefetch ring (kNoBreakpoint: the held
+  // segment after the last sample), and the ring moved one sample on.
+  BSX_HD BSX_INLINE void open_next() {
+    tb = ft[0];
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < G) vb[g] = clampv(fv[0][g]);
+    for (int k = 0; k + 1 < kAhead; ++k) {
+      ft[k] = ft[k + 1];
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < G) fv[k][g] = fv[k + 1][g];
+    }
+    ++next_i;
+    fetch(kAhead - 1, next_i + kAhead - 1);
+  }
+
+  // first_loud: the index of the first input sample ending a segment that phase A did not find quiet
   // under the input state (n: none). The walk starts at that segment, with the quiet ones before it
-  // left as the walk would leave them (t_search, the piece counts; phase A's tests are not counted as
-  // leaf intervals).
+  // left as the walk would leave them (t_search, the piece counts; phase A's screens are not counted
+  // as leaf intervals).
   BSX_HD BSX_INLINE void begin(int64_t t_end, int first_loud = 0) {
     t_end_ = t_end;
     next_i = skip;
@@ -679,61 +623,11 @@ struct InstantLane {
       load_sample(first_loud - 1, ta, va);
       t_search = vmax(t_search, ta);
     }
-    nt = kNoBreakpoint;
-    if (next_i < n) load_sample(next_i, nt, nv);
-    load_block();
+    for (int k = 0; k < kAhead; ++k) fetch(k, next_i + k);
     t_cur = ta;
-    q_next = Prep{};
-    if (prep != nullptr && next_i < n) q_next = prep[next_i];
     seg_open = false;
     in_piece = false;
     tb = kNoBreakpoint;
-  }
-
-  // ---- Quiet blocks ------------------------------------------------------------------------------
-  // At the start of a segment whose position in the camera's segment space is a multiple of
-  // kBlockSegments, the walk may pass the whole block of kBlockSegments segments [ta, t_last) at
-  // once. Each segment's groups move one way, so its x(t) = ln(I(t) / i_scale) lies between its x_a
-  // and x_c, hence in [x_min, x_max], and dl(t) lies in [dl(ta), dl(t_last - 1)] (leak >= 0): every
-  // Delta(t) = fl(S(x(t)) + dl(t)) of the block lies in [fl(S(x_min) + dl(ta)),
-  // fl(S(x_max) + dl(t_last - 1))] (S(x) = (x - s3_ref) + (0 - n3_ref), the monotone steps of the
-  // transition searches). When that range is inside the levels, no blindness is pending, no shot
-  // arrival falls before t_last and no 2^23-ns cut inside, the segment-by-segment walk would fire no
-  // event in the block and leave exactly this: t_search = max(t_search, t_last), one sub-step and one
-  // piece per segment, and the block's last sample as the held segment start (phase A's tests are not
-  // counted as leaf intervals).
-  BSX_HD BSX_INLINE void load_block() {
-    blk.x_min = nan_<float>();
-    if (blocks == nullptr || next_i + (kBlockSegments - 1) >= n) return;
-    const int64_t p = pos0 + next_i;
-    if ((p & (kBlockSegments - 1)) == 0) blk = blocks[p / kBlockSegments];
-  }
-
-  BSX_HD BSX_INLINE bool block_quiet() const {
-    if (!(blk.x_min == blk.x_min)) return false;
-    const int64_t tl = blk.t_last;
-    const bool free = (blind_until == kNotBlind) & (shot_next >= tl) & (tl <= t_end_) &
-                      (((ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= tl) & (leak >= 0.0f);
-    if (!free) return false;
-    const float S_hi = (blk.x_max - s3_ref) + (0.0f - n3_ref);
-    const float S_lo = (blk.x_min - s3_ref) + (0.0f - n3_ref);
-    return (S_hi + dl_at(tl - 1) < theta_on) & (S_lo + dl_at(ta) > -theta_off);
-  }
-
-  BSX_HD BSX_INLINE void skip_block() {
-    const int last = next_i + (kBlockSegments - 1);
-    n_substeps += kBlockSegments;
-    n_pieces += kBlockSegments;
-    t_search = vmax(t_search, blk.t_last);
-    load_sample(last, ta, va);
-    t_cur = ta;
-    next_i = last + 1;
-    nt = kNoBreakpoint;
-    if (next_i < n) {
-      load_sample(next_i, nt, nv);
-      if (prep != nullptr) q_next = prep[next_i];
-    }
-    load_block();
   }
 
   template <class Sink>
@@ -741,24 +635,7 @@ struct InstantLane {
     while (true) {
       if (!seg_open) {
         if (!(t_cur < t_end_)) return kDone;
-        if (block_quiet()) {
-          skip_block();
-          if (--budget <= 0) return kWalk;
-          continue;
-        }
-        tb = nt;
-        for (int g = 0; g < kMaxG; ++g)
-          if (g < G) vb[g] = nv[g];
-        q_cur = q_next;
-        has_prep = prep != nullptr && tb != kNoBreakpoint;
-        ++next_i;
-        if (next_i < n) {  // prefetch the next segment's end sample and prepared values
-          load_sample(next_i, nt, nv);
-          if (prep != nullptr) q_next = prep[next_i];
-        } else {
-          nt = kNoBreakpoint;
-        }
-        load_block();
+        open_next();
         stop = vmin(t_end_, tb);
         seg_open = true;
       }
@@ -773,7 +650,7 @@ struct InstantLane {
           ++n_pieces;
           in_piece = true;
         }
-        if (!piece_fast(t_cur, e_cur, sink, has_prep ? &q_cur : nullptr)) return kSearch;
+        if (!piece_fast(t_cur, e_cur, sink)) return kSearch;
         t_cur = e_cur;
         in_piece = false;
         continue;

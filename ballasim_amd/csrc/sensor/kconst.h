@@ -103,27 +103,10 @@ struct RunsDev {
   const BSX_GLOBAL float* irr;
 };
 
-// The reference-independent values of a whole-segment search of the instantaneous response,
-// prepared for every segment by a data-parallel pass (InstantLane::prep_segment):
-// x_a = ln(I(t_a) / i_scale) and x_c = ln(I(t_b - 1) / i_scale), each the value the exact search
-// computes itself, by the same operations, so deciding on them changes no bit.
-struct InstantPrep {
-  float x_a, x_c;
-};
-
-// Phase A's summary of kBlockSegments consecutive positions [16 b, 16 b + 16) of a camera's segment
-// space: when all of them are segments of one run, each with an end sample and no two groups moving
-// against each other, the least and greatest of their x_a and x_c, and the time of the last one's end
-// sample; otherwise x_min = x_max = NaN.
-inline constexpr int kBlockSegments = 16;
 // Phase A: consecutive positions per thread, and the positions of a 256-thread block, to which every
 // camera's segment base is aligned.
 inline constexpr int kPrepPerThread = 4;  // even: the records are stored in pairs
 inline constexpr int64_t kSegmentAlign = 256 * kPrepPerThread;
-struct InstantBlock {
-  float x_min, x_max;
-  int64_t t_last;
-};
 
 struct RawEventDev;
 
@@ -139,8 +122,6 @@ struct CameraDev {
   RunsDev runs;
   const BSX_GLOBAL uint32_t* pixel;       // the runs' pixel ids
   const BSX_GLOBAL int64_t* pos;          // the runs' segment positions (exclusive scan of the run lengths)
-  BSX_GLOBAL InstantPrep* prep;           // phase A's values (null: exact searches only)
-  BSX_GLOBAL InstantBlock* blocks;        // phase A's block summaries (null: none)
   BSX_GLOBAL int* first_loud;             // per run (null: every walk from its start)
   BSX_GLOBAL float* work;                 // per run: estimated walk cost (null: no order information)
   BSX_GLOBAL RawEventDev* ev;

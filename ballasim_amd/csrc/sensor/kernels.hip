@@ -133,33 +133,6 @@ __device__ __forceinline__ void wave_segmented_count(uint32_t* dst, int key, uin
   if (key >= 0 && (lane == 63 || kn != key) && v != 0) atomicAdd(dst + key, v);
 }
 
-// Phase A's block summaries: the kBlockSegments / kPrepPerThread lanes of positions [16 b, 16 b + 16)
-// (a block of phase A starts at a multiple of kSegmentAlign) combine their threads' x ranges; the
-// block is valid when every thread's positions all join and share one run. ok: this thread's
-// positions all joined one run, run_of: that run. The first thread of each block writes the summary,
-// when the block's first position is below total.
-__device__ __forceinline__ void block_summary(InstantBlock* blocks, int64_t q_first, int64_t total, bool ok,
-                                              int run_of, float lo, float hi, int64_t t_last) {
-  constexpr int kLanes = kBlockSegments / kPrepPerThread;
-  const int lane = __lane_id();
-  int kmin = ok ? run_of : -1 - lane, kmax = kmin;
-  for (int off = 1; off < kLanes; off <<= 1) {
-    kmin = min(kmin, __shfl_xor(kmin, off));
-    kmax = max(kmax, __shfl_xor(kmax, off));
-    lo = vmin(lo, __shfl_xor(lo, off));
-    hi = vmax(hi, __shfl_xor(hi, off));
-  }
-  const int64_t tl = __shfl(t_last, lane | (kLanes - 1));
-  if ((lane & (kLanes - 1)) == 0 && q_first < total) {
-    const bool valid = kmin == kmax && kmin >= 0;
-    InstantBlock b;
-    b.x_min = valid ? lo : nan_<float>();
-    b.x_max = valid ? hi : nan_<float>();
-    b.t_last = tl;
-    blocks[q_first / kBlockSegments] = b;
-  }
-}
-
 // The run i of position q (the last i with pos[i] <= q; pos non-decreasing), by bisection.
 __device__ __forceinline__ int64_t run_of_position(const int64_t* pos, int64_t n_runs, int64_t q) {
   int64_t lo = 0, hi = n_runs;
@@ -292,142 +265,6 @@ __device__ __forceinline__ void wave_segmented_cost_loud(float* work, int* first
   }
 }
 
-// Phase A of the instantaneous response over every segment of every camera's runs: one block of
-// kSegmentAlign consecutive positions of one camera (the segment bases are multiples of it), each
-// thread kPrepPerThread consecutive positions, so that a pixel's constants are loaded once for its
-// consecutive segments. A thread adds the costs and first loud segments of the runs that end among
-// its positions directly, and those of its last run through the wave's segmented reduction.
-template <int kMaxG>
-__global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, int n_cams, const CallDev* call) {
-  const int64_t g0 = static_cast<int64_t>(blockIdx.x) * kSegmentAlign;
-  if (g0 >= call->n_segments) return;  // beyond this call's segments (a grid sized for a larger one)
-  // The block's camera, copied to registers before any store (so that no store forces a reload).
-  const CameraDev& cd = cams[camera_at<Space::kSegments>(cams, n_cams, g0)];
-  const KConst kc = cd.kc;
-  const MapsDev mp = cd.mp;
-  const StateDev in = cd.in;
-  const RunsDev runs = cd.runs;
-  const uint32_t* const pixel = cd.pixel;
-  const int64_t* const pos = cd.pos;
-  InstantPrep* const prep = cd.prep;
-  InstantBlock* const blocks = cd.blocks;
-  float* const work = cd.work;
-  int* const first_loud = cd.first_loud;
-  const int64_t t_state = cd.t_state, total = cd.total, n_runs = cd.n_runs;
-  const int64_t q0 = g0 - cd.seg_base, qt = q0 + static_cast<int64_t>(threadIdx.x) * kPrepPerThread;
-  __shared__ BlockRuns br;
-  RunRef rr = stage_block_runs(br, pos, runs, pixel, n_runs, q0, qt, total);
-  // the partial of the run being accumulated (acc_run), and this thread's block-summary part
-  int acc_run = -1;
-  float cost_acc = 0.0f;
-  int loud_acc = INT32_MAX;
-  bool blk_ok = true;
-  int blk_run = -1;
-  float b_lo = inf_<float>(), b_hi = -inf_<float>();
-  int64_t t_last = 0;
-  InstantLane<kMaxG> L;
-  int64_t loaded = -1;    // the run whose pixel constants L holds
-  int64_t prev_end = -1;  // the position whose segment's end sample L.tb, L.vb hold (-1: none)
-  // the thread's records, stored together after the loop (one contiguous 32-byte write per thread)
-  InstantPrep rec[kPrepPerThread];
-  for (int k = 0; k < kPrepPerThread; ++k) rec[k] = InstantPrep{0.0f, 0.0f};
-  for (int k = 0; k < kPrepPerThread; ++k) {
-    const int64_t q = qt + k;
-    if (!(q < total)) {
-      blk_ok = false;
-      continue;
-    }
-    advance_run(rr, br, pos, runs, pixel, n_runs, q);
-    const int run = static_cast<int>(rr.i);
-    if (run != acc_run) {
-      if (acc_run >= 0) {
-        if (work != nullptr) atomicAdd(work + acc_run, cost_acc);
-        if (first_loud != nullptr && loud_acc != INT32_MAX) atomicMin(first_loud + acc_run, loud_acc);
-      }
-      acc_run = run;
-      cost_acc = 0.0f;
-      loud_acc = INT32_MAX;
-    }
-    if (k == 0) blk_run = run;
-    else if (blk_run != run) blk_ok = false;
-    const uint32_t p = rr.pixel;
-    if (rr.i != loaded) {
-      loaded = rr.i;
-      L.G = kc.groups;
-      L.irr_max = kc.irr_max;
-      L.dark = mp.dark[p];
-      L.scale = mp.scale[p];
-      L.leak = mp.leak[p];
-      L.theta_on = mp.theta_on[p];
-      L.theta_off = mp.theta_off[p];
-      L.i_scale = in.i_scale[p];
-      L.s3_ref = in.s3_ref[p];
-      L.n3_ref = in.n3_ref[p];
-      L.dr_ref = in.dr_ref[p];
-      L.t_ref = in.t_ref[p];
-      L.blind_until = in.blind_until[p];
-      L.t_search = in.t_search[p];
-      L.shot_next = in.shot_next[p];
-    }
-    const int64_t j = q - rr.pos;
-    const int64_t* t = runs.t + rr.row_start;
-    const float* v = runs.irr + rr.row_start * kc.groups;
-    if (j == 0) {
-      if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
-        blk_ok = false;
-        prev_end = -1;
-        continue;
-      }
-      L.ta = t_state;
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
-    } else if (prev_end == q - 1) {  // the previous position's segment ends where this one starts
-      L.ta = L.tb;
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < kc.groups) L.va[g] = L.vb[g];
-    } else {
-      L.ta = t[j - 1];
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < kc.groups) L.va[g] = L.clampv(v[(j - 1) * kc.groups + g]);
-    }
-    L.tb = t[j];
-    for (int g = 0; g < kMaxG; ++g)
-      if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
-    prev_end = q;
-    const InstantPrep r = L.prep_segment();
-    rec[k] = r;
-    const float theta = vmin(L.theta_on, L.theta_off);
-    const bool quiet = L.quiet_segment(r);
-    bool up = true, down = true;
-    for (int g = 0; g < kMaxG; ++g) {
-      if (g < kc.groups) {
-        if (L.vb[g] < L.va[g]) up = false;
-        if (L.vb[g] > L.va[g]) down = false;
-      }
-    }
-    if (!(up || down)) blk_ok = false;
-    b_lo = vmin(b_lo, vmin(r.x_a, r.x_c));
-    b_hi = vmax(b_hi, vmax(r.x_a, r.x_c));
-    t_last = L.tb;
-    if (!quiet) {
-      loud_acc = vmin(loud_acc, static_cast<int>(j));
-      // (a schedule only: the hardware reciprocal instead of a correctly rounded division)
-      cost_acc += 1.0f + kSearchWeight * fabs_(r.x_c - r.x_a) * __builtin_amdgcn_rcpf(theta);
-    }
-  }
-  if (qt + kPrepPerThread <= total) {
-    // 32-byte aligned: qt is a multiple of kPrepPerThread and the records are 8 bytes
-    float4* dst = reinterpret_cast<float4*>(prep + qt);
-    for (int k = 0; k < kPrepPerThread; k += 2)
-      dst[k / 2] = make_float4(rec[k].x_a, rec[k].x_c, rec[k + 1].x_a, rec[k + 1].x_c);
-  } else {
-    for (int k = 0; k < kPrepPerThread; ++k)
-      if (qt + k < total) prep[qt + k] = rec[k];
-  }
-  wave_segmented_cost_loud(work, first_loud, acc_run, cost_acc, loud_acc);
-  if (blocks != nullptr) block_summary(blocks, qt, total, blk_ok, blk_run, b_lo, b_hi, t_last);
-}
-
 // Phase A of the instantaneous response as a screen of every segment of every camera's runs, without
 // logarithms: one block of kSegmentAlign consecutive positions of one camera, each thread
 // kPrepPerThread consecutive positions (a pixel's constants and photocurrent bounds formed once for its
@@ -437,7 +274,7 @@ __global__ __launch_bounds__(256) void chain_prep_kernel(const CameraDev* cams, 
 // (InstantLane::screen_bounds from t_state to t_end - 1, InstantLane::screened): no integer of it fires
 // under that state. Each run's first segment that is not quiet (the walk starts there) and its walk
 // cost (one unit per loud segment plus kSearchWeight per threshold's worth of log-intensity change, fast
-// logs: a schedule only) are folded per run as in chain_prep_kernel. Nothing is written per segment.
+// logs: a schedule only) are folded per run. Nothing is written per segment.
 template <int kMaxG>
 __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams, int n_cams, const CallDev* call) {
   const int64_t g0 = static_cast<int64_t>(blockIdx.x) * kSegmentAlign;
@@ -742,20 +579,12 @@ __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams,
     L.ideal_reset = kc.ideal_reset != 0;
     L.hot = mp.hot[p] != 0;
     const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
-    L.prep = nullptr;
-    L.blocks = nullptr;
-    L.pos0 = 0;
     int loud_start = 0;
     if (row >= 0) {
       const int64_t start = runs.row_start[row];
       L.ts = runs.t + start;
       L.irr = runs.irr + start * kc.groups;
       L.n = runs.row_count[row];
-      if (cam->prep != nullptr) {
-        L.pos0 = cam->pos[row];
-        L.prep = cam->prep + L.pos0;
-        L.blocks = cam->blocks;
-      }
       if (cam->first_loud != nullptr) loud_start = vmin(cam->first_loud[row], L.n);
     } else {
       L.ts = nullptr;
@@ -929,7 +758,6 @@ __device__ __forceinline__ void cascade_lane(const CameraDev* cams, int n_cams, 
     L.hot = mp.hot[p] != 0;
     L.cc = cascade::Consts{kc.k, kc.a2, kc.a3, kc.tau_o, kc.tau_sf, kc.eps_accept, kc.eps_drop};
     const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
-    L.prep = nullptr;
     if (row >= 0) {
       const int64_t start = runs.row_start[row];
       L.ts = runs.t + start;
@@ -1043,7 +871,6 @@ __device__ __forceinline__ void noise_lane(const CameraDev* cams, int n_cams, ui
     L.mid = mid;
     L.mid_stride = mid_stride;
     const int32_t row = runs.row_of != nullptr ? runs.row_of[p] : -1;
-    L.prep = nullptr;
     if (row >= 0) {
       const int64_t start = runs.row_start[row];
       L.ts = runs.t + start;

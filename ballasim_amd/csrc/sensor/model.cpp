@@ -539,8 +539,6 @@ CameraDev SensorModel::stage(const RunsDev& runs, const uint32_t* pixel, size_t 
   cd.runs = runs;
   cd.pixel = pixel;
   cd.pos = d_pos_.as<const int64_t>();
-  cd.prep = nullptr;  // phase A screens the segments and writes no per-segment records
-  cd.blocks = nullptr;
   cd.first_loud = inst ? d_loud_.as<int>() : nullptr;
   cd.work = d_work_.as<float>();
   cd.ev = d_events_.as<RawEventDev>();

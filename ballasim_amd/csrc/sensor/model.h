@@ -198,8 +198,7 @@ class SensorModel {
   DeviceBuffer d_flags_;      // scatter flags + check counters
   DeviceBuffer d_row_of_;
   DeviceBuffer d_t_, d_irr_, d_row_start_, d_row_count_, d_frame_, d_ids_;
-  DeviceBuffer d_pos_, d_scan_, d_prep_;  // the runs' segment positions and the prepared segment values
-  DeviceBuffer d_blocks_;                 // the prepared values' block summaries
+  DeviceBuffer d_pos_, d_scan_;  // the runs' segment positions (exclusive scan of the run lengths)
   DeviceBuffer d_work_;  // per run: estimated walk cost (the heavy-first order's key)
   DeviceBuffer d_loud_;  // per run: the first segment that is not quiet under the input state
   std::unique_ptr<LaunchChain> chain_;

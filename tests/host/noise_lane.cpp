@@ -69,7 +69,6 @@ static void setup(NoiseLane<kG>& L, const HostConst& c, const float* tab, const 
   L.cc = bsx::cascade::Consts{c.k, c.a2, c.a3, c.tau_o, c.tau_sf, c.eps_accept, c.eps_drop};
   L.nd = bsx::NoiseDev{tab, c.levels, c.n_nodes, c.stride, c.ln_min, c.inv_dln, c.margin_z};
   L.margin_z = c.margin_z;
-  L.prep = nullptr;
   L.n_substeps = L.n_pieces = L.n_leaves = 0;
   L.n_attempts = L.n_settled = L.n_fallbacks = L.n_quarantines = 0;
   L.n_cells = L.n_refinements = 0;

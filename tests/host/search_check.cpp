@@ -3,8 +3,8 @@
 // (Delta running level with a threshold over long stretches), transition_search must return exactly what the
 // generic time-ordered bisection (the reference's instant_search, src/sensor/pixel.h:1884-1916)
 // returns: found, polarity, crossing time and log value. Prints the counts and the evaluations of
-// both; exit status 1 on any mismatch. With a third argument "fast", checks the whole-segment fast
-// search instead (fast_check).
+// both; exit status 1 on any mismatch. With a third argument "screen", checks the photocurrent screen
+// instead (screen_check).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,14 +17,16 @@
 
 using bsx::InstantLane;
 
-// The whole-segment fast search on random segments (one to three groups rising, falling or mixed,
-// with and without leak, Delta placed near either level): whenever it decides (0: no crossing, 1: a
-// crossing at t_a), the generic bisection over [t_a, t_b) must agree.
-static int fast_check(int trials, unsigned long long seed) {
+// The photocurrent screen (InstantLane::screen_bounds, screened) on random segments: one to three
+// groups rising, falling or mixed, with and without leak, photocurrents from 1e-3 to 1e9 of i_scale,
+// Delta at the segment's start placed near either level (inside the screen's margin, just outside it,
+// or anywhere between the levels). Whenever the screen passes a segment, the generic bisection over
+// [t_a, t_b) must find no crossing.
+static int screen_check(int trials, unsigned long long seed) {
   std::mt19937_64 rng(seed);
   std::uniform_real_distribution<double> U(0.0, 1.0);
   std::vector<uint32_t> stack(InstantLane<4>::kSearchDepth);
-  long quiet = 0, at_start = 0, undecided = 0, bad = 0;
+  long quiet = 0, near_quiet = 0, loud = 0, bad = 0;
   for (int it = 0; it < trials; ++it) {
     InstantLane<4> L{};
     L.G = 1 + it % 3;
@@ -34,14 +36,14 @@ static int fast_check(int trials, unsigned long long seed) {
     L.dark = static_cast<float>(1e-3 * U(rng) + 1e-6);
     L.scale = 1.0f;
     L.irr_max = 1e30f;
-    L.i_scale = static_cast<float>(0.5 + U(rng));
+    L.i_scale = static_cast<float>(std::exp(U(rng) * 12.0 - 6.0));
     const int64_t span = 1 + static_cast<int64_t>(U(rng) * (it % 5 == 0 ? 8e6 : 2e4));
     L.ta = static_cast<int64_t>(U(rng) * 1e9);
     L.tb = L.ta + span;
     L.t_ref = L.ta - static_cast<int64_t>(U(rng) * 5e7);
     L.dr_ref = static_cast<float>(0.2 * (U(rng) - 0.5));
-    const double swing = (U(rng) - 0.5) * 0.6;  // ln I change of the first group over the segment
-    const double I0 = 0.1 + U(rng);
+    const double swing = (U(rng) - 0.5) * (it % 3 == 0 ? 1e-3 : 0.3);  // ln I change over the segment
+    const double I0 = L.i_scale * std::exp(U(rng) * 27.0 - 7.0);
     for (int g = 0; g < L.G; ++g) {
       const double w = g == 0 ? 1.0 : 0.3 * U(rng);
       const double sg = g == 0 || U(rng) < 0.7 ? swing : -swing;  // some segments mix directions
@@ -51,42 +53,35 @@ static int fast_check(int trials, unsigned long long seed) {
     float x0;
     L.s3_ref = 0.0f;
     (void)L.s_at(L.ta, x0);
-    // Delta(t_a) near either level (inside or just beyond it) or anywhere between them
-    const double r = U(rng);
+    // Delta(t_a) near either level: within a few margins of it, or anywhere between the levels
+    const double r = U(rng), m = 0x1p-9;
     double target = (U(rng) - 0.5) * 0.2;
-    if (r < 0.3) target = L.theta_on - U(rng) * 0.02;
-    else if (r < 0.6) target = -L.theta_off + U(rng) * 0.02;
-    else if (r < 0.7) target = L.theta_on + U(rng) * 1e-3;  // firing at t_a
-    else if (r < 0.8) target = -L.theta_off - U(rng) * 1e-3;
+    if (r < 0.35) target = L.theta_on - m * (0.5 + 3.0 * U(rng)) - std::fabs(swing);
+    else if (r < 0.7) target = -L.theta_off + m * (0.5 + 3.0 * U(rng)) + std::fabs(swing);
     L.s3_ref = static_cast<float>(static_cast<double>(x0) + L.dl_at(L.ta) - target);
     if (it % 11 == 0) L.va[0] = -0.0f;  // a negative zero sample (clampv passes it)
     if (it % 13 == 0) L.dark = 0.0f;
-    const float I_start = L.photocurrent_start(), I_ta = L.photocurrent(L.ta);
-    if (std::memcmp(&I_start, &I_ta, sizeof(float)) != 0 && ++bad <= 10)
-      std::printf("photocurrent_start differs at trial %d: %a vs %a\n", it, I_start, I_ta);
-    const InstantLane<4>::Prep q = L.prep_segment();
-    InstantLane<4>::Hit h{false, false, L.ta, 0.0f};
-    const int verdict = L.fast_search(q, h);
-    if (verdict == 2) {
-      ++undecided;
+    L.screen_bounds(L.ta, L.tb - 1);
+    if (!L.screened()) {
+      ++loud;
       continue;
     }
+    ++quiet;
+    if (r < 0.7) ++near_quiet;
     const InstantLane<4>::Hit b = L.search(L.ta, L.tb, stack.data(), 1);
-    const bool ok = verdict == 0 ? !b.found : (b.found && b.t == L.ta && b.on == h.on && b.x == h.x);
-    (verdict == 0 ? quiet : at_start) += 1;
-    if (!ok && ++bad <= 10)
-      std::printf("fast mismatch at trial %d: verdict %d, bisection found %d t %lld (t_a %lld)\n", it, verdict, b.found,
+    if (b.found && ++bad <= 10)
+      std::printf("screen mismatch at trial %d: screened, but the bisection fires at %lld (t_a %lld)\n", it,
                   static_cast<long long>(b.t), static_cast<long long>(L.ta));
   }
-  std::printf("fast trials %d quiet %ld at_start %ld undecided %ld mismatches %ld\n", trials, quiet, at_start,
-              undecided, bad);
+  std::printf("screen trials %d quiet %ld near_quiet %ld loud %ld mismatches %ld\n", trials, quiet, near_quiet, loud,
+              bad);
   return bad != 0;
 }
 
 int main(int argc, char** argv) {
   const int trials = argc > 1 ? std::atoi(argv[1]) : 20000;
   const unsigned long long seed = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 12345;
-  if (argc > 3 && std::string(argv[3]) == "fast") return fast_check(trials, seed);
+  if (argc > 3 && std::string(argv[3]) == "screen") return screen_check(trials, seed);
   std::mt19937_64 rng(seed);
   std::uniform_real_distribution<double> U(0.0, 1.0);
   std::vector<uint32_t> stack(InstantLane<4>::kSearchDepth);

@@ -1,8 +1,8 @@
-// The instantaneous lane program on the host, on random runs: the walk with phase A's per-segment
-// values and block summaries (formed here by the same rules as chain_prep_kernel: the records from
-// prep_segment, the blocks' x ranges over 16 aligned positions of one run whose segments move one
-// way) must give exactly the events and final state of the walk with neither (exact searches only,
-// segment by segment). Prints the counts; exit status 1 on any difference.
+// The instantaneous lane program on the host, on random runs: the walk as the kernels run it (from
+// phase A's first loud segment, found by the log-free photocurrent screen under the input state, each
+// piece screened with the current reference before any exact search, in the walk/search alternation
+// with small budgets) must give exactly the events and final state of the walk on exact searches
+// alone, segment by segment. Prints the counts; exit status 1 on any difference.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -12,9 +12,7 @@
 
 #include "instant.h"
 
-using bsx::InstantBlock;
 using bsx::InstantLane;
-using bsx::InstantPrep;
 using bsx::RawEventDev;
 
 struct VecSink {
@@ -31,8 +29,7 @@ int main(int argc, char** argv) {
   std::mt19937_64 rng(seed);
   std::uniform_real_distribution<double> U(0.0, 1.0);
   std::normal_distribution<double> N01(0.0, 1.0);
-  std::vector<uint32_t> stack(InstantLane<kG>::kSearchDepth);
-  long n_events = 0, n_blocks_skipped = 0, n_screened = 0, bad = 0;
+  long n_events = 0, n_screened = 0, bad = 0;
   for (int it = 0; it < trials; ++it) {
     const int G = 1 + it % 2;
     const int n = 20 + static_cast<int>(U(rng) * 300);
@@ -86,76 +83,12 @@ int main(int argc, char** argv) {
                                           : INT64_MAX;
     base.shot_index = 0;
     base.n_substeps = base.n_pieces = base.n_leaves = 0;
-    // phase A: records and block summaries, the run placed at position pos0 of a segment space
-    const int64_t pos0 = static_cast<int64_t>(U(rng) * 64);
-    std::vector<InstantPrep> prep(n);
-    std::vector<float> lo(n), hi(n);
-    std::vector<int> joins(n);
-    for (int j = 0; j < n; ++j) {
-      InstantLane<kG> L = base;
-      const bool empty = j == 0 && ts[0] == t_state;
-      if (empty) {
-        prep[j] = InstantPrep{0.0f, 0.0f};
-        joins[j] = 0;
-        continue;
-      }
-      if (j == 0) {
-        L.ta = t_state;
-        for (int g = 0; g < G; ++g) L.va[g] = L.held[g];
-      } else {
-        L.ta = ts[j - 1];
-        for (int g = 0; g < G; ++g) L.va[g] = L.clampv(irr[static_cast<size_t>(j - 1) * G + g]);
-      }
-      L.tb = ts[j];
-      bool up = true, down = true;
-      for (int g = 0; g < G; ++g) {
-        L.vb[g] = L.clampv(irr[static_cast<size_t>(j) * G + g]);
-        if (L.vb[g] < L.va[g]) up = false;
-        if (L.vb[g] > L.va[g]) down = false;
-      }
-      prep[j] = L.prep_segment();
-      lo[j] = std::fmin(prep[j].x_a, prep[j].x_c);
-      hi[j] = std::fmax(prep[j].x_a, prep[j].x_c);
-      joins[j] = up || down;
-    }
-    const int64_t n_blocks = (pos0 + n + 15) / 16;
-    std::vector<InstantBlock> blocks(n_blocks, InstantBlock{NAN, NAN, 0});
-    for (int64_t b = 0; b < n_blocks; ++b) {
-      const int64_t j0 = 16 * b - pos0;
-      if (j0 < 0 || j0 + 15 >= n) continue;
-      bool ok = true;
-      float mn = lo[j0], mx = hi[j0];
-      for (int64_t j = j0; j < j0 + 16; ++j) {
-        ok = ok && joins[j];
-        mn = std::fmin(mn, lo[j]);
-        mx = std::fmax(mx, hi[j]);
-      }
-      if (ok) blocks[b] = InstantBlock{mn, mx, ts[j0 + 15]};
-    }
-    VecSink s0, s1, s2;
-    InstantLane<kG> A = base, B = base, C = base;
-    A.prep = nullptr;
-    A.blocks = nullptr;
+    VecSink s0, s2;
+    InstantLane<kG> A = base, C = base;
     A.screen = false;  // the reference: every piece by the exact search
-    B.prep = prep.data();
-    B.blocks = blocks.data();
-    B.pos0 = pos0;
     A.advance(t_end, s0);
-    // the walk with the prepared values, in the kernel's walk/search alternation with a small budget
-    // (budget 1: the walk returns at every segment or block boundary, where the skips are counted)
-    B.begin(t_end);
-    const int budget = 1 + it % 3;
-    auto step = [&]() {
-      if (!B.seg_open && B.t_cur < t_end && B.block_quiet()) ++n_blocks_skipped;
-      return B.walk(s1, budget);
-    };
-    for (int m = step(); m != InstantLane<kG>::kDone; m = step())
-      if (m == InstantLane<kG>::kSearch) B.search_step(s1, stack.data(), 1);
-    B.end();
     // the kernel's path: phase A's log-free screen of the segments under the input state (the rule of
     // chain_screen_kernel) gives the first loud segment, and the walk screens each piece from there
-    C.prep = nullptr;
-    C.blocks = nullptr;
     int first_loud = n;
     {
       InstantLane<kG> S = base;
@@ -189,24 +122,18 @@ int main(int argc, char** argv) {
         if (m == InstantLane<kG>::kSearch) C.search_step(s2, stack2, 1);
       C.end();
     }
-    bool ok = s0.ev.size() == s1.ev.size() && s0.ev.size() == s2.ev.size();
+    bool ok = s0.ev.size() == s2.ev.size();
     for (size_t k = 0; ok && k < s0.ev.size(); ++k) ok = same_event(s0.ev[k], s2.ev[k]);
     ok = ok && A.t_search == C.t_search && A.t_ref == C.t_ref && A.blind_until == C.blind_until &&
          std::memcmp(&A.s3_ref, &C.s3_ref, 4) == 0 && std::memcmp(&A.dr_ref, &C.dr_ref, 4) == 0 &&
          A.flags == C.flags && A.shot_next == C.shot_next && A.shot_index == C.shot_index &&
          A.n_substeps == C.n_substeps && A.n_pieces == C.n_pieces && std::memcmp(A.held, C.held, sizeof(A.held)) == 0;
-    for (size_t k = 0; ok && k < s0.ev.size(); ++k) ok = same_event(s0.ev[k], s1.ev[k]);
-    ok = ok && A.t_search == B.t_search && A.t_ref == B.t_ref && A.blind_until == B.blind_until &&
-         std::memcmp(&A.s3_ref, &B.s3_ref, 4) == 0 && std::memcmp(&A.dr_ref, &B.dr_ref, 4) == 0 &&
-         A.flags == B.flags && A.shot_next == B.shot_next && A.shot_index == B.shot_index &&
-         A.n_substeps == B.n_substeps && A.n_pieces == B.n_pieces && std::memcmp(A.held, B.held, sizeof(A.held)) == 0;
     n_events += static_cast<long>(s0.ev.size());
     if (!ok && ++bad <= 10)
       std::printf("difference at trial %d: events %zu vs %zu, t_search %lld vs %lld, pieces %u vs %u\n", it, s0.ev.size(),
-                  s1.ev.size(), static_cast<long long>(A.t_search), static_cast<long long>(B.t_search), A.n_pieces,
-                  B.n_pieces);
+                  s2.ev.size(), static_cast<long long>(A.t_search), static_cast<long long>(C.t_search), A.n_pieces,
+                  C.n_pieces);
   }
-  std::printf("walk trials %d events %ld blocks_skipped %ld screened %ld differences %ld\n", trials, n_events,
-              n_blocks_skipped, n_screened, bad);
+  std::printf("walk trials %d events %ld screened %ld differences %ld\n", trials, n_events, n_screened, bad);
   return bad != 0;
 }
