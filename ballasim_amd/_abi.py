@@ -178,6 +178,12 @@ SIGNATURES = {
     "bsx_output_destroy": (C.c_int, [P]),
     "bsx_output_window": (C.c_int, [P, C.POINTER(DeviceEvents), C.c_int64, P, P]),
     "bsx_output_flush": (C.c_int, [P, P, P]),
+    "bsx_event_stream_create": (C.c_int, [C.c_size_t, P, P, C.POINTER(P)]),
+    "bsx_event_stream_destroy": (C.c_int, [P]),
+    "bsx_output_window_to_stream": (C.c_int, [P, C.POINTER(DeviceEvents), C.c_int64, P, P, P, P]),
+    "bsx_output_flush_to_stream": (C.c_int, [P, P, P, P]),
+    "bsx_event_stream_finish": (C.c_int, [P, P]),
+    "bsx_event_stream_stats": (C.c_int, [P, P]),
     "bsx_scene_render_requests": (C.c_int, [P, C.POINTER(View), C.c_int64, P, C.c_size_t, P, C.c_int32, P, P, P, P]),
     "bsx_draw_maps": (C.c_int, [C.POINTER(Params), C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, P, P]),
     "bsx_sensor_create_from_constants": (C.c_int, [P, P, C.c_size_t, P, C.c_size_t, C.POINTER(P)]),

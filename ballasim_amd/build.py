@@ -26,6 +26,7 @@ SOURCES = [
     "sensor/rig.cpp",
     "mlp/proposer.cpp",
     "output/output.cpp",
+    "output/stream.cpp",
     "output/output.hip",
     "mlp/mlp.hip",
     "sensor/kernels.hip",

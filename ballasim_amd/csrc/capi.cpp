@@ -18,6 +18,7 @@
 #include "../../include/bsx/bsx_sensor.h"
 #include "mlp/proposer.h"
 #include "output/output.h"
+#include "output/stream.h"
 #include "sensor/kernels.h"
 #include "sensor/model.h"
 #include "sensor/noise_tables.h"
@@ -574,6 +575,15 @@ bsx::OutputStage& output_of(bsx_output* o) {
   return *o->o;
 }
 
+static_assert(sizeof(bsx_stream_event) == sizeof(bsx::StreamEventDev), "stream record layout");
+static_assert(offsetof(bsx_stream_event, x) == offsetof(bsx::StreamEventDev, x), "stream record layout");
+static_assert(offsetof(bsx_stream_event, flags) == offsetof(bsx::StreamEventDev, flags), "stream record layout");
+
+bsx::EventStream& stream_of(bsx_event_stream* s) {
+  if (s == nullptr || s->s == nullptr) throw std::invalid_argument("bsx: null event stream handle");
+  return *s->s;
+}
+
 void fill_output(const bsx::OutputColumns& c, size_t held, bsx_output_events* out) {
   *out = bsx_output_events{c.x.size(),          c.x.data(),         c.y.data(),       c.p.data(),
                            c.t_us.data(),       c.t_ns_rem.data(),  c.t_reported.data(), c.t_cross.data(),
@@ -620,6 +630,63 @@ int bsx_output_flush(bsx_output* o, void* stream, bsx_output_events* out) {
     bsx::OutputStage& s = output_of(o);
     const bsx::OutputColumns& c = s.flush(static_cast<hipStream_t>(stream));
     fill_output(c, s.held(), out);
+  });
+}
+
+int bsx_event_stream_create(size_t chunk_bytes, bsx_chunk_sink sink, void* user, bsx_event_stream** out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    *out = nullptr;
+    auto* s = new bsx::EventStream(chunk_bytes, reinterpret_cast<bsx::EventStream::Sink>(sink), user);
+    *out = new bsx_event_stream{s};
+  });
+}
+
+int bsx_event_stream_destroy(bsx_event_stream* s) {
+  return guarded([&] {
+    if (s == nullptr) return;
+    delete s->s;
+    delete s;
+  });
+}
+
+int bsx_output_window_to_stream(bsx_output* o, const bsx_device_events* events, int64_t t_w, void* stream,
+                                bsx_event_stream* s, size_t* released, size_t* held) {
+  return guarded([&] {
+    need_ptr(events, "events");
+    bsx::OutputStage& st = output_of(o);
+    bsx::EventStream& es = stream_of(s);
+    const auto hs = static_cast<hipStream_t>(stream);
+    const bsx::OutEventDev* ev = nullptr;
+    const size_t m = st.window_device(reinterpret_cast<const bsx::RawEventDev*>(events->data), events->count, t_w,
+                                      hs, &ev);
+    es.push(ev, m, hs);
+    if (released != nullptr) *released = m;
+    if (held != nullptr) *held = st.held();
+  });
+}
+
+int bsx_output_flush_to_stream(bsx_output* o, void* stream, bsx_event_stream* s, size_t* released) {
+  return guarded([&] {
+    bsx::OutputStage& st = output_of(o);
+    bsx::EventStream& es = stream_of(s);
+    const auto hs = static_cast<hipStream_t>(stream);
+    const bsx::OutEventDev* ev = nullptr;
+    const size_t m = st.flush_device(hs, &ev);
+    es.push(ev, m, hs);
+    if (released != nullptr) *released = m;
+  });
+}
+
+int bsx_event_stream_finish(bsx_event_stream* s, void* stream) {
+  return guarded([&] { stream_of(s).finish(static_cast<hipStream_t>(stream)); });
+}
+
+int bsx_event_stream_stats(const bsx_event_stream* s, bsx_stream_stats* out) {
+  return guarded([&] {
+    need_ptr(out, "out");
+    const bsx::EventStreamStats st = stream_of(const_cast<bsx_event_stream*>(s)).stats();
+    *out = bsx_stream_stats{st.chunks, st.events, st.bytes, st.wait_ms, st.copy_ms};
   });
 }
 

@@ -8,6 +8,7 @@ class Proposer;
 class Scene;
 class LearnedSampler;
 class OutputStage;
+class EventStream;
 }  // namespace bsx
 
 struct bsx_sensor {
@@ -27,4 +28,7 @@ struct bsx_learned_sampler {
 };
 struct bsx_output {
   bsx::OutputStage* o;
+};
+struct bsx_event_stream {
+  bsx::EventStream* s;
 };

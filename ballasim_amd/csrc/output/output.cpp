@@ -49,7 +49,7 @@ OutputStage::OutputStage(uint32_t width, uint32_t height, const Readout& readout
   scratch_.ensure(kOutScratch * 8);
 }
 
-const OutputColumns& OutputStage::window(const RawEventDev* raw, size_t n, int64_t t_w, hipStream_t stream) {
+size_t OutputStage::take_window(const RawEventDev* raw, size_t n, int64_t t_w, hipStream_t stream, int64_t* cut) {
   if (n > 0 && raw == nullptr) throw std::invalid_argument("OutputStage.window: null events");
   const size_t n_max = n_held_ + n;
   if (n_max > 0xffffffffull) throw std::invalid_argument("OutputStage.window: at most 2^32 - 1 events");
@@ -71,30 +71,54 @@ const OutputColumns& OutputStage::window(const RawEventDev* raw, size_t n, int64
   hip_check(launch_out_build(raw, n, width_, readout_, crop_on_, crop_, held_.as<OutEventDev>() + n_held_, sc, stream),
             "build events");
   hip_check(hipStreamSynchronize(stream), "build events");  // init[] must outlive its copy
-  const int64_t cut = output_cut(readout_, t_w + kPlainSegmentNs);
+  *cut = output_cut(readout_, t_w + kPlainSegmentNs);
   hip_check(launch_out_minmax(held_.as<const OutEventDev>(), n_held_, n_max, sc, stream), "min/max");
   unsigned long long h[kOutScratch];
   hip_check(hipMemcpyAsync(h, scratch_.get(), sizeof(h), hipMemcpyDeviceToHost, stream), "read scratch");
   hip_check(hipStreamSynchronize(stream), "output window");
-  const size_t total = n_held_ + static_cast<size_t>(h[0]);
-  return release(total, cut, stream);
+  return n_held_ + static_cast<size_t>(h[0]);
 }
 
-const OutputColumns& OutputStage::flush(hipStream_t stream) {
+size_t OutputStage::take_flush(hipStream_t stream) {
   unsigned long long init[kOutScratch] = {0, ~0ull, 0, ~0ull, 0, 0, 0, 0};
   hip_check(hipMemcpyAsync(scratch_.get(), init, sizeof(init), hipMemcpyHostToDevice, stream), "scratch");
   hip_check(launch_out_minmax(held_.as<const OutEventDev>(), n_held_, n_held_, scratch_.as<unsigned long long>(),
                               stream),
             "min/max");
   hip_check(hipStreamSynchronize(stream), "output flush");
-  return release(n_held_, INT64_MAX, stream);
+  return n_held_;
 }
 
-const OutputColumns& OutputStage::release(size_t total, int64_t cut, hipStream_t stream) {
+const OutputColumns& OutputStage::window(const RawEventDev* raw, size_t n, int64_t t_w, hipStream_t stream) {
+  int64_t cut = 0;
+  const size_t total = take_window(raw, n, t_w, stream, &cut);
+  return download(release_sorted(total, cut, stream), stream);
+}
+
+const OutputColumns& OutputStage::flush(hipStream_t stream) {
+  const size_t total = take_flush(stream);
+  return download(release_sorted(total, INT64_MAX, stream), stream);
+}
+
+size_t OutputStage::window_device(const RawEventDev* raw, size_t n, int64_t t_w, hipStream_t stream,
+                                  const OutEventDev** released) {
+  int64_t cut = 0;
+  const size_t total = take_window(raw, n, t_w, stream, &cut);
+  const size_t m = release_sorted(total, cut, stream);
+  *released = sorted_.as<const OutEventDev>();
+  return m;
+}
+
+size_t OutputStage::flush_device(hipStream_t stream, const OutEventDev** released) {
+  const size_t m = release_sorted(take_flush(stream), INT64_MAX, stream);
+  *released = sorted_.as<const OutEventDev>();
+  return m;
+}
+
+size_t OutputStage::release_sorted(size_t total, int64_t cut, hipStream_t stream) {
   if (total == 0) {
     n_held_ = 0;
-    out_.resize(0);
-    return out_;
+    return 0;
   }
   unsigned long long h[kOutScratch];
   hip_check(hipMemcpyAsync(h, scratch_.get(), sizeof(h), hipMemcpyDeviceToHost, stream), "read scratch");
@@ -129,19 +153,6 @@ const OutputColumns& OutputStage::release(size_t total, int64_t cut, hipStream_t
             "read released");
   hip_check(hipStreamSynchronize(stream), "output gather");
   const size_t m = static_cast<size_t>(ready);
-  cx_.ensure(std::max<size_t>(m, 1) * 2);
-  cy_.ensure(std::max<size_t>(m, 1) * 2);
-  cp_.ensure(std::max<size_t>(m, 1));
-  ct_.ensure(std::max<size_t>(m, 1) * 8);
-  crem_.ensure(std::max<size_t>(m, 1) * 2);
-  ctr_.ensure(std::max<size_t>(m, 1) * 8);
-  ctc_.ensure(std::max<size_t>(m, 1) * 8);
-  clab_.ensure(std::max<size_t>(m, 1));
-  cfl_.ensure(std::max<size_t>(m, 1) * 2);
-  hip_check(launch_out_columns(sorted_.as<const OutEventDev>(), m, t_offset_us_, cx_.as<uint16_t>(), cy_.as<uint16_t>(),
-                               cp_.as<uint8_t>(), ct_.as<int64_t>(), crem_.as<uint16_t>(), ctr_.as<int64_t>(),
-                               ctc_.as<int64_t>(), clab_.as<uint8_t>(), cfl_.as<uint16_t>(), stream),
-            "columns");
   // The unreleased rest, in output order, becomes the held buffer.
   n_held_ = total - m;
   if (n_held_ > 0) {
@@ -149,21 +160,37 @@ const OutputColumns& OutputStage::release(size_t total, int64_t cut, hipStream_t
                              hipMemcpyDeviceToDevice, stream),
               "hold");
   }
+  return m;
+}
+
+const OutputColumns& OutputStage::download(size_t m, hipStream_t stream) {
   out_.resize(m);
-  if (m > 0) {
-    const auto get = [&](void* dst, const DeviceBuffer& src, size_t bytes) {
-      hip_check(hipMemcpyAsync(dst, src.get(), bytes, hipMemcpyDeviceToHost, stream), "download columns");
-    };
-    get(out_.x.data(), cx_, m * 2);
-    get(out_.y.data(), cy_, m * 2);
-    get(out_.p.data(), cp_, m);
-    get(out_.t_us.data(), ct_, m * 8);
-    get(out_.t_ns_rem.data(), crem_, m * 2);
-    get(out_.t_reported.data(), ctr_, m * 8);
-    get(out_.t_cross.data(), ctc_, m * 8);
-    get(out_.label.data(), clab_, m);
-    get(out_.flags.data(), cfl_, m * 2);
-  }
+  if (m == 0) return out_;
+  cx_.ensure(m * 2);
+  cy_.ensure(m * 2);
+  cp_.ensure(m);
+  ct_.ensure(m * 8);
+  crem_.ensure(m * 2);
+  ctr_.ensure(m * 8);
+  ctc_.ensure(m * 8);
+  clab_.ensure(m);
+  cfl_.ensure(m * 2);
+  hip_check(launch_out_columns(sorted_.as<const OutEventDev>(), m, t_offset_us_, cx_.as<uint16_t>(), cy_.as<uint16_t>(),
+                               cp_.as<uint8_t>(), ct_.as<int64_t>(), crem_.as<uint16_t>(), ctr_.as<int64_t>(),
+                               ctc_.as<int64_t>(), clab_.as<uint8_t>(), cfl_.as<uint16_t>(), stream),
+            "columns");
+  const auto get = [&](void* dst, const DeviceBuffer& src, size_t bytes) {
+    hip_check(hipMemcpyAsync(dst, src.get(), bytes, hipMemcpyDeviceToHost, stream), "download columns");
+  };
+  get(out_.x.data(), cx_, m * 2);
+  get(out_.y.data(), cy_, m * 2);
+  get(out_.p.data(), cp_, m);
+  get(out_.t_us.data(), ct_, m * 8);
+  get(out_.t_ns_rem.data(), crem_, m * 2);
+  get(out_.t_reported.data(), ctr_, m * 8);
+  get(out_.t_cross.data(), ctc_, m * 8);
+  get(out_.label.data(), clab_, m);
+  get(out_.flags.data(), cfl_, m * 2);
   hip_check(hipStreamSynchronize(stream), "output columns");
   return out_;
 }

@@ -29,6 +29,16 @@ struct OutEventDev {
 };
 static_assert(sizeof(OutEventDev) == 24, "OutEventDev layout");
 
+// One released event as the event stream carries it (16 bytes): the reported time on the sensor
+// clock in ns, the pixel, the polarity, the S1 label and the sensor's flags.
+struct StreamEventDev {
+  int64_t t_reported;
+  uint16_t x, y;
+  uint8_t p, label;
+  uint16_t flags;
+};
+static_assert(sizeof(StreamEventDev) == 16, "StreamEventDev layout");
+
 struct CropBox {
   int32_t x0, y0, x1, y1;  // kept: x0 <= x < x1, y0 <= y < y1
 };
@@ -54,10 +64,21 @@ class OutputStage {
   const OutputColumns& window(const RawEventDev* raw, size_t n, int64_t t_w, hipStream_t stream);
   // Every held event, in output order (the end of a run).
   const OutputColumns& flush(hipStream_t stream);
+  // window and flush with the released events left on the device, in output order: returns their
+  // count; *released points at them until the stage's next call (work queued on stream).
+  size_t window_device(const RawEventDev* raw, size_t n, int64_t t_w, hipStream_t stream,
+                       const OutEventDev** released);
+  size_t flush_device(hipStream_t stream, const OutEventDev** released);
   size_t held() const { return n_held_; }
 
  private:
-  const OutputColumns& release(size_t total, int64_t cut, hipStream_t stream);
+  // Sorts the total events of held_, moves those reported before cut to the front of sorted_ (their
+  // count returned) and the rest back to held_.
+  size_t release_sorted(size_t total, int64_t cut, hipStream_t stream);
+  const OutputColumns& download(size_t m, hipStream_t stream);
+  // The window's raw events joined to the held ones; returns the total and sets *cut.
+  size_t take_window(const RawEventDev* raw, size_t n, int64_t t_w, hipStream_t stream, int64_t* cut);
+  size_t take_flush(hipStream_t stream);
 
   uint32_t width_, height_;
   int y_bits_;

@@ -112,6 +112,14 @@ __global__ void columns_kernel(const OutEventDev* ev, size_t n, int64_t t_offset
   flags[i] = o.flags;
 }
 
+// The compact 16-byte records of ev[0 .. n) (one thread per event; 16-byte stores).
+__global__ void pack_kernel(const OutEventDev* ev, size_t n, StreamEventDev* dst) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const OutEventDev o = ev[i];
+  dst[i] = StreamEventDev{o.t_reported, o.x, o.y, o.p, o.label, o.flags};
+}
+
 inline unsigned grid(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
 }  // namespace
@@ -162,6 +170,12 @@ hipError_t launch_out_columns(const OutEventDev* ev, size_t n, int64_t t_offset_
   if (n == 0) return hipSuccess;
   columns_kernel<<<grid(n), 256, 0, stream>>>(ev, n, t_offset_us, x, y, p, t_us, t_ns_rem, t_rep, t_cross, label,
                                               flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_out_pack(const OutEventDev* ev, size_t n, StreamEventDev* dst, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  pack_kernel<<<grid(n), 256, 0, stream>>>(ev, n, dst);
   return hipGetLastError();
 }
 

@@ -31,4 +31,7 @@ hipError_t launch_out_columns(const OutEventDev* ev, size_t n, int64_t t_offset_
                               uint8_t* p, int64_t* t_us, uint16_t* t_ns_rem, int64_t* t_rep, int64_t* t_cross,
                               uint8_t* label, uint16_t* flags, hipStream_t stream);
 
+// The compact stream records of ev[0 .. n) at dst.
+hipError_t launch_out_pack(const OutEventDev* ev, size_t n, StreamEventDev* dst, hipStream_t stream);
+
 }  // namespace bsx

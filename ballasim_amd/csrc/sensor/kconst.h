@@ -163,7 +163,9 @@ struct RawEventDev {
 };
 static_assert(sizeof(RawEventDev) == 32, "RawEventDev must keep the reference's 32-byte RawEvent layout");
 
-// Per-call work counters, summed over a wave before one atomic add each.
+// Per-call work counters, summed over a wave before one atomic add each. kStClamps and kStNonFinite
+// count the input samples of the call's runs (the clamp changes a value; a value is not finite) and
+// are counted by the chain's phase A, before the walk.
 enum StatIdx : int {
   kStSubsteps = 0,
   kStAttempts,
@@ -175,6 +177,7 @@ enum StatIdx : int {
   kStClamps,
   kStQuarantines,
   kStPieces,
+  kStNonFinite,
   kStCount
 };
 

@@ -22,6 +22,21 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
   if (__lane_id() == 0 && s != 0) atomicAdd(dst, s);
 }
 
+// The input check of one sample value, as advance_runs makes it on the host: a value that is not
+// finite, or one the clamp changes.
+struct InputTally {
+  uint32_t bad = 0, clamped = 0;
+  __device__ __forceinline__ void add(float x, float vmax_) {
+    const bool finite = x - x == 0.0f;
+    bad += finite ? 0u : 1u;
+    clamped += finite && clamp_irr(x, vmax_) != x ? 1u : 0u;
+  }
+  __device__ __forceinline__ void flush(unsigned long long* counters) {
+    wave_add(counters + 1 + kStNonFinite, bad);
+    wave_add(counters + 1 + kStClamps, clamped);
+  }
+};
+
 template <int kMaxG>
 __global__ __launch_bounds__(256) void init_kernel(KConst kc, MapsDev mp, StateDev st, const float* irr0, int64_t t0,
                                                    unsigned long long* counters) {
@@ -274,7 +289,11 @@ __device__ __forceinline__ void wave_segmented_cost_loud(float* work, int* first
 // (InstantLane::screen_bounds from t_state to t_end - 1, InstantLane::screened): no integer of it fires
 // under that state. Each run's first segment that is not quiet (the walk starts there) and its walk
 // cost (one unit per loud segment plus kSearchWeight per threshold's worth of log-intensity change, fast
-// logs: a schedule only) are folded per run. Nothing is written per segment.
+// logs: a schedule only) are folded per run. Nothing is written per segment. The kernel also makes the
+// call's input check: every value of every run's samples is read here exactly once (a segment's end
+// sample; a run's first sample when it lies at the model time and ends no segment), and the
+// camera's kStNonFinite and kStClamps counters take the values that are not finite or that the clamp
+// changes.
 template <int kMaxG>
 __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams, int n_cams, const CallDev* call) {
   const int64_t g0 = static_cast<int64_t>(blockIdx.x) * kSegmentAlign;
@@ -295,6 +314,7 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
   int acc_run = -1;
   float cost_acc = 0.0f;
   int loud_acc = INT32_MAX;
+  InputTally tally;
   InstantLane<kMaxG> L;
   bool usable = false;    // the pixel's screen applies (leak >= 0, n3_ref = 0, not blind)
   int64_t loaded = -1;    // the run whose pixel constants L holds
@@ -338,6 +358,8 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
     const float* v = runs.irr + rr.row_start * kc.groups;
     if (j == 0) {
       if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
+        for (int g = 0; g < kMaxG; ++g)
+          if (g < kc.groups) tally.add(v[g], kc.irr_max);
         prev_end = -1;
         continue;
       }
@@ -354,8 +376,13 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
         if (g < kc.groups) L.va[g] = L.clampv(v[(j - 1) * kc.groups + g]);
     }
     L.tb = t[j];
-    for (int g = 0; g < kMaxG; ++g)
-      if (g < kc.groups) L.vb[g] = L.clampv(v[j * kc.groups + g]);
+    for (int g = 0; g < kMaxG; ++g) {
+      if (g < kc.groups) {
+        const float x = v[j * kc.groups + g];
+        tally.add(x, kc.irr_max);
+        L.vb[g] = L.clampv(x);
+      }
+    }
     prev_end = q;
     const bool quiet = usable & (L.shot_next >= L.tb) &
                        (((L.ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= L.tb) & L.screened();
@@ -374,21 +401,26 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
     }
   }
   wave_segmented_cost_loud(work, first_loud, acc_run, cost_acc, loud_acc);
+  tally.flush(cd.counters);
 }
 
 // Before phase A: zero each camera's counters, the order histogram and the queue counter (block 0),
 // and set every run's cost to 0 and first loud segment to none (one thread per run, cameras' run
-// bases multiples of 256).
-__device__ __forceinline__ void reset_counters(const CameraDev* cams, int n_cams, unsigned int* queue) {
-  for (int k = threadIdx.x; k < n_cams * (1 + kStCount); k += blockDim.x)
-    cams[k / (1 + kStCount)].counters[k % (1 + kStCount)] = 0ull;
+// bases multiples of 256). walk_only: the walk's counters only, phase A's input counts kept.
+__device__ __forceinline__ void reset_counters(const CameraDev* cams, int n_cams, unsigned int* queue,
+                                               bool walk_only) {
+  for (int k = threadIdx.x; k < n_cams * (1 + kStCount); k += blockDim.x) {
+    const int idx = k % (1 + kStCount);
+    if (walk_only && (idx == 1 + kStClamps || idx == 1 + kStNonFinite)) continue;
+    cams[k / (1 + kStCount)].counters[idx] = 0ull;
+  }
   if (threadIdx.x == 0 && queue != nullptr) *queue = 0u;
 }
 
 __global__ __launch_bounds__(256) void chain_init_kernel(const CameraDev* cams, int n_cams, const CallDev* call,
                                                          uint32_t* hist, unsigned int* queue) {
   if (blockIdx.x == 0) {
-    reset_counters(cams, n_cams, queue);
+    reset_counters(cams, n_cams, queue, false);
     if (threadIdx.x < 128) hist[threadIdx.x] = 0u;
   }
   const int64_t g0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
@@ -401,9 +433,9 @@ __global__ __launch_bounds__(256) void chain_init_kernel(const CameraDev* cams, 
   }
 }
 
-// Before a re-run of the walk: the counters and the queue counter only.
+// Before a re-run of the walk: the walk's counters and the queue counter only.
 __global__ void chain_reset_kernel(const CameraDev* cams, int n_cams, unsigned int* queue) {
-  reset_counters(cams, n_cams, queue);
+  reset_counters(cams, n_cams, queue, true);
 }
 
 // The pixel order of a walk: pixels by estimated cost, heaviest first (64 logarithmic buckets, bucket
@@ -525,6 +557,14 @@ struct RigSink {
     }
   }
 };
+
+// Whether phase A found a value that is not finite in some camera's runs: the call is then refused
+// and the walk advances no pixel (read once per block, before any lane work).
+__device__ __forceinline__ bool call_refused(const CameraDev* cams, int n_cams) {
+  bool refused = false;
+  for (int c = 0; c < n_cams; ++c) refused |= cams[c].counters[1 + kStNonFinite] != 0;
+  return refused;
+}
 
 // Adds each lane's v to counters[idx] of its camera: one atomic per camera present in the wave
 // (cam_index < 0: no contribution). Called by every lane of the wave.
@@ -681,6 +721,7 @@ template <int kMaxG>
 __global__ __launch_bounds__(kInstantBlock) __attribute__((amdgpu_waves_per_eu(3))) void instant_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
                                                                    const uint32_t* order, unsigned int* queue,
                                                                    WalkTuning tu, const CallDev* call) {
+  if (call_refused(cams, n_cams)) return;
   // The lanes' bisection stacks, one column per lane (conflict-free: a level is a row of the block;
   // blockDim.x <= kInstantBlock by the launch bound).
   __shared__ uint32_t stacks[InstantLane<kMaxG>::kSearchDepth * kInstantBlock];
@@ -692,7 +733,8 @@ __global__ __launch_bounds__(kInstantBlock) __attribute__((amdgpu_waves_per_eu(3
 
 // Estimated cost of each run's cascade walk for the heavy-first pixel order, over every segment of
 // every camera's runs: one unit per segment plus kSearchWeight per threshold's worth of
-// log-intensity change (fast logs: a schedule only).
+// log-intensity change (fast logs: a schedule only). Segment j's end sample is sample j of its run,
+// so the kernel also makes the call's input check as chain_screen_kernel does.
 __global__ __launch_bounds__(256) void chain_cost_kernel(const CameraDev* cams, int n_cams, const CallDev* call) {
   const int64_t g0 = static_cast<int64_t>(blockIdx.x) * blockDim.x;
   if (g0 >= call->n_segments) return;
@@ -703,6 +745,7 @@ __global__ __launch_bounds__(256) void chain_cost_kernel(const CameraDev* cams, 
   const MapsDev& mp = c.mp;
   int run = -1;
   float cost = 0.0f;
+  InputTally tally;
   if (q < c.total) {
     const int64_t i = i_q, j = q - c.pos[i];
     run = static_cast<int>(i);
@@ -710,13 +753,16 @@ __global__ __launch_bounds__(256) void chain_cost_kernel(const CameraDev* cams, 
     const float* v = c.runs.irr + c.runs.row_start[i] * kc.groups;
     float Ia = mp.dark[p], Ib = mp.dark[p];
     for (int g = 0; g < kc.groups; ++g) {
+      const float x = v[j * kc.groups + g];
+      tally.add(x, kc.irr_max);
       Ia += mp.scale[p] * (j == 0 ? c.in.held[static_cast<int64_t>(g) * kc.n_pixels + p] : v[(j - 1) * kc.groups + g]);
-      Ib += mp.scale[p] * v[j * kc.groups + g];
+      Ib += mp.scale[p] * x;
     }
     const float theta = vmin(mp.theta_on[p], mp.theta_off[p]);
     cost = 1.0f + kSearchWeight * fabs_(__logf(Ib) - __logf(Ia)) / theta;
   }
   if (c.work != nullptr) wave_segmented_add(c.work, run, cost);
+  tally.flush(c.counters);
 }
 
 // The cascade without noise: lane i of the launch's pixel order takes pixel order[i] of its camera
@@ -820,6 +866,7 @@ template <int kMaxG>
 __global__ __launch_bounds__(256) void cascade_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
                                                            const uint32_t* order, unsigned int* queue, WalkTuning tu,
                                                            const CallDev* call) {
+  if (call_refused(cams, n_cams)) return;
   const int64_t t_end = call->t_end;
   walk_groups(n_total, queue, tu, [&](uint32_t i) { cascade_lane<kMaxG>(cams, n_cams, n_total, order, i, t_end); });
 }
@@ -950,6 +997,7 @@ template <int kMaxG>
 __global__ __launch_bounds__(256) void noise_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
                                                          const uint32_t* order, unsigned int* queue, WalkTuning tu,
                                                          const CallDev* call) {
+  if (call_refused(cams, n_cams)) return;
   // The lanes' path memory of the tree walk, one column per lane (blockDim.x <= 256 by the launch
   // bound).
   __shared__ float mid[noise::kMidSlots * 256];

@@ -601,6 +601,7 @@ void SensorModel::run_advance(const RunsDev& runs, const uint32_t* pixel, size_t
     hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event time");
     stats_.kernel_ms += ms;
     ++stats_.launches;
+    refuse_non_finite(c);
     if (c[0] <= cap_) break;
     grow_events(c[0], cams[0]);
     hip_check(hipEventRecord(ev0_, stream), "event record");
@@ -621,15 +622,9 @@ void SensorModel::enqueue_device_checks(const bsx_device_runs& runs, int64_t t_e
   hip_check(launch_scatter_rows(runs.pixel, runs.row_start, runs.row_count, runs.n, runs.n_samples,
                                 d_row_of_.as<int32_t>(), static_cast<uint32_t>(P_), flags, stream),
             "scatter rows");
-  unsigned long long* chk = reinterpret_cast<unsigned long long*>(d_flags_.as<char>() + 16);
-  hip_check(launch_fill_u64(chk, 2, 0, stream), "fill");
-  hip_check(launch_check_runs(runs.irr, runs.row_start, runs.row_count, runs.n, runs.n_samples, G_, irr_max_, chk,
-                            stream),
-            "check samples");
   hip_check(hipMemcpyAsync(check_flags_, d_flags_.get(), 16, hipMemcpyDeviceToHost, stream), "read flags");
-  hip_check(hipMemcpyAsync(check_samples_, chk, 16, hipMemcpyDeviceToHost, stream), "read checks");
   // The scan reads row_count of runs the scatter may have refused; its result is used only if every
-  // check passes.
+  // check passes. The samples' values are checked by the chain's phase A (refuse_non_finite).
   RunsDev r{};
   r.row_count = runs.row_count;
   enqueue_scan(r, runs.n, stream);
@@ -639,16 +634,19 @@ void SensorModel::finish_device_checks() {
   if (check_flags_[0] != 0) throw std::invalid_argument("SensorModel.advance: pixel id out of range");
   if (check_flags_[1] != 0) throw std::invalid_argument("SensorModel.advance: duplicate pixel id");
   if (check_flags_[2] != 0) throw std::invalid_argument("SensorModel.advance: sample run out of range");
-  if (check_samples_[0] != 0) {
-    throw std::invalid_argument("SensorModel.advance_device: non-finite irradiance in the runs");
-  }
+}
+
+void SensorModel::refuse_non_finite(const unsigned long long* c) {
+  if (c[1 + kStNonFinite] == 0) return;
+  initialized_ = true;  // the walk advanced no pixel and nothing was committed
+  throw std::invalid_argument("SensorModel.advance: " + std::to_string(c[1 + kStNonFinite]) +
+                              " non-finite irradiance samples");
 }
 
 bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream) {
   enqueue_device_checks(runs, t_end_ns, stream);
   hip_check(hipStreamSynchronize(stream), "checks");
   finish_device_checks();
-  stats_.input_clamps += check_samples_[1];
   RunsDev r{};
   r.row_of = d_row_of_.as<const int32_t>();
   r.row_start = runs.row_start;
@@ -657,6 +655,7 @@ bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64
   r.irr = runs.irr;
   initialized_ = false;
   run_advance(r, runs.pixel, runs.n, t_end_ns, stream, true);
+  stats_.input_clamps += counters_host_[1 + kStClamps];
   initialized_ = true;
   return bsx_device_events{reinterpret_cast<const bsx_raw_event*>(d_events_.get()),
                            static_cast<size_t>(counters_host_[0])};
@@ -697,6 +696,10 @@ uint64_t SensorModel::advance_device_noalloc(const bsx_device_runs& runs, int64_
   hip_check(hipMemcpyAsync(check_samples_, chk, 16, hipMemcpyDeviceToHost, stream), "read checks");
   hip_check(hipStreamSynchronize(stream), "checks");
   finish_device_checks();
+  if (check_samples_[0] != 0) {
+    throw std::invalid_argument("SensorModel.advance: " + std::to_string(check_samples_[0]) +
+                                " non-finite irradiance samples");
+  }
   stats_.input_clamps += check_samples_[1];
   // one walk over every pixel in pixel order, exact searches only
   CameraDev& cd = noalloc_cam_;

@@ -142,11 +142,16 @@ class SensorModel {
   void upload_maps();
   StateDev state_view(int which) const;
   RunsDev runs_view() const;
-  // The checks of device runs (ids, run bounds, finite samples) and the scan of the run lengths,
-  // queued on stream; their results are read by finish_device_checks after the stream is synchronized.
+  // The checks of device runs' ids and bounds and the scan of the run lengths, queued on stream; their
+  // results are read by finish_device_checks after the stream is synchronized.
   void enqueue_device_checks(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream);
   // Throws std::invalid_argument for the first failed check (before any state change).
   void finish_device_checks();
+  // The samples' values are checked by the chain's phase A, which counts the values that are not
+  // finite into c[1 + kStNonFinite] (c: this camera's counters after the launch); the walk then
+  // advances no pixel. A nonzero count refuses the call (std::invalid_argument), the model left at
+  // its state before the call.
+  void refuse_non_finite(const unsigned long long* c);
   // This camera's descriptor for a launch of the chain over the runs (row_of built, pos scanned:
   // total segments), its buffers sized for them.
   CameraDev stage(const RunsDev& runs, const uint32_t* pixel, size_t n_runs, int64_t total);

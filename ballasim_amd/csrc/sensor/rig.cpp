@@ -76,13 +76,21 @@ std::vector<CameraDev> Rig::check_and_stage(const bsx_device_runs* runs, int64_t
   return cds;
 }
 
+void Rig::refuse_non_finite() {
+  for (size_t c = 0; c < cams_.size(); ++c) {
+    try {
+      cams_[c]->refuse_non_finite(counters_.data() + c * kK);
+    } catch (const std::invalid_argument& e) {
+      for (SensorModel* m : cams_) m->initialized_ = true;  // no camera advanced
+      throw std::invalid_argument("Rig camera " + std::to_string(c) + ": " + e.what());
+    }
+  }
+}
+
 void Rig::advance_device(const bsx_device_runs* runs, int64_t t_end_ns, hipStream_t stream, bsx_device_events* out) {
   if (out == nullptr) throw std::invalid_argument("Rig: null events output");
   std::vector<CameraDev> cds = check_and_stage(runs, t_end_ns, stream);
-  for (SensorModel* m : cams_) {
-    m->stats_.input_clamps += m->check_samples_[1];
-    m->initialized_ = false;
-  }
+  for (SensorModel* m : cams_) m->initialized_ = false;
   const bool instant = cams_[0]->instant();
   hip_check(hipEventRecord(ev0_, stream), "event record");
   chain_.launch(cds, instant, t_end_ns, stream);
@@ -97,6 +105,7 @@ void Rig::advance_device(const bsx_device_runs* runs, int64_t t_end_ns, hipStrea
     float ms = 0.0f;
     hip_check(hipEventElapsedTime(&ms, ev0_, ev1_), "event time");
     kernel_ms_ += ms;
+    refuse_non_finite();
     bool overflow = false;
     for (size_t c = 0; c < cams_.size(); ++c) {
       const unsigned long long n = counters_[c * kK];
@@ -112,6 +121,7 @@ void Rig::advance_device(const bsx_device_runs* runs, int64_t t_end_ns, hipStrea
   for (size_t c = 0; c < cams_.size(); ++c) {
     SensorModel& m = *cams_[c];
     m.commit(counters_.data() + c * kK, t_end_ns);
+    m.stats_.input_clamps += counters_[c * kK + 1 + kStClamps];
     ++m.stats_.launches;
     m.initialized_ = true;
     out[c] = bsx_device_events{reinterpret_cast<const bsx_raw_event*>(m.d_events_.get()),
@@ -124,7 +134,13 @@ std::vector<RigTrial> Rig::autotune(const bsx_device_runs* runs, int64_t t_end_n
   std::vector<CameraDev> cds = check_and_stage(runs, t_end_ns, stream);
   const bool instant = cams_[0]->instant();
   chain_.launch(cds, instant, t_end_ns, stream);  // phase A and the order, reused by every trial
+  for (size_t c = 0; c < cams_.size(); ++c) {
+    hip_check(hipMemcpyAsync(counters_.data() + c * kK, cams_[c]->d_counters_.get(), kK * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, stream),
+              "read counters");
+  }
   hip_check(hipStreamSynchronize(stream), "rig autotune");
+  refuse_non_finite();
   std::vector<RigTrial> trials;
   const auto time_walk = [&](const WalkTuning& tu) {
     for (const RigTrial& r : trials)

@@ -47,6 +47,9 @@ class Rig {
  private:
   // Checks every camera's runs (one synchronization), then stages their descriptors.
   std::vector<CameraDev> check_and_stage(const bsx_device_runs* runs, int64_t t_end_ns, hipStream_t stream);
+  // After a launch (counters_ read back): a camera whose runs hold a value that is not finite refuses
+  // the whole call, naming the camera; no camera has advanced.
+  void refuse_non_finite();
   RunsDev device_runs(size_t c, const bsx_device_runs& r) const;
 
   std::vector<SensorModel*> cams_;

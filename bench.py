@@ -25,6 +25,15 @@ refractory 100 us, noise off. A step advances all four cameras over one window b
 launch chain, after the autotuner has picked the walk's launch shape on the warm-up window; the
 same windows advanced camera by camera (four calls per window, the reference engine's loop) are
 timed beside it.
+
+--workload stress: a 2048x1536 sensor for 1 s (120 windows) watching 256 textured rectangles
+(velocity U(+-5000 px/s), rotation U(+-3 rad/s), value noise of 7 octaves from a 32-64 px cell down
+to 0.25-0.5 px), the instantaneous response with shot noise 0.5 Hz/px and leak 0.1 Hz/px, sampled by
+the plain controller window by window on the device (the runs of a whole second do not fit at once),
+and every released event streamed to the host in output order through the double-buffered 256 MiB
+pinned chunks of the event stream. A step is one window: sampling, the advance, the output stage and
+the stream. The sink on the host checks that the chunks arrive in order and their events in output
+order.
 """
 import argparse
 import json
@@ -42,7 +51,7 @@ S_STATE, S_MAP, S_EVENT = 48, 16, 16  # the fixed accounting's per-pixel state, 
 # The kernels of one advance call (launched back to back on the call's stream and timed together
 # with HIP events around them: the roofline's "launch").
 ADVANCE_CHAIN = {
-    "instantaneous": "advance chain: chain_init_kernel + chain_prep_kernel + order_count/offsets/scatter + instant_walk_kernel",
+    "instantaneous": "advance chain: chain_init_kernel + chain_screen_kernel + order_count/offsets/scatter + instant_walk_kernel",
     "cascade": "advance chain: chain_init_kernel + chain_cost_kernel + order_count/offsets/scatter + cascade_walk_kernel",
 }
 
@@ -383,9 +392,11 @@ def main():
     ap.add_argument("--noise", action="store_true",
                     help="plain, cascade: the reference's in-path noise on (the instantaneous response carries the "
                          "shot process instead)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "rig", "learned"],
+    ap.add_argument("--workload", default="plain", choices=["plain", "rig", "learned", "stress"],
                     help="plain: the 1280x720 sensor (default); rig: the 4-camera ragged batch; learned: the "
-                         "proposer's training and the learned controller against the plain one")
+                         "proposer's training and the learned controller against the plain one; stress: the "
+                         "2048x1536 sensor for 1 s with the event stream")
+    ap.add_argument("--chunk-mib", type=int, default=256, help="stress: event stream chunk size (MiB)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shape (no autotuner trials before the timed windows)")
     ap.add_argument("--train-examples", type=int, default=20_000_000, help="learned: training examples")
@@ -411,11 +422,13 @@ def main():
                                    "(pixels shard by tile or camera)", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.steps is None:
-        args.steps = 8 if args.workload == "rig" else 20
+        args.steps = {"rig": 8, "stress": 120}.get(args.workload, 20)
     if args.workload == "rig":
         return bench_rig(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "stress":
+        return bench_stress(args)
     return bench_plain(args)
 
 
@@ -438,6 +451,109 @@ def time_windows(sensor, runs, totals, first, count, tuning=None):
     st["kernel_ms"] = rig.stats()["kernel_ms"] - k0
     rig.close()
     return elapsed, sum(totals[first:first + count]), ev_counts, st
+
+
+def bench_stress(args):
+    import threading
+
+    import numpy as np
+
+    import ballasim_amd as bsx
+    from ballasim_amd import configs
+    from ballasim_amd import scene as S
+    from ballasim_amd.output import EventStream, OutputStage
+    from ballasim_amd.sensor import Rig, SensorModel
+
+    W, H = 2048, 1536
+    P = W * H
+    shapes, bg = S.config2_shapes(args.seed, n=256, width=W, height=H, v_max=5000.0, cell_range=(32.0, 64.0),
+                                  octaves=7)
+    scene, view = S.Scene(shapes, bg), S.view(W, H)
+    p = configs.config2_params("instantaneous")
+    p.shot_rate_hz = 0.5
+    p.leak_rate_hz = 0.1
+    rules = S.rules(float(p.dark_current_fA) / float(p.illuminance_scale), tol=0.01)
+    frame = scene.render(view, 0).reshape(-1, 1)
+    lock = threading.Lock()
+    seen = {"chunks": 0, "events": 0, "last_t": None, "ordered": True}
+
+    def sink(rec, index):  # on the stream's delivery thread
+        t = rec["t_reported_ns"]
+        with lock:
+            ok = index == seen["chunks"] and bool(np.all(t[1:] >= t[:-1]))
+            if len(t) and seen["last_t"] is not None:
+                ok = ok and int(t[0]) >= seen["last_t"]
+            seen["ordered"] &= ok
+            seen["chunks"] += 1
+            seen["events"] += len(t)
+            if len(t):
+                seen["last_t"] = int(t[-1])
+
+    def run(n_windows, stream_chunks):
+        m = SensorModel(p, W, H, 1, ["constant"], 0, device="cuda")
+        m.init(0, frame)
+        rig = Rig([m])
+        stage = OutputStage(m)
+        es = EventStream(sink, chunk_bytes=args.chunk_mib << 20) if stream_chunks else None
+        samples = events = released = 0
+        t_sample = 0.0
+        k0 = rig.stats()["kernel_ms"]
+        t0 = time.perf_counter()
+        for w in range(n_windows):
+            ts = time.perf_counter()
+            runs, n_s = scene.sample_window(view, rules, w * WINDOW, slot=0)
+            t_sample += time.perf_counter() - ts
+            (ptr, n_ev), = rig.advance_device([runs], (w + 1) * WINDOW)
+            samples += n_s
+            events += n_ev
+            if es is not None:
+                released += es.window(stage, (ptr, n_ev), w * WINDOW)
+            if (w + 1) % 10 == 0:
+                print(f"# stress window {w + 1}/{n_windows}: {samples} samples, {events} events", file=sys.stderr,
+                      flush=True)
+        if es is not None:
+            released += es.flush(stage)
+            es.finish()
+        elapsed = time.perf_counter() - t0
+        kernel_ms = rig.stats()["kernel_ms"] - k0
+        st = es.stats() if es is not None else None
+        if es is not None:
+            es.close()
+        rig.close()
+        return elapsed, t_sample, samples, events, released, kernel_ms, st
+
+    run(max(1, args.warmup), False)  # warm-up: kernels loaded, the sampler's buffers sized
+    seen.update(chunks=0, events=0, last_t=None, ordered=True)
+    n = args.steps
+    elapsed, t_sample, samples, events, released, kernel_ms, st = run(n, True)
+    assert released == events == seen["events"], (released, events, seen["events"])
+    bytes_total = algorithmic_bytes(samples, P * n, P * n, events)  # every pixel listed in every window
+    achieved = bytes_total / (kernel_ms * 1e-3) / 1e9
+    line = {
+        "metric": "pixel-samples/s + events/s on 1 MI355X; irradiance samples/event at fixed error",
+        "value": samples / elapsed,
+        "unit": "pixel-samples/s",
+        "n_gpus": 1, "steps": n, "warmup": args.warmup, "ms_per_step": elapsed / n * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (procedural scene, seed %d)" % args.seed,
+        "config": {"workload": "stress: 2048x1536 sensor, 256 textured affine rectangles (+-5000 px/s, 7 octaves "
+                               "down to 0.25-0.5 px), %d windows of 2^23 ns (%.1f ms), plain controller tol 0.01, "
+                               "instantaneous response, shot 0.5 Hz/px, leak 0.1 Hz/px, events streamed in %d MiB "
+                               "double-buffered pinned chunks; a step = sampling + advance + output + stream of one "
+                               "window" % (n, n * WINDOW / 1e6, args.chunk_mib),
+                   "pixels": P, "samples": samples, "events": events, "windows": n},
+        "events_per_s": events / elapsed,
+        "samples_per_event": samples / max(events, 1),
+        "sampling_s": t_sample,
+        "stream": {"chunks": st["chunks"], "bytes": st["bytes"], "copy_ms": st["copy_ms"],
+                   "copy_GBps": st["bytes"] / max(st["copy_ms"], 1e-9) / 1e6, "producer_wait_ms": st["wait_ms"],
+                   "in_order": seen["ordered"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": ADVANCE_CHAIN["instantaneous"],
+                     "kernel_ms_per_launch": kernel_ms / n, "bytes_per_launch": bytes_total / n},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_plain(args):

@@ -163,18 +163,29 @@ def test_input_refusal_before_state_change():
         m.advance(np.zeros(0, np.uint32), np.zeros((0, 0), np.int64), np.zeros((0, 0, 1), np.float32), -5)
 
 
+def _kind_params(kind):
+    import ballasim_amd as bsx
+    if kind == "instantaneous":
+        return bsx.preset("ideal")
+    p = bsx.preset("samsung_gen3")
+    p.noise = False
+    return p
+
+
+@pytest.mark.parametrize("kind", ["instantaneous", "cascade"])
 @pytest.mark.parametrize("offset", [0, 1])
-def test_device_sample_checks_large_runs(offset):
-    """The device check of the runs' irradiance on arrays of 163,840 values, through 16-byte loads
-    (offset 0) and through 4-byte loads (offset 1: the array starts one float past an aligned
-    address): a NaN deep inside and an infinity in the last value are refused before any state
-    changes, and values above the largest accepted sample are counted as clamps, exactly."""
+def test_device_sample_checks_large_runs(offset, kind):
+    """The device check of the runs' irradiance, made by the chain's phase A (the instantaneous
+    response's screen, the cascade's cost pass), on arrays of 163,840 values, aligned (offset 0) and
+    starting one float past an aligned address (offset 1): a NaN deep inside and an infinity in the
+    last value are refused before any state changes, and values above the largest accepted sample are
+    counted as clamps, exactly."""
     import torch
 
     import ballasim_amd as bsx
     W, H, n = 64, 64, 40
     P = W * H
-    m = bsx.SensorModel("ideal", W, H, 1, device="cuda")
+    m = bsx.SensorModel(_kind_params(kind), W, H, 1, device="cuda")
     m.init(0, np.ones((P, 1), np.float32))
     dev = "cuda"
     pixel = torch.arange(P, dtype=torch.int32, device=dev)
@@ -199,6 +210,37 @@ def test_device_sample_checks_large_runs(offset):
     torch.cuda.synchronize()
     assert m.stats()["input_clamps"] - c0 == len(hot)
     assert m.time_ns == t_end
+
+
+@pytest.mark.parametrize("kind", ["instantaneous", "cascade"])
+def test_device_check_of_a_first_sample_at_the_model_time(kind):
+    """A run whose first sample lies at the model time ends no segment, but its value is checked all
+    the same: a NaN there refuses the call with the model unchanged (the next call then runs), and an
+    over-range value there counts as one clamp."""
+    import torch
+
+    import ballasim_amd as bsx
+    W, H, n = 16, 8, 6
+    P = W * H
+    m = bsx.SensorModel(_kind_params(kind), W, H, 3, device="cuda")
+    m.init(5000, np.full((P, 1), 20.0, np.float32))
+    dev = "cuda"
+    pixel = torch.arange(P, dtype=torch.int32, device=dev)
+    count = torch.full((P,), n, dtype=torch.int32, device=dev)
+    start = torch.arange(P, dtype=torch.int64, device=dev) * n
+    t = (torch.arange(n, dtype=torch.int64, device=dev) * 1000 + 5000).repeat(P)
+    irr = torch.full((P * n,), 20.0, dtype=torch.float32, device=dev)
+    t_end = 5000 + n * 1000
+    before = m.save_state()
+    irr[37 * n] = float("nan")  # pixel 37's first sample, at the model time
+    with pytest.raises(ValueError, match="1 non-finite"):
+        m.advance_device(pixel, start, count, t, irr, t_end)
+    assert m.initialized and m.time_ns == 5000 and m.save_state() == before
+    irr[37 * n] = m.irr_max * 2.0
+    c0 = m.stats()["input_clamps"]
+    m.advance_device(pixel, start, count, t, irr, t_end)
+    torch.cuda.synchronize()
+    assert m.stats()["input_clamps"] - c0 == 1 and m.time_ns == t_end
 
 
 def test_device_runs_padding_not_checked():

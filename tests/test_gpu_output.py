@@ -77,5 +77,9 @@ def test_output_stage_matches_oracle(crop, readout, tmp_path):
     d = np.load(tmp_path / "cam.npz")
     t = d["events/t"]
     assert len(t) == released and np.all(np.diff(t) >= 0)
-    ms = d["ms_to_idx"]
-    assert np.all(t[ms[ms < len(t)]] >= 1000 * np.nonzero(ms < len(t))[0])
+    ms = d["ms_to_idx"].astype(np.int64)
+    k = np.arange(len(ms))
+    assert len(ms) == t[-1] // 1000 + 1
+    # entry k is the FIRST event with t >= 1000 k
+    assert np.all((ms == len(t)) | (t[np.minimum(ms, len(t) - 1)] >= 1000 * k))
+    assert np.all((ms == 0) | (t[np.maximum(ms - 1, 0)] < 1000 * k))

@@ -199,6 +199,17 @@ def test_rig_refusals_change_no_camera():
     with pytest.raises(ValueError, match="camera 1"):
         rig.advance_device(dev, 2 ** 23)
     assert [c.save_state() for c in cams] == before
+    nan = list(runs[2])
+    nan[2] = nan[2].copy()
+    nan[2][len(nan[2]) // 2] = np.nan  # a non-finite sample in camera 2: found by phase A, after the launch
+    dev = [to_device(*runs[0]), to_device(*runs[1]), to_device(*nan)]
+    torch.cuda.synchronize()
+    with pytest.raises(ValueError, match="camera 2.*non-finite"):
+        rig.advance_device(dev, 2 ** 23)
+    assert [c.save_state() for c in cams] == before and all(c.initialized for c in cams)
+    dev = [to_device(*r) for r in runs]
+    rig.advance_device(dev, 2 ** 23)
+    assert all(c.time_ns == 2 ** 23 for c in cams)
     with pytest.raises(ValueError):
         bsx.Rig([cams[0], cams[0]])
     other = bsx.SensorModel(_params("cascade"), 8, 8, 1, device="cuda")
