@@ -281,7 +281,7 @@ BSX_HD BSX_INLINE void sincos(T x, T* s, T* c) {
 // its Taylor polynomial, by that direct form for y > 1 and by the nested series
 // ((-y)^j / j!) (1 - y/(j+1) (1 - y/(j+2) (...))) (10 terms fp32, 17 fp64) for y <= 1.
 template <class T>
-BSX_HD BSX_NOINLINE T resid_exp(int j, T y) {
+BSX_HD BSX_INLINE T resid_exp(int j, T y) {
   if (j <= 0) return exp(-y);
   T yj = T(1), jf = T(1);
   for (int i = 1; i <= j; ++i) {
@@ -306,7 +306,7 @@ BSX_HD BSX_NOINLINE T resid_exp(int j, T y) {
 // function on the imaginary axis: 2 p(Z) / (L + x)^2 + (1/sqrt pi) / (L + x), Z = (L - x)/(L + x),
 // N = 24 terms in fp32 and 40 in fp64. erfcx(+inf) = 0, negative arguments give 1, NaN propagates.
 template <class T>
-BSX_HD BSX_NOINLINE T erfcx(T x) {
+BSX_HD BSX_INLINE T erfcx(T x) {
   if (!(x > T(0) && x < inf_<T>())) return x > T(0) ? T(0) : (x == x ? T(1) : x);
   constexpr bool kD = sizeof(T) == 8;
   const T lam = static_cast<T>(kD ? tables::kWeidemanL40 : tables::kWeidemanL24);
@@ -334,7 +334,7 @@ BSX_HD BSX_NOINLINE T erfcx(T x) {
 // Taylor series x sum (-2x^2)^n / (2n+1)!! (12 terms, nested); above, Rybicki's sampling sum with
 // h = 1/4 and 13 terms; beyond 1e8, 1/(2x).
 template <class T>
-BSX_HD BSX_NOINLINE T dawson(T x) {
+BSX_HD BSX_INLINE T dawson(T x) {
   const T ax = fabs_(x);
   constexpr bool kD = sizeof(T) == 8;
   const T switch_at = kD ? T(0.2) : T(1);

@@ -110,7 +110,7 @@ BSX_HD BSX_INLINE float root_tolerance(float hi) { return vmax(1e-11f, 16.0f * 0
 
 // ITP root of f on [a, b] with f(a) f(b) < 0 (Oliveira and Takahashi 2020; kappa1 = 0.2 / (b - a),
 // kappa2 = 2, n0 = 1).
-BSX_HD BSX_NOINLINE float itp_root(const ExpCubic& f, float a, float b, float fa, float fb, float tol) {
+BSX_HD BSX_INLINE float itp_root(const ExpCubic& f, float a, float b, float fa, float fb, float tol) {
   const float width0 = b - a;
   const float k1 = 0.2f / width0;
   int n_half = 0;
@@ -180,7 +180,7 @@ BSX_HD BSX_INLINE void level_roots(const ExpCubic& g, float lo, float hi, float 
   }
 }
 
-BSX_HD BSX_NOINLINE RootList critical_points(const ExpCubic& g, float lo, float hi) {
+BSX_HD BSX_INLINE RootList critical_points(const ExpCubic& g, float lo, float hi) {
   RootList crit;
   crit.n = 0;
   level_roots<1>(g, lo, hi, root_tolerance(hi), crit);
@@ -245,7 +245,7 @@ BSX_HD BSX_INLINE bool may_cross(const ExpCubic& delta, int64_t e0, int64_t e1, 
 }
 
 // The first crossing of Delta on [max(lo, e0), e1) (the reference's search after its hull test).
-BSX_HD BSX_NOINLINE Hit search(const ExpCubic& delta, int64_t e0, int64_t e1, int64_t lo, float theta_on,
+BSX_HD BSX_INLINE Hit search(const ExpCubic& delta, int64_t e0, int64_t e1, int64_t lo, float theta_on,
                                float theta_off) {
   const Hit none{false, false, 0};
   const int64_t k_lo = vmax(lo, e0) - e0, k_hi = e1 - 1 - e0;
@@ -283,7 +283,7 @@ BSX_HD BSX_INLINE float gl8_pair(float kh, float I0, float dI, double xi, double
 
 // z after h from z0 for I linear I0 -> I1: GL8 (P <= 1/2, with the fp32 small-slope expansion),
 // zero-order hold (slope below one ulp), Dawson (rising), erfcx (falling).
-BSX_HD BSX_NOINLINE float z_step(float z0, float I0, float I1, float h, float k) {
+BSX_HD BSX_INLINE float z_step(float z0, float I0, float I1, float h, float k) {
   const float kh = k * h;
   const float Im = 0.5f * (I0 + I1);
   const float P = kh * Im;
@@ -354,7 +354,7 @@ BSX_HD BSX_INLINE float boundary_layer_factor(float P) {
 
 // Attempts the sub-step [0, h] with linear input I_a -> I_b; on success fills sub (but t_a, t_b) and
 // next. at_floor forces acceptance (1 ns).
-BSX_HD BSX_NOINLINE bool try_substep(const Consts& cc, const State& st, float i_scale, float I_a, float I_b, float h,
+BSX_HD BSX_INLINE bool try_substep(const Consts& cc, const State& st, float i_scale, float I_a, float I_b, float h,
                                      bool at_floor, SubStep& sub, State& next) {
   const float k = cc.k;
   const float z_a = st.z;

@@ -105,7 +105,7 @@ struct RunsDev {
 
 // Phase A: consecutive positions per thread, and the positions of a 256-thread block, to which every
 // camera's segment base is aligned.
-inline constexpr int kPrepPerThread = 4;  // even: the records are stored in pairs
+inline constexpr int kPrepPerThread = 8;  // consecutive segment positions per phase-A thread
 inline constexpr int64_t kSegmentAlign = 256 * kPrepPerThread;
 
 struct RawEventDev;

@@ -186,7 +186,7 @@ __device__ __forceinline__ int camera_at(const CameraDev* cams, int n_cams, int6
 // The runs of a 256-thread block of consecutive positions q0 + threadIdx.x: the runs [first, first +
 // 256) staged in LDS (first = the run of q0), so that each thread finds its run by bisection in LDS
 // and reads its pixel and row start there. Positions beyond the staged runs (a block that spans more
-// than 256 runs, some of them empty) continue with a forward walk in global memory. Called by every
+// than 256 runs: short or empty ones) continue with a bisection in global memory. Called by every
 // thread of the block.
 struct BlockRuns {
   int64_t pos[256];
@@ -226,8 +226,14 @@ __device__ __forceinline__ RunRef stage_block_runs(BlockRuns& br, const int64_t*
     else hi = mid;
   }
   int64_t i = first + lo;
-  if (lo == m - 1 && i + 1 < n_runs) {
-    while (i + 1 < n_runs && pos[i + 1] <= q) ++i;
+  if (lo == m - 1 && i + 1 < n_runs && pos[i + 1] <= q) {  // past the staged runs: bisect the rest
+    int64_t a = i + 1, b = n_runs;                          // pos[a] <= q; the answer lies in [a, b)
+    while (b - a > 1) {
+      const int64_t mid = a + ((b - a) >> 1);
+      if (pos[mid] <= q) a = mid;
+      else b = mid;
+    }
+    i = a;
   }
   if (i == first + lo) {
     ref.i = i;
@@ -862,8 +868,12 @@ __device__ __forceinline__ void cascade_lane(const CameraDev* cams, int n_cams, 
   camera_add(cams, ci, 1 + kStPieces, L.n_pieces);
 }
 
+// The cascade's helpers (sub-steps, the comparator's search) are inlined into the lane program: as
+// calls they saved and restored the caller's live registers through scratch on every sub-step attempt.
+// Two waves per SIMD (256 registers a lane) keep the occupancy the calls had, with a fifth of their
+// scratch.
 template <int kMaxG>
-__global__ __launch_bounds__(256) void cascade_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void cascade_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
                                                            const uint32_t* order, unsigned int* queue, WalkTuning tu,
                                                            const CallDev* call) {
   if (call_refused(cams, n_cams)) return;
@@ -994,7 +1004,7 @@ __device__ __forceinline__ void noise_lane(const CameraDev* cams, int n_cams, ui
 }
 
 template <int kMaxG>
-__global__ __launch_bounds__(256) void noise_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void noise_walk_kernel(const CameraDev* cams, int n_cams, uint32_t n_total,
                                                          const uint32_t* order, unsigned int* queue, WalkTuning tu,
                                                          const CallDev* call) {
   if (call_refused(cams, n_cams)) return;

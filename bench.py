@@ -649,7 +649,10 @@ def bench_plain(args):
                            "windows": nw, "ms_per_window": el / nw * 1e3,
                            "kernel_ms_per_launch": stc["kernel_ms"] / nw, "samples": sm, "events": sum(evc),
                            "noise": "in-path LNA noise (prophesee_gen3 model)" if noise else "off",
-                           "cells": stc["cells"], "refinements": stc["refinements"]}
+                           "cells": stc["cells"], "refinements": stc["refinements"],
+                           "substeps": stc["substeps"], "substep_attempts": stc["substep_attempts"],
+                           "settled_substeps": stc["settled_substeps"], "pieces": stc["pieces"],
+                           "leaf_intervals": stc["leaf_intervals"]}
             print(f"# {name}: {nw} windows in {el:.3f} s", file=sys.stderr, flush=True)
             del m
         line["cascade"] = extra["cascade"]

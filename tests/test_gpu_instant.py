@@ -243,6 +243,40 @@ def test_device_check_of_a_first_sample_at_the_model_time(kind):
     assert m.stats()["input_clamps"] - c0 == 1 and m.time_ns == t_end
 
 
+@pytest.mark.parametrize("kind", ["instantaneous", "cascade"])
+def test_short_runs_many_runs_per_phase_a_block(kind):
+    """Runs of zero to two samples over a 192x128 camera, so one phase-A block of 2048 consecutive
+    positions spans far more than the 256 runs it stages in LDS and its threads find their runs by
+    bisection in global memory: events, reported times and the state equal the oracle's."""
+    import ballasim_amd as bsx
+    W, H = 192, 128
+    P = W * H
+    rng = np.random.default_rng(31)
+    p = _kind_params(kind)
+    p.leak_rate_hz = 2.0
+    gpu = bsx.SensorModel(p, W, H, 9, device="cuda")
+    ref = O.OracleModel(p.c, W, H, 9, 1, 0)
+    irr0 = np.exp(rng.normal(3, 1, size=(P, 1))).astype(np.float32)
+    gpu.init(0, irr0)
+    ref.init(0, irr0)
+    t = 0
+    for call in range(3):
+        t1 = t + 3_000_000
+        ids, ts, ir, offs = random_runs(rng, P, t, t1, max_n=3, frac=0.95, log_sigma=2.5)
+        assert len(ids) > 2048 * 4 and len(ts) < 2 * len(ids)
+        ev = gpu.advance(ids, ts, ir, t1, offsets=offs)
+        want, rep = ref.advance_runs(ids, ts, ir, offs, t1)
+        got = events_from_dict(ev, W)
+        i, a, b = first_difference(got, want)
+        assert len(got) == len(want) and i == len(got), f"call {call}: first difference at {i}: gpu {a} oracle {b}"
+        assert np.array_equal(ev["t_reported_ns"], rep)
+        t = t1
+    st = np.frombuffer(gpu.save_state()[56:], O.REF_STATE_DTYPE)
+    so = ref.states()
+    for f in ("z", "s3_ref", "t_ref", "t_search", "blind_until", "flags", "held"):
+        assert np.array_equal(st[f], so[f]), f
+
+
 def test_device_runs_padding_not_checked():
     """Device runs that leave samples unreferenced: the padding between runs holds NaN, infinities
     and over-range values. Only the referenced samples are checked, as the reference's

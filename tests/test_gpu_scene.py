@@ -21,6 +21,24 @@ def _small_scene(seed=3, n=6, width=96, height=64):
     return S.Scene(shapes, bg), S.view(width, height), shapes, bg
 
 
+def _level_scene(seed=21, n=8, width=80, height=60):
+    """Small rectangles and disks (half sizes 5-14 px) at log-uniform speeds of 30-4000 px/s over a
+    100 lux background: pixels at many lattice levels, and pixels no shape reaches."""
+    from ballasim_amd import scene as S
+    rng = np.random.default_rng(seed)
+    shapes = []
+    for i in range(n):
+        speed = float(np.exp(rng.uniform(np.log(30.0), np.log(4000.0))))
+        ang = float(rng.uniform(0, 2 * np.pi))
+        hx, hy = (float(v) for v in rng.uniform(5, 14, 2))
+        shapes.append(S.shape("disk" if i % 3 == 0 else "rect", float(rng.uniform(10, width - 10)),
+                              float(rng.uniform(10, height - 10)), speed * np.cos(ang), speed * np.sin(ang), hx, hy,
+                              float(np.exp(rng.uniform(np.log(20), np.log(500)))), float(rng.uniform(0.2, 0.8)),
+                              int(rng.integers(1, 2 ** 31)), angle=float(rng.uniform(0, 6.28)),
+                              omega=float(rng.uniform(-2, 2)), octaves=4, cell=float(rng.uniform(4, 12))))
+    return S.Scene(shapes, 100.0), S.view(width, height), shapes, 100.0
+
+
 def _eta(dark, a, m, b):
     a, m, b = np.float32(a), np.float32(m), np.float32(b)
     dev = np.float32(abs((a + b) * np.float32(-0.5) + m))
@@ -28,30 +46,35 @@ def _eta(dark, a, m, b):
     return np.float32(np.float32(dev) / np.float32(np.float32(dark) + low))
 
 
-def _expected_offsets(vals, level, dark, tol, max_levels=4, min_split=2048):
-    """The controller's sample offsets for one pixel, re-derived from the rendered values alone
-    (signature flags off): vals maps offset -> value (the run plus the carried sample at 0)."""
+def _expected_offsets(vals, level, dark, tol, max_levels=4, min_split=2048, sigs=None):
+    """The controller's sample offsets for one pixel, re-derived from the rendered values alone: vals
+    maps offset -> value (the run plus the carried sample at 0); sigs (None: signature flags off) maps
+    offset -> signature, and a flip between an interval's ends flags it, as the reference's
+    signature_flag does under its default rules (no jump gate: src/engine/sampling.h:126-134)."""
     n, step = 1 << level, WINDOW >> level
     e = [vals.get(j * step) for j in range(n + 1)]
     if any(v is None for v in e):
         return None
     out = []
 
+    def flip(a, b):
+        return sigs is not None and sigs[a] != sigs[b]
+
     def refine(lo, hi, depth):
         g = (lo + hi) >> 1
         if g not in vals:
             raise KeyError(g)
         bad = _eta(dark, vals[lo], vals[g], vals[hi]) > np.float32(tol)
-        if bad and g - lo >= min_split and depth < max_levels:
+        if (flip(lo, g) or bad) and g - lo >= min_split and depth < max_levels:
             refine(lo, g, depth + 1)
         out.append(g)
-        if bad and hi - g >= min_split and depth < max_levels:
+        if (flip(g, hi) or bad) and hi - g >= min_split and depth < max_levels:
             refine(g, hi, depth + 1)
 
     for j in range(1, n + 1):
         m = (j + 1 <= n and _eta(dark, e[j - 1], e[j], e[j + 1]) > np.float32(tol)) or \
             (j >= 2 and _eta(dark, e[j - 2], e[j - 1], e[j]) > np.float32(tol))
-        if m and step >= min_split:
+        if (m or flip((j - 1) * step, j * step)) and step >= min_split:
             try:
                 refine((j - 1) * step, j * step, 1)
             except KeyError:
@@ -60,37 +83,109 @@ def _expected_offsets(vals, level, dark, tol, max_levels=4, min_split=2048):
     return out
 
 
-def test_runs_well_formed_and_rules_hold():
-    """Every run is sorted inside (t_w, t_w + 2^23], on the 1024 ns grid, and is exactly the sample
-    set the controller's rules give on its own values at some lattice level (signature flags off)."""
+def _levels(shapes, view, t_w, d_px=0.25, l_min=1, l_max=13):
+    """Each pixel's lattice level under the motion-bound rule l = clamp(ceil(log2(v_hat W / d_px)), l_min,
+    l_max) (src/engine/motion.cpp:1330-1340), with v_hat the largest image speed |v| + |omega| r of the
+    shapes whose bounding circle (r + 1.5 px), swept over the window, passes the pixel centre; -1: no shape
+    reaches the pixel (static, no samples). Restated in numpy from the shapes' parameters."""
+    W, H = view.width, view.height
+    xs = view.origin_x + (np.arange(W * H) % W + 0.5) * view.px_scale
+    ys = view.origin_y + (np.arange(W * H) // W + 0.5) * view.px_scale
+    t0, t1 = t_w * 1e-9, t_w * 1e-9 + WINDOW * 1e-9
+    v_hat = np.zeros(W * H, np.float32)
+    for s in shapes:
+        radius = np.float32(s.hx if s.kind == 1 else math.sqrt(s.hx * s.hx + s.hy * s.hy))
+        speed = np.float32(math.sqrt(s.vx * s.vx + s.vy * s.vy) + abs(s.omega) * float(radius))
+        ax, ay = s.cx + s.vx * t0, s.cy + s.vy * t0
+        bx, by = s.cx + s.vx * t1, s.cy + s.vy * t1
+        ex, ey = bx - ax, by - ay
+        px, py = xs - ax, ys - ay
+        ee = ex * ex + ey * ey
+        f = np.clip((px * ex + py * ey) / ee, 0.0, 1.0) if ee > 0 else np.zeros_like(px)
+        qx, qy = px - f * ex, py - f * ey
+        r = float(radius) + 1.5
+        v_hat = np.where(qx * qx + qy * qy <= r * r, np.maximum(v_hat, speed), v_hat)
+    with np.errstate(divide="ignore"):
+        lv = np.ceil(np.log2(v_hat.astype(np.float64) / view.px_scale * (WINDOW * 1e-9) / d_px))
+    level = np.clip(lv, l_min, l_max).astype(np.int64)
+    return np.where(v_hat > 0, level, -1)
+
+
+def _signatures(sc, view, t_w, ids, t, offs):
+    """The scene's signature (1 + shape index, 0 the background) at the carried sample and at every sample
+    of every run, rendered on the device through the engine's request interface."""
+    import torch
+
     from ballasim_amd import scene as S
-    sc, view, _, _ = _small_scene()
+    rows = []
+    for i, p in enumerate(ids):
+        rows.append((p, 0))
+        rows.extend((p, int(x - t_w)) for x in t[offs[i]:offs[i + 1]])
+    req = np.zeros(len(rows), S.REQUEST_DTYPE)
+    req["pixel"] = [r[0] for r in rows]
+    req["k_count"] = 1
+    req["t_off_ns"] = [r[1] for r in rows]
+    dev = torch.device("cuda:0")
+    rq = torch.from_numpy(req.view(np.uint8)).to(dev)
+    irr = torch.zeros(len(rows), dtype=torch.float32, device=dev)
+    sig = torch.zeros(len(rows), dtype=torch.int64, device=dev)
+    sc.render_requests(view, t_w, rq, len(rows), None, irr, sig)
+    torch.cuda.synchronize()
+    return sig.cpu().numpy(), rows
+
+
+@pytest.mark.parametrize("signature", [False, True])
+def test_runs_well_formed_and_rules_hold(signature):
+    """Every run is sorted inside (t_w, t_w + 2^23], on the 1024 ns grid, at the pixel's own lattice level
+    (the motion-bound rule, restated in numpy; a pixel no shape reaches takes no samples), and is exactly
+    the sample set the controller's rules give on its own values and, with signature flags on, on the
+    scene's signatures at those samples (a flip between an interval's ends refines it). The signatures
+    come from the engine's request interface, which forms a sample's time as (t_w + dt) 1e-9 s where the
+    controller forms t_w 1e-9 + dt 1e-9; the two agree bit for bit only at t_w = 0, so the signature
+    case checks the first window."""
+    from ballasim_amd import scene as S
+    sc, view, shapes, _ = _level_scene()
     dark, tol = 0.0021, 0.01
-    rules = S.rules(dark, tol=tol, signature=False)
-    checked = refined = 0
-    for w in range(3):
+    rules = S.rules(dark, tol=tol, signature=signature)
+    checked = refined = flips = static = 0
+    levels_seen = set()
+    for w in ([0] if signature else range(3)):
         runs, total = sc.sample_window(view, rules, w * WINDOW, slot=0)
         ids, t, irr, offs = S.runs_to_host(runs)
         carried = sc.render(view, w * WINDOW)
+        level_of = _levels(shapes, view, w * WINDOW)
         assert np.array_equal(ids, np.arange(view.width * view.height))
         assert total == len(t) == offs[-1]
+        sig_at = None
+        if signature:
+            sig, rows = _signatures(sc, view, w * WINDOW, ids, t, offs)
+            sig_at = {}
+            for (p, off), v in zip(rows, sig):
+                sig_at.setdefault(p, {})[off] = int(v)
         for p in range(len(ids)):
             tt = t[offs[p]:offs[p + 1]] - w * WINDOW
-            if len(tt) == 0:
+            if level_of[p] < 0:
+                assert len(tt) == 0, f"window {w} pixel {p}: no shape reaches it, yet it has samples"
+                static += 1
                 continue
+            assert len(tt) > 0
             assert np.all(np.diff(tt) > 0) and tt[0] > 0 and tt[-1] == WINDOW and np.all(tt % 1024 == 0)
             vals = {0: carried[p]}
             vals.update({int(d): v for d, v in zip(tt, irr[offs[p]:offs[p + 1], 0])})
-            ok = False
-            for level in range(1, 14):
-                exp = _expected_offsets(vals, level, dark, tol)
-                if exp is not None and exp == [int(x) for x in tt]:
-                    ok = True
-                    refined += len(tt) > (1 << level)
-                    break
-            assert ok, f"window {w} pixel {p}: the run is not the controller's sample set"
+            sigs = sig_at[p] if signature else None
+            level = int(level_of[p])
+            exp = _expected_offsets(vals, level, dark, tol, sigs=sigs)
+            assert exp is not None and exp == [int(x) for x in tt], \
+                f"window {w} pixel {p}: the run is not the controller's sample set at level {level}"
+            refined += len(tt) > (1 << level)
+            levels_seen.add(level)
+            if sigs is not None:
+                flips += len(set(sigs.values())) > 1
             checked += 1
-    assert checked > 100 and refined > 10
+    assert checked > 100 and refined > 10 and static > 100
+    assert len(levels_seen) >= 3, levels_seen
+    if signature:
+        assert flips > 20
 
 
 def test_scene_values_match_numpy_evaluation():

@@ -1,8 +1,9 @@
 """GPU parity of the instantaneous response on randomized configurations: resets, refractory
 periods, leak from none to 40 Hz, threshold spreads, one to three light groups, hot pixels and the
 shot process, each over four calls of ragged runs with up to 200 samples per pixel (up to 5.6 M
-events per call, so the event buffer overflows and the walk re-runs). Events, reported times and the
-sub-step and piece counters must equal the oracle's bit for bit."""
+events per call: a call with more than 2^20 events overflows the event buffer's first size and the
+walk re-runs, which those trials assert). Events, reported times and the sub-step and piece counters
+must equal the oracle's bit for bit."""
 import numpy as np
 import pytest
 
@@ -50,6 +51,7 @@ def test_random_configuration_matches_oracle(trial):
     gpu.init(-1000, irr0)
     ref.init(-1000, irr0)
     t = -1000
+    most = 0
     for call in range(4):
         t1 = t + int(rng.integers(2 ** 20, 3 * 2 ** 23))
         ids, ts, ir, offs = random_runs(rng, W * H, t, t1, G=G, max_n=int(rng.choice([12, 60, 200])))
@@ -59,7 +61,10 @@ def test_random_configuration_matches_oracle(trial):
         i, a, b = first_difference(got, want)
         assert len(got) == len(want) and i == len(got), f"call {call}: first difference at {i}: gpu {a} oracle {b}"
         assert np.array_equal(ev["t_reported_ns"], rep)
+        most = max(most, len(want))
         t = t1
     ks, ko = gpu.stats(), ref.stats()
+    if most > 1 << 20:
+        assert ks["overflow_reruns"] >= 1, most
     for f in ("substeps", "pieces"):
         assert ks[f] == ko[f], (f, ks[f], ko[f])
