@@ -438,9 +438,11 @@ struct InstantLane {
     scr_valid = true;
   }
 
-  // True when I(t) over the current segment stays strictly inside (scr_lo, scr_hi).
-  BSX_HD BSX_INLINE bool screened() const {
-    float lo = dark, hi = dark;
+  // The range of the current segment's end photocurrents, lo <= I(t) <= hi up to the rounding the
+  // screen allows for.
+  BSX_HD BSX_INLINE void segment_range(float& lo, float& hi) const {
+    lo = dark;
+    hi = dark;
     const bool held = tb == kNoBreakpoint;
     for (int g = 0; g < kMaxG; ++g) {
       if (g < G) {
@@ -449,7 +451,51 @@ struct InstantLane {
         hi = hi + scale * vmax(va[g], b);
       }
     }
+  }
+
+  // True when the photocurrent range [lo, hi], widened by 2^-12, lies strictly inside (scr_lo, scr_hi).
+  BSX_HD BSX_INLINE bool in_screen(float lo, float hi) const {
     return (hi * (1.0f + 0x1p-12f) < scr_hi) & (lo * (1.0f - 0x1p-12f) > scr_lo);
+  }
+
+  // True when I(t) over the current segment stays strictly inside (scr_lo, scr_hi).
+  BSX_HD BSX_INLINE bool screened() const {
+    float lo, hi;
+    segment_range(lo, hi);
+    return in_screen(lo, hi);
+  }
+
+  // ---- Steady light -------------------------------------------------------------------------------
+  // When every group's segment ends are equal (or the segment is the held one), photocurrent(t) is the
+  // one value I = photocurrent_start() at every t ((vb - va) f is +0), so x(t) = log_ratio(I) is one
+  // value and Delta(t) = fl(fl(x - s3_ref) + (0 - n3_ref)) + dl(t)) is non-decreasing in t (dl is,
+  // with leak >= 0). Then no integer of [lo, e) fires if and only if Delta(lo) > -theta_off and
+  // Delta(e - 1) < theta_on: the exact answer of the search, from one logarithm, which is kept while
+  // the light and i_scale stay the same. It decides the pixels whose light is constant and whose
+  // Delta sits within the screen's margin of a level (a static scene seen near a threshold).
+  float st_I = -1.0f, st_scale = 0.0f, st_x = 0.0f;
+
+  BSX_HD BSX_INLINE bool steady_segment() const {
+    if (tb == kNoBreakpoint) return true;
+    bool eq = true;
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < G) eq &= va[g] == vb[g];
+    return eq;
+  }
+
+  BSX_HD BSX_INLINE float steady_x(float I) {
+    if (!(I == st_I && i_scale == st_scale)) {
+      st_x = log_ratio(I);
+      st_I = I;
+      st_scale = i_scale;
+    }
+    return st_x;
+  }
+
+  // Under steady light I over [lo, last] (lo <= last): true when no integer of it fires.
+  BSX_HD BSX_INLINE bool steady_quiet(float I, int64_t lo, int64_t last) {
+    const float x = steady_x(I);
+    return (delta_at(last, x) < theta_on) & (delta_at(lo, x) > -theta_off);
   }
 
   BSX_HD BSX_INLINE static uint8_t split_label(float ds, float dl, float D, float cut) {
@@ -517,7 +563,7 @@ struct InstantLane {
       if (lo >= e) return true;
       if (screen && n3_ref == 0.0f && leak >= 0.0f) {
         if (!scr_valid) screen_bounds(lo, t_end_ - 1);
-        if (screened()) {
+        if (screened() || (steady_segment() && steady_quiet(photocurrent_start(), lo, e - 1))) {
           t_search = vmax(t_search, e);
           return true;
         }
@@ -579,6 +625,11 @@ struct InstantLane {
   float fv[kAhead][kMaxG];
   bool seg_open, in_piece;
   int64_t t_cur, stop, t_end_, s_lo, e_cur;
+  // Phase A's photocurrent ranges of aligned blocks of kScreenBlock segments (null: none), and the
+  // camera position of the run's first sample.
+  const BSX_GLOBAL ScreenBlock* qb = nullptr;
+  int64_t q_base = 0;
+  uint32_t n_block_skips = 0;  // blocks passed at once (a diagnostic; no reference counterpart)
 
   BSX_HD BSX_INLINE void fetch(int k, int i) {
     ft[k] = kNoBreakpoint;
@@ -602,6 +653,41 @@ struct InstantLane {
     }
     ++next_i;
     fetch(kAhead - 1, next_i + kAhead - 1);
+  }
+
+  // At a segment boundary, kScreenBlock segments at once: when the next segment starts an aligned block
+  // of the run whose photocurrent range (phase A's summary, the extremes of the segments' own ranges)
+  // passes the current screen, with no release pending, no shot arrival up to the block's last sample
+  // and no 2^23-ns cut inside it, each of its segments would be screened in turn (or lie in dead time)
+  // and leave exactly this: one sub-step and one piece each, t_search at least the block's end, and
+  // the block's last sample as the next segment's start. The bounds, when formed here, are formed from
+  // max(t_search, t_cur), as the block's first piece would form them; bounds formed earlier are valid
+  // for later times too (dl does not decrease).
+  BSX_HD BSX_INLINE bool try_block_skip() {
+    if (qb == nullptr || !screen || blind_until != kNotBlind || n3_ref != 0.0f || !(leak >= 0.0f)) return false;
+    const int64_t q = q_base + next_i;
+    if ((q & (kScreenBlock - 1)) != 0 || next_i + kScreenBlock > n) return false;
+    const int last = next_i + kScreenBlock - 1;
+    const int64_t t_last = ts[last];
+    if (!(shot_next > t_last) || ((t_cur >> kPlainSegmentLog2) + 1) * kPlainSegmentNs < t_last) return false;
+    const ScreenBlock b = qb[q >> kScreenBlockLog2];
+    const int64_t from = vmax(t_search, t_cur);
+    if (!scr_valid) screen_bounds(from, t_end_ - 1);
+    // a steady block (the light b.lo at every time of it, the current start value's) is decided
+    // exactly by its two ends; a block wholly in dead time fires nothing
+    if (!in_screen(b.lo, b.hi) && !(b.steady != 0u && (from >= t_last || steady_quiet(b.lo, from, t_last - 1))))
+      return false;
+    n_substeps += kScreenBlock;
+    n_pieces += kScreenBlock;
+    ++n_block_skips;
+    t_search = vmax(t_search, t_last);
+    ta = t_last;
+    for (int g = 0; g < kMaxG; ++g)
+      if (g < G) va[g] = clampv(irr[static_cast<int64_t>(last) * G + g]);
+    t_cur = t_last;
+    next_i += kScreenBlock;
+    for (int k = 0; k < kAhead; ++k) fetch(k, next_i + k);
+    return true;
   }
 
   // first_loud: the index of the first input sample ending a segment that phase A did not find quiet
@@ -635,6 +721,10 @@ struct InstantLane {
     while (true) {
       if (!seg_open) {
         if (!(t_cur < t_end_)) return kDone;
+        if (try_block_skip()) {
+          if (--budget <= 0) return kWalk;
+          continue;
+        }
         open_next();
         stop = vmin(t_end_, tb);
         seg_open = true;

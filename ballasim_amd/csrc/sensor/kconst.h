@@ -115,6 +115,18 @@ struct RawEventDev;
 // run_base + n_runs)) and their segments one segment space ([seg_base, seg_base + total)); run_base
 // is a multiple of 256 and seg_base of kSegmentAlign, so that no block of a run- or segment-indexed
 // kernel spans two cameras.
+// The photocurrent range of kScreenBlock consecutive segments of one run (aligned positions of the
+// camera's segment space), written by phase A: the least lower and the greatest upper end of the
+// segments' photocurrent ranges, as InstantLane::segment_range forms them, and whether every one of
+// them has equal end values in every group (steady: the photocurrent is then lo = hi at every time of
+// the block). lo = 0, hi = +inf: the block holds a position that is not a segment of one run.
+inline constexpr int kScreenBlockLog2 = 4;
+inline constexpr int kScreenBlock = 1 << kScreenBlockLog2;
+struct ScreenBlock {
+  float lo, hi;
+  uint32_t steady;
+};
+
 struct CameraDev {
   KConst kc;
   MapsDev mp;
@@ -123,6 +135,7 @@ struct CameraDev {
   const BSX_GLOBAL uint32_t* pixel;       // the runs' pixel ids
   const BSX_GLOBAL int64_t* pos;          // the runs' segment positions (exclusive scan of the run lengths)
   BSX_GLOBAL int* first_loud;             // per run (null: every walk from its start)
+  BSX_GLOBAL ScreenBlock* screen_blocks;  // per kScreenBlock positions (null: none)
   BSX_GLOBAL float* work;                 // per run: estimated walk cost (null: no order information)
   BSX_GLOBAL RawEventDev* ev;
   unsigned long long cap;

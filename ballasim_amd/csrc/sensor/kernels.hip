@@ -159,18 +159,33 @@ __device__ __forceinline__ int64_t run_of_position(const int64_t* pos, int64_t n
   return lo;
 }
 
+// The last run at or after run i (pos[i] <= q) whose position is <= q: a few steps forward, then a
+// bisection, so that a long stretch of empty runs (pixels without samples share one position) costs
+// a logarithm, not a step per run.
+__device__ __forceinline__ int64_t forward_run(const int64_t* pos, int64_t n_runs, int64_t i, int64_t q) {
+  for (int s = 0; s < 4; ++s) {
+    if (!(i + 1 < n_runs && pos[i + 1] <= q)) return i;
+    ++i;
+  }
+  if (!(i + 1 < n_runs && pos[i + 1] <= q)) return i;
+  int64_t a = i + 1, b = n_runs;  // pos[a] <= q; the answer lies in [a, b)
+  while (b - a > 1) {
+    const int64_t mid = a + ((b - a) >> 1);
+    if (pos[mid] <= q) a = mid;
+    else b = mid;
+  }
+  return a;
+}
+
 // The same for the block's threads at once (q = the block's first position + threadIdx.x): one
-// bisection for the block's first position, then a short forward walk per thread (a block's 256
-// consecutive positions span few runs). Called by every thread of the block.
+// bisection for the block's first position, then forward_run per thread. Called by every thread of
+// the block.
 __device__ __forceinline__ int64_t block_run_of_position(const int64_t* pos, int64_t n_runs, int64_t q0,
                                                          int64_t q, int64_t total) {
   __shared__ int64_t first;
   if (threadIdx.x == 0) first = q0 < total ? run_of_position(pos, n_runs, q0) : 0;
   __syncthreads();
-  int64_t i = first;
-  if (q < total)
-    while (i + 1 < n_runs && pos[i + 1] <= q) ++i;
-  return i;
+  return q < total ? forward_run(pos, n_runs, first, q) : first;
 }
 
 // The camera of a launch that holds offset q of the joint run (kRuns) or segment space (cameras in
@@ -253,15 +268,22 @@ __device__ __forceinline__ RunRef stage_block_runs(BlockRuns& br, const int64_t*
 // global memory).
 __device__ __forceinline__ void advance_run(RunRef& ref, const BlockRuns& br, const int64_t* pos, const RunsDev& runs,
                                             const uint32_t* pixel, int64_t n_runs, int64_t q) {
-  while (ref.i + 1 < n_runs) {
+  for (int s = 0; s < 4 && ref.i + 1 < n_runs; ++s) {
     const int64_t k = ref.i + 1 - br.first;
     const bool staged = k < br.count;
     const int64_t next = staged ? br.pos[k] : pos[ref.i + 1];
-    if (next > q) break;
+    if (next > q) return;
     ref.i += 1;
     ref.pos = next;
     ref.row_start = staged ? br.row_start[k] : runs.row_start[ref.i];
     ref.pixel = staged ? br.pixel[k] : pixel[ref.i];
+  }
+  const int64_t i = forward_run(pos, n_runs, ref.i, q);  // a long stretch of empty runs
+  if (i != ref.i) {
+    ref.i = i;
+    ref.pos = pos[i];
+    ref.row_start = runs.row_start[i];
+    ref.pixel = pixel[i];
   }
 }
 
@@ -325,11 +347,20 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
   bool usable = false;    // the pixel's screen applies (leak >= 0, n3_ref = 0, not blind)
   int64_t loaded = -1;    // the run whose pixel constants L holds
   int64_t prev_end = -1;  // the position whose segment's end sample L.tb, L.vb hold (-1: none)
+  // this thread's half of a screen block: its positions all segments of run blk_run, their range
+  bool blk_ok = true, blk_steady = true;
+  int blk_run = -1;
+  float blk_lo = INFINITY, blk_hi = -INFINITY;
   for (int k = 0; k < kPrepPerThread; ++k) {
     const int64_t q = qt + k;
-    if (!(q < total)) break;
+    if (!(q < total)) {
+      blk_ok = false;
+      break;
+    }
     advance_run(rr, br, pos, runs, pixel, n_runs, q);
     const int run = static_cast<int>(rr.i);
+    blk_ok &= blk_run < 0 || run == blk_run;
+    blk_run = run;
     if (run != acc_run) {
       if (acc_run >= 0) {
         if (work != nullptr) atomicAdd(work + acc_run, cost_acc);
@@ -367,6 +398,7 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
         for (int g = 0; g < kMaxG; ++g)
           if (g < kc.groups) tally.add(v[g], kc.irr_max);
         prev_end = -1;
+        blk_ok = false;
         continue;
       }
       L.ta = t_state;
@@ -390,8 +422,13 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
       }
     }
     prev_end = q;
+    float seg_lo, seg_hi;
+    L.segment_range(seg_lo, seg_hi);
+    blk_lo = fminf(blk_lo, seg_lo);
+    blk_hi = fmaxf(blk_hi, seg_hi);
+    blk_steady &= L.steady_segment();
     const bool quiet = usable & (L.shot_next >= L.tb) &
-                       (((L.ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= L.tb) & L.screened();
+                       (((L.ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= L.tb) & L.in_screen(seg_lo, seg_hi);
     if (!quiet) {
       loud_acc = vmin(loud_acc, static_cast<int>(j));
       float Ia = L.dark, Ib = L.dark;
@@ -408,6 +445,19 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
   }
   wave_segmented_cost_loud(work, first_loud, acc_run, cost_acc, loud_acc);
   tally.flush(cd.counters);
+  // Screen blocks: the two threads of an even-odd lane pair cover one aligned block of kScreenBlock
+  // positions (kPrepPerThread each); the even one writes its summary.
+  static_assert(2 * kPrepPerThread == kScreenBlock, "a screen block is two threads' positions");
+  const int p_run = __shfl_xor(blk_run, 1);
+  const bool p_ok = __shfl_xor(static_cast<int>(blk_ok), 1) != 0;
+  const bool p_steady = __shfl_xor(static_cast<int>(blk_steady), 1) != 0;
+  const float p_lo = __shfl_xor(blk_lo, 1), p_hi = __shfl_xor(blk_hi, 1);
+  if (cd.screen_blocks != nullptr && (threadIdx.x & 1) == 0 && qt < total) {
+    const bool ok = blk_ok & p_ok & (blk_run == p_run);
+    cd.screen_blocks[qt >> kScreenBlockLog2] =
+        ok ? ScreenBlock{fminf(blk_lo, p_lo), fmaxf(blk_hi, p_hi), (blk_steady & p_steady) ? 1u : 0u}
+           : ScreenBlock{0.0f, INFINITY, 0u};
+  }
 }
 
 // Before phase A: zero each camera's counters, the order histogram and the queue counter (block 0),
@@ -632,6 +682,10 @@ __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams,
       L.irr = runs.irr + start * kc.groups;
       L.n = runs.row_count[row];
       if (cam->first_loud != nullptr) loud_start = vmin(cam->first_loud[row], L.n);
+      if (cam->screen_blocks != nullptr) {
+        L.qb = cam->screen_blocks;
+        L.q_base = cam->pos[row];
+      }
     } else {
       L.ts = nullptr;
       L.irr = nullptr;

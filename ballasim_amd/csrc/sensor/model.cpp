@@ -531,6 +531,7 @@ CameraDev SensorModel::stage(const RunsDev& runs, const uint32_t* pixel, size_t 
   d_events_.ensure(cap_ * sizeof(RawEventDev));
   d_work_.ensure(std::max<size_t>(n_runs, 1) * sizeof(float));
   d_loud_.ensure(std::max<size_t>(n_runs, 1) * sizeof(int));
+  if (inst) d_sblocks_.ensure((static_cast<size_t>(total) / kScreenBlock + 1) * sizeof(ScreenBlock));
   CameraDev cd{};
   cd.kc = kconst();
   cd.mp = maps_view(d_maps_, P_);
@@ -540,6 +541,7 @@ CameraDev SensorModel::stage(const RunsDev& runs, const uint32_t* pixel, size_t 
   cd.pixel = pixel;
   cd.pos = d_pos_.as<const int64_t>();
   cd.first_loud = inst ? d_loud_.as<int>() : nullptr;
+  cd.screen_blocks = inst ? d_sblocks_.as<ScreenBlock>() : nullptr;
   cd.work = d_work_.as<float>();
   cd.ev = d_events_.as<RawEventDev>();
   cd.cap = cap_;

@@ -206,6 +206,7 @@ class SensorModel {
   DeviceBuffer d_pos_, d_scan_;  // the runs' segment positions (exclusive scan of the run lengths)
   DeviceBuffer d_work_;  // per run: estimated walk cost (the heavy-first order's key)
   DeviceBuffer d_loud_;  // per run: the first segment that is not quiet under the input state
+  DeviceBuffer d_sblocks_;  // per kScreenBlock segment positions: phase A's photocurrent range
   std::unique_ptr<LaunchChain> chain_;
   // Host copies of a device call's check words and scan total (written by queued copies).
   unsigned int check_flags_[4] = {0, 0, 0, 0};

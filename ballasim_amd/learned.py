@@ -180,3 +180,104 @@ def event_error(test, ref, width):
             "median_abs_dt_ns": float(np.median(dt)) if dt.size else float("nan"),
             "p90_abs_dt_ns": float(np.percentile(dt, 90)) if dt.size else float("nan"),
             "paired_share": float(keep_b.sum() / max(len(tb), 1))}
+
+
+WINDOW_NS = 1 << 23
+
+
+def train_proposer(dark, n_examples=20_000_000, scene_seeds=tuple(range(1000, 1020)), theta_band=0.2, tol_log=0.01,
+                   batch=4096, schedule=((3e-3, 1), (1e-3, 1), (3e-4, 1))):
+    """The proposer trained on dense-trace examples of seeded config-1-style scenes (one equal share of
+    n_examples per seed, from 30 ms into each trace), the last bias set to the mean target, then Adam
+    over the (learning rate, epochs) schedule at `batch` rows per step. Returns (proposer, stats, X, T),
+    the examples left on the device for the caller."""
+    import time
+
+    import torch
+
+    from . import scene as S
+    n_per = n_examples // len(scene_seeds)
+    X = torch.empty((n_per * len(scene_seeds), N_IN), dtype=torch.float32, device="cuda")
+    T = torch.empty(n_per * len(scene_seeds), dtype=torch.float32, device="cuda")
+    t0 = time.perf_counter()
+    for k, seed in enumerate(scene_seeds):
+        sc, v = S.config1_scene(seed)
+        learned_examples(sc, v, X[k * n_per:(k + 1) * n_per], T[k * n_per:(k + 1) * n_per], dark, t0_ns=30_000_000,
+                         seed=seed, theta_band=theta_band, tol_log=tol_log)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    prop = Proposer(1)
+    w = prop.params
+    w[-1] = float(T.mean())
+    prop.params = w
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    loss = None
+    for lr, ep in schedule:
+        loss = prop.train(X, T, batch=batch, epochs=ep, lr=lr)
+    e1.record()
+    torch.cuda.synchronize()
+    stats = {"examples": int(X.shape[0]), "generation_s": gen_s, "rows_trained": int(X.shape[0]) * sum(e for _, e in schedule),
+             "train_ms": e0.elapsed_time(e1), "final_loss": loss, "batch": batch,
+             "learning_rates": [lr for lr, _ in schedule]}
+    return prop, stats, X, T
+
+
+def compare_controllers(scene, view, params, proposer, n_windows, theta_band=0.2, tol_log=0.01, margin=0.5, near=0.05,
+                        d_px_curve=(0.125, 0.25, 0.5, 1.0)):
+    """The learned controller (linked to the sensor it feeds) against the plain one (tol_log, d_px 0.25)
+    on the same scene over n_windows windows from t = 0, both judged against a 10-us dense reference
+    through the same sensor (event_error). The plain controller's samples-against-error curve over its
+    lattice density d_px gives the plain samples per event at the learned controller's median timing
+    error (log-log interpolation between the bracketing points): the gain at equal error."""
+    from . import scene as S
+    from .sensor import SensorModel
+    dark = float(params.dark_current_fA) / float(params.illuminance_scale)
+    frame = scene.render(view, 0)
+
+    def run(kind, d_px=0.25):
+        m = SensorModel(params, view.width, view.height, 1, ["constant"], 0, device="cuda")
+        m.init(0, frame.reshape(-1, 1))
+        ls = None
+        if kind == "learned":
+            ls = LearnedSampler(scene, view, learned_rules(S.rules(dark, tol=tol_log), tol_log=tol_log,
+                                                           theta_band=theta_band, margin=margin, near=near), proposer)
+            ls.init(0)
+        evs, samples, fallbacks, proposals = [], 0, 0, 0
+        for i in range(n_windows):
+            if kind == "dense":
+                r, tot = scene.sample_window(view, S.rules(dark, fixed_period_ns=10_000), i * WINDOW_NS, slot=0)
+            elif kind == "plain":
+                r, tot = scene.sample_window(view, S.rules(dark, tol=tol_log, d_px=d_px), i * WINDOW_NS, slot=0)
+            else:  # the controller reads the sensor's reference state at each window start
+                r, tot, pr, fb = ls.window(i * WINDOW_NS, slot=0, sensor=m)
+                fallbacks += fb
+                proposals += pr
+            samples += tot
+            ptr, cnt = m.advance_device_runs(r, (i + 1) * WINDOW_NS)
+            evs.append(m.download_events(ptr, cnt))
+        ev = {k: np.concatenate([e[k] for e in evs]) for k in evs[0]}
+        return ev, samples, proposals, fallbacks
+
+    ref, ref_samples, _, _ = run("dense")
+
+    def summary(ev, samples, proposals=0, fallbacks=0):
+        err = event_error(ev, ref, view.width)
+        return dict(samples=samples, events=err["n_test"], samples_per_event=samples / max(1, err["n_test"]),
+                    count_mismatch=err["count_mismatch"], median_abs_dt_ns=err["median_abs_dt_ns"],
+                    p90_abs_dt_ns=err["p90_abs_dt_ns"], proposals=proposals, fallbacks=fallbacks)
+
+    res = {"plain": summary(*run("plain")), "learned": summary(*run("learned"))}
+    curve = [dict(d_px=d, **summary(*run("plain", d))) for d in d_px_curve]
+    target = res["learned"]["median_abs_dt_ns"]
+    pts = sorted((c["median_abs_dt_ns"], c["samples_per_event"]) for c in curve)
+    equal = None
+    for (e0, s0), (e1, s1) in zip(pts, pts[1:]):
+        if e0 <= target <= e1 and e1 > e0:
+            f = (np.log(target) - np.log(e0)) / (np.log(e1) - np.log(e0))
+            equal = float(np.exp(np.log(s0) + f * (np.log(s1) - np.log(s0))))
+    res.update(curve=curve, dense={"samples": ref_samples, "events": int(len(ref["t_cross_ns"]))},
+               plain_samples_per_event_at_learned_median_error=equal,
+               gain_at_equal_median_error=equal / res["learned"]["samples_per_event"] if equal is not None else None,
+               gain_vs_plain=res["plain"]["samples_per_event"] / res["learned"]["samples_per_event"])
+    return res

@@ -246,103 +246,42 @@ BF16_PEAK_TFLOPS = 2500.0
 def bench_learned(args):
     """Trains the proposer on dense-trace examples of seeded config-1-style scenes, then compares the
     learned controller with the plain one on a held-out 1280x720 scene against a dense-sampled reference."""
-    import numpy as np
     import torch
 
     import ballasim_amd as bsx
     from ballasim_amd import scene as S
-    from ballasim_amd.learned import LearnedSampler, Proposer, event_error, learned_examples, learned_rules
+    from ballasim_amd.learned import compare_controllers, train_proposer
 
     p = rig_params(bsx, "instantaneous")
     dark = float(p.dark_current_fA) / float(p.illuminance_scale)
     band, tol, margin, near = 0.2, 0.01, 0.5, 0.05
-    n_train, n_scenes = args.train_examples, 20
-    n_per = n_train // n_scenes
-    X = torch.empty((n_per * n_scenes, 12), dtype=torch.float32, device="cuda")
-    T = torch.empty(n_per * n_scenes, dtype=torch.float32, device="cuda")
-    t0 = time.perf_counter()
-    for i in range(n_scenes):
-        sc, v = S.config1_scene(1000 + i)
-        learned_examples(sc, v, X[i * n_per:(i + 1) * n_per], T[i * n_per:(i + 1) * n_per], dark, t0_ns=30_000_000,
-                         seed=1000 + i, theta_band=band, tol_log=tol)
-    torch.cuda.synchronize()
-    gen_s = time.perf_counter() - t0
-    prop = Proposer(1)
-    w = prop.params
-    w[-1] = float(T.mean())
-    prop.params = w
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    schedule = ((3e-3, 1), (1e-3, 1), (3e-4, 1))
-    e0.record()
-    for lr, ep in schedule:
-        loss = prop.train(X, T, batch=4096, epochs=ep, lr=lr)
-    e1.record()
-    torch.cuda.synchronize()
-    train_ms = e0.elapsed_time(e1)
-    rows = X.shape[0] * sum(ep for _, ep in schedule)
+    # learning rates scaled with the square root of the batch against the 4096-row schedule
+    lr_scale = (args.train_batch / 4096) ** 0.5
+    schedule = ((3e-3 * lr_scale, 1), (1e-3 * lr_scale, 1), (3e-4 * lr_scale, 1))
+    prop, training, X, T = train_proposer(dark, n_examples=args.train_examples, theta_band=band, tol_log=tol,
+                                          batch=args.train_batch, schedule=schedule)
+    rows = training["rows_trained"]
+    train_ms = training["train_ms"]
+    training["rows_per_s"] = rows / (train_ms * 1e-3)
     train_tflops = rows * MLP_TRAIN_FLOP / (train_ms * 1e-3) / 1e12
     # inference throughput of the batched forward kernel
     Y = torch.empty(X.shape[0], dtype=torch.float32, device="cuda")
     prop.predict(X, Y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(3):
         prop.predict(X, Y)
     e1.record()
     torch.cuda.synchronize()
     infer_ms = e0.elapsed_time(e1) / 3
-    infer_tflops = X.shape[0] * MLP_FWD_FLOP / (infer_ms * 1e-3) / 1e12
+    n_rows = X.shape[0]
+    infer_tflops = n_rows * MLP_FWD_FLOP / (infer_ms * 1e-3) / 1e12
     del X, T, Y
     torch.cuda.empty_cache()
 
     sc, v = S.learned_eval_scene(args.seed + 7)
-    frame = sc.render(v, 0)
     n_win = max(1, int(round(args.eval_ms * 1e6 / WINDOW)))
-
-    def run(kind, d_px=0.25):
-        m = SensorModel(p, v.width, v.height, 1, ["constant"], 0, device="cuda")
-        m.init(0, frame.reshape(-1, 1))
-        ls = None
-        if kind == "learned":
-            ls = LearnedSampler(sc, v, learned_rules(S.rules(dark, tol=tol), tol_log=tol, theta_band=band,
-                                                     margin=margin, near=near), prop)
-            ls.init(0)
-        evs, samples, fallbacks, proposals = [], 0, 0, 0
-        for i in range(n_win):
-            if kind == "dense":
-                r, tot = sc.sample_window(v, S.rules(dark, fixed_period_ns=10_000), i * WINDOW, slot=0)
-            elif kind == "plain":
-                r, tot = sc.sample_window(v, S.rules(dark, tol=0.01, d_px=d_px), i * WINDOW, slot=0)
-            else:
-                # the controller reads the sensor's reference state at each window start
-                r, tot, pr, fb = ls.window(i * WINDOW, slot=0, sensor=m)
-                fallbacks += fb
-                proposals += pr
-            samples += tot
-            ptr, cnt = m.advance_device_runs(r, (i + 1) * WINDOW)
-            evs.append(m.download_events(ptr, cnt))
-        ev = {k: np.concatenate([e[k] for e in evs]) for k in evs[0]}
-        return ev, samples, proposals, fallbacks
-
-    from ballasim_amd.sensor import SensorModel
-    ref, ref_samples, _, _ = run("dense")
-
-    def summary(ev, samples, proposals=0, fallbacks=0):
-        err = event_error(ev, ref, v.width)
-        return dict(samples=samples, events=err["n_test"], samples_per_event=samples / max(1, err["n_test"]),
-                    count_mismatch=err["count_mismatch"], median_abs_dt_ns=err["median_abs_dt_ns"],
-                    p90_abs_dt_ns=err["p90_abs_dt_ns"], proposals=proposals, fallbacks=fallbacks)
-
-    res = {"plain": summary(*run("plain")), "learned": summary(*run("learned"))}
-    # The plain controller's trade of samples against error (its lattice density d_px), to compare at equal error:
-    # the plain samples per event that reach the learned controller's median timing error (log-log interpolation).
-    curve = [dict(d_px=d, **summary(*run("plain", d))) for d in (0.125, 0.25, 0.5, 1.0)]
-    target = res["learned"]["median_abs_dt_ns"]
-    pts = sorted((c["median_abs_dt_ns"], c["samples_per_event"]) for c in curve)
-    equal_err_spe = None
-    for (e0_, s0_), (e1_, s1_) in zip(pts, pts[1:]):
-        if e0_ <= target <= e1_ and e1_ > e0_:
-            f = (np.log(target) - np.log(e0_)) / (np.log(e1_) - np.log(e0_))
-            equal_err_spe = float(np.exp(np.log(s0_) + f * (np.log(s1_) - np.log(s0_))))
+    res = compare_controllers(sc, v, p, prop, n_win, theta_band=band, tol_log=tol, margin=margin, near=near)
     line = {
         "metric": "pixel-samples/s + events/s on 1 MI355X; irradiance samples/event at fixed error",
         "value": res["learned"]["samples_per_event"],
@@ -355,25 +294,22 @@ def bench_learned(args):
                                "20 seeded 346x260 disk scenes; evaluated at 1280x720 (12 textured disks, 800 px/s, "
                                "100 lux) over %d windows (%.0f ms) against a 10-us dense reference; thresholds "
                                "0.2 +- 0.02, refractory 100 us, instantaneous response, noise off"
-                               % (n_per * n_scenes, n_win, n_win * WINDOW / 1e6),
+                               % (training["examples"], n_win, n_win * WINDOW / 1e6),
                    "band": band, "tol_log": tol, "margin": margin, "near": near,
                    "sensor_linked": True},
-        "plain": res["plain"], "learned": res["learned"], "plain_density_curve": curve,
-        "dense_reference": {"samples": ref_samples, "events": int(len(ref["t_cross_ns"]))},
-        "samples_per_event_gain_vs_plain": res["plain"]["samples_per_event"] / res["learned"]["samples_per_event"],
-        "plain_samples_per_event_at_learned_median_error": equal_err_spe,
-        "gain_at_equal_median_error": (equal_err_spe / res["learned"]["samples_per_event"]
-                                       if equal_err_spe is not None else None),
-        "training": {"examples": int(n_per * n_scenes), "generation_s": gen_s, "rows_trained": int(rows),
-                     "train_ms": train_ms, "rows_per_s": rows / (train_ms * 1e-3), "final_loss": loss},
+        "plain": res["plain"], "learned": res["learned"], "plain_density_curve": res["curve"],
+        "dense_reference": res["dense"],
+        "samples_per_event_gain_vs_plain": res["gain_vs_plain"],
+        "plain_samples_per_event_at_learned_median_error": res["plain_samples_per_event_at_learned_median_error"],
+        "gain_at_equal_median_error": res["gain_at_equal_median_error"],
+        "training": training,
         "roofline": {"bound": "mfma", "achieved": train_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": train_tflops / BF16_PEAK_TFLOPS, "traffic": None,
                      "kernel": "gradients_kernel + adam_kernel (training step)",
-                     "inference_tflops": infer_tflops, "inference_rows_per_s": X_rows_per_s(infer_ms, n_per * n_scenes)},
+                     "inference_tflops": infer_tflops, "inference_rows_per_s": X_rows_per_s(infer_ms, n_rows)},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
-
 
 def X_rows_per_s(ms, rows):
     return rows / (ms * 1e-3)
@@ -400,6 +336,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shape (no autotuner trials before the timed windows)")
     ap.add_argument("--train-examples", type=int, default=20_000_000, help="learned: training examples")
+    ap.add_argument("--train-batch", type=int, default=4096,
+                    help="learned: rows per optimizer step (a multiple of 64; 64 rows per group of four waves)")
     ap.add_argument("--eval-ms", type=float, default=500.0, help="learned: evaluation span (ms)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work (0: skip)")
     ap.add_argument("--seed", type=int, default=0)

@@ -277,6 +277,43 @@ def test_short_runs_many_runs_per_phase_a_block(kind):
         assert np.array_equal(st[f], so[f]), f
 
 
+@pytest.mark.parametrize("kind", ["instantaneous", "cascade"])
+def test_sparse_runs_among_many_empty_ones(kind):
+    """Every pixel of a 1024x512 camera listed, all but a few hundred with no samples (a static scene's
+    pixels take none), so phase A's threads cross stretches of thousands of empty runs, which share one
+    position: events, reported times and the state equal the oracle's."""
+    import ballasim_amd as bsx
+    W, H = 1024, 512
+    P = W * H
+    rng = np.random.default_rng(57)
+    p = _kind_params(kind)
+    gpu = bsx.SensorModel(p, W, H, 4, device="cuda")
+    ref = O.OracleModel(p.c, W, H, 4, 1, 0)
+    irr0 = np.exp(rng.normal(3, 1, size=(P, 1))).astype(np.float32)
+    gpu.init(0, irr0)
+    ref.init(0, irr0)
+    ids = np.arange(P, dtype=np.uint32)
+    counts = np.zeros(P, np.int64)
+    busy = np.sort(rng.choice(P, size=300, replace=False))
+    counts[busy] = rng.integers(1, 40, size=len(busy))
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    t = np.zeros(offs[-1], np.int64)
+    for k, pix in enumerate(busy):
+        s0, c = offs[pix], counts[pix]
+        t[s0:s0 + c] = np.sort(rng.choice(np.arange(1, 2 ** 23), size=c, replace=False))
+    irr = np.exp(rng.normal(3, 1.5, size=(offs[-1], 1))).astype(np.float32)
+    ev = gpu.advance(ids, t, irr, 2 ** 23, offsets=offs)
+    want, rep = ref.advance_runs(ids, t, irr, offs, 2 ** 23)
+    got = events_from_dict(ev, W)
+    i, a, b = first_difference(got, want)
+    assert len(got) == len(want) and i == len(got) and len(want) > 100, f"first difference at {i}: gpu {a} oracle {b}"
+    assert np.array_equal(ev["t_reported_ns"], rep)
+    st = np.frombuffer(gpu.save_state()[56:], O.REF_STATE_DTYPE)
+    so = ref.states()
+    for f in ("z", "s3_ref", "t_ref", "t_search", "blind_until", "flags", "held"):
+        assert np.array_equal(st[f], so[f]), f
+
+
 def test_device_runs_padding_not_checked():
     """Device runs that leave samples unreferenced: the padding between runs holds NaN, infinities
     and over-range values. Only the referenced samples are checked, as the reference's

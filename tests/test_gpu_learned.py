@@ -114,3 +114,28 @@ def test_learned_controller_gain():
     assert out["learned"]["samples_per_event"] < out["untrained"]["samples_per_event"] or \
         out["learned"]["fallbacks"] < out["untrained"]["fallbacks"]
     assert out["learned"]["count_mismatch"] < 0.01
+
+
+def test_learned_gain_at_equal_error_config4():
+    """Config 4 at its own workload: the proposer trained on 20 M dense-trace examples of 20 seeded
+    scenes, then the learned controller against the plain one at 1280x720 over 503 ms (60 windows)
+    against a 10-us dense reference. The plain controller's samples-against-error curve over its lattice
+    density gives its samples per event at the learned controller's median timing error; the learned
+    controller must need at least 2x fewer there, with a count mismatch of at most 1 %. The north star's
+    1-us median timing error is not reached by either controller (printed, not asserted)."""
+    from ballasim_amd import scene as S
+    from ballasim_amd.configs import config3_params
+    from ballasim_amd.learned import compare_controllers, train_proposer
+    p = config3_params("instantaneous")
+    dark = float(p.dark_current_fA) / float(p.illuminance_scale)
+    prop, training, X, T = train_proposer(dark, n_examples=20_000_000)
+    del X, T
+    sc, v = S.learned_eval_scene(7)
+    res = compare_controllers(sc, v, p, prop, 60)
+    print(json.dumps({k: res[k] for k in ("plain", "learned", "gain_at_equal_median_error", "gain_vs_plain")},
+                     indent=1))
+    print(f"median timing error: learned {res['learned']['median_abs_dt_ns'] / 1e3:.1f} us, plain "
+          f"{res['plain']['median_abs_dt_ns'] / 1e3:.1f} us (target 1 us: not met)")
+    assert res["gain_at_equal_median_error"] is not None and res["gain_at_equal_median_error"] >= 2.0
+    assert res["learned"]["count_mismatch"] <= 0.01 and res["plain"]["count_mismatch"] <= 0.01
+    assert res["gain_vs_plain"] >= 2.0
