@@ -139,6 +139,7 @@ SIGNATURES = {
     "bsx_sensor_advance": (C.c_int, [P, P, C.c_size_t, P, P, C.c_size_t, P, C.c_int64]),
     "bsx_sensor_advance_runs": (C.c_int, [P, P, C.c_size_t, P, P, C.c_size_t, P, P, C.c_int64]),
     "bsx_sensor_advance_device": (C.c_int, [P, C.POINTER(DeviceRuns), C.c_int64, P, C.POINTER(DeviceEvents)]),
+    "bsx_sensor_held_frame": (C.c_int, [P, P, P, P]),
     "bsx_sensor_advance_device_noalloc": (C.c_int, [P, C.POINTER(DeviceRuns), C.c_int64, P, P, C.c_uint64,
                                                     C.POINTER(C.c_uint64)]),
     "bsx_sensor_download_events": (C.c_int, [P, C.POINTER(DeviceEvents)]),

@@ -301,6 +301,10 @@ int bsx_sensor_advance_runs(bsx_sensor* s, const uint32_t* pixel_ids, size_t n, 
   });
 }
 
+int bsx_sensor_held_frame(bsx_sensor* s, const bsx_device_runs* runs, void* stream, float* out) {
+  return guarded([&] { model_of(s).held_frame(runs, static_cast<hipStream_t>(stream), out); });
+}
+
 int bsx_sensor_advance_device(bsx_sensor* s, const bsx_device_runs* runs, int64_t t_end_ns, void* stream,
                               bsx_device_events* out) {
   return guarded([&] {

@@ -49,8 +49,11 @@ __global__ __launch_bounds__(256) void gradients_kernel(const uint16_t* wb, cons
   __shared__ WeightsLds W;
   __shared__ WaveLds S;
   __shared__ uint16_t dz2[kRowsPerWave * kS16];
-  __shared__ uint16_t dz1[kRowsPerWave * kS16];
   __shared__ float dys[kRowsPerWave];
+  // dz1 lives in z2's storage, which no step reads once dw3 is formed (a barrier separates them): 74 KB
+  // of LDS per group, so that two groups fit in a CU's 160 KB.
+  static_assert(sizeof(uint16_t) * kRowsPerWave * kS16 <= sizeof(S.z2), "dz1 fits in z2's storage");
+  uint16_t* const dz1 = reinterpret_cast<uint16_t*>(S.z2);
   load_weights(W, wb, w);
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
   const int mt = wave >> 1, nt = wave & 1;
@@ -120,6 +123,7 @@ __global__ __launch_bounds__(256) void gradients_kernel(const uint16_t* wb, cons
     mma_tile(acc, S.h1 + mt * 32, 1, kS16, dz2 + nt * 32, kS16, 1, kRowsPerWave);
     for (int j = 0; j < 16; ++j) P[kOffW2 + (mt * 32 + acc_row(j)) * kHid + nt * 32 + acc_col()] = acc[j];
   }
+  __syncthreads();  // z2 is read (dw3) before its storage takes dz1
   // dh1 = dz2 W2^T, dz1 = dh1 silu'(z1), this wave's tile
   {
     f32x16 acc = {};

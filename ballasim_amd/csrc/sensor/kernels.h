@@ -51,6 +51,11 @@ hipError_t launch_scatter_rows(const uint32_t* pixel, const int64_t* row_start, 
                                size_t n_samples, int32_t* row_of, uint32_t n_pixels, unsigned int* flags,
                                hipStream_t stream);
 hipError_t launch_fill_u64(unsigned long long* dst, size_t n, unsigned long long v, hipStream_t stream);
+// out[p * groups + g]: pixel p's run's last sample (row_of[p] >= 0 and a nonempty run; row_of null: no
+// runs), else held[g * n_pixels + p].
+hipError_t launch_held_frame(const float* held, const int32_t* row_of, const int64_t* row_start,
+                             const int32_t* row_count, const float* irr, uint32_t n_pixels, int groups, float* out,
+                             hipStream_t stream);
 // Non-finite (out[0]) and clamped (out[1]) values of the samples the runs reference, one wave per run.
 hipError_t launch_check_runs(const float* irr, const int64_t* row_start, const int32_t* row_count, size_t n_runs,
                              size_t n_samples, int groups, float irr_max, unsigned long long* out,

@@ -1093,6 +1093,21 @@ __global__ void scatter_rows_kernel(const uint32_t* pixel, const int64_t* row_st
   if (prev >= 0) atomicOr(flags + 1, 1u);
 }
 
+// The input each pixel holds after a call on the runs (row_of: its run, or -1): its run's last
+// sample, raw, else its held input (reference: end_frame_kernel, sensor_cuda.cu:390-404); out is
+// pixels x groups.
+__global__ void held_frame_kernel(const float* held, const int32_t* row_of, const int64_t* row_start,
+                                  const int32_t* row_count, const float* irr, uint32_t n_pixels, int groups,
+                                  float* out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pixels) return;
+  const int32_t row = row_of != nullptr ? row_of[p] : -1;
+  const bool from_run = row >= 0 && row_count[row] > 0;
+  const float* v = from_run ? irr + static_cast<int64_t>(row_start[row] + row_count[row] - 1) * groups : nullptr;
+  for (int g = 0; g < groups; ++g)
+    out[static_cast<int64_t>(p) * groups + g] = from_run ? v[g] : held[static_cast<int64_t>(g) * n_pixels + p];
+}
+
 __global__ void fill_i32_kernel(int32_t* dst, size_t n, int32_t v) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = v;
@@ -1291,6 +1306,15 @@ hipError_t launch_scatter_rows(const uint32_t* pixel, const int64_t* row_start, 
   if (n > 0)
     scatter_rows_kernel<<<grid_for(n, block), block, 0, stream>>>(pixel, row_start, row_count, n, n_samples, row_of,
                                                                   n_pixels, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_held_frame(const float* held, const int32_t* row_of, const int64_t* row_start,
+                             const int32_t* row_count, const float* irr, uint32_t n_pixels, int groups, float* out,
+                             hipStream_t stream) {
+  if (n_pixels == 0) return hipSuccess;
+  held_frame_kernel<<<grid_for(n_pixels, 256), 256, 0, stream>>>(held, row_of, row_start, row_count, irr, n_pixels,
+                                                                 groups, out);
   return hipGetLastError();
 }
 

@@ -645,6 +645,31 @@ void SensorModel::refuse_non_finite(const unsigned long long* c) {
                               " non-finite irradiance samples");
 }
 
+void SensorModel::held_frame(const bsx_device_runs* runs, hipStream_t stream, float* out) {
+  if (out == nullptr && P_ > 0) throw std::invalid_argument("SensorModel.held_frame: null output");
+  const bool with_runs = runs != nullptr && runs->n > 0;
+  if (with_runs) {
+    d_row_of_.ensure(P_ * 4);
+    d_flags_.ensure(64);
+    hip_check(launch_scatter_rows(runs->pixel, runs->row_start, runs->row_count, runs->n, runs->n_samples,
+                                  d_row_of_.as<int32_t>(), static_cast<uint32_t>(P_), d_flags_.as<unsigned int>(),
+                                  stream),
+              "held frame rows");
+  }
+  d_frame_.ensure(std::max<size_t>(P_ * G_, 1) * sizeof(float));
+  const StateDev st = state_view(cur_);
+  hip_check(launch_held_frame(st.held, with_runs ? d_row_of_.as<const int32_t>() : nullptr,
+                              with_runs ? runs->row_start : nullptr, with_runs ? runs->row_count : nullptr,
+                              with_runs ? runs->irr : nullptr, static_cast<uint32_t>(P_), static_cast<int>(G_),
+                              d_frame_.as<float>(), stream),
+            "held frame");
+  if (P_ > 0) {
+    hip_check(hipMemcpyAsync(out, d_frame_.get(), P_ * G_ * sizeof(float), hipMemcpyDeviceToHost, stream),
+              "held frame download");
+  }
+  hip_check(hipStreamSynchronize(stream), "held frame");
+}
+
 bsx_device_events SensorModel::advance_device(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream) {
   enqueue_device_checks(runs, t_end_ns, stream);
   hip_check(hipStreamSynchronize(stream), "checks");

@@ -101,6 +101,9 @@ class SensorModel {
   void advance_runs(const uint32_t* ids, size_t n, const int64_t* t, const float* irr, size_t n_samples,
                     const int64_t* row_start, const int32_t* row_count, int64_t t_end_ns);
   bsx_device_events advance_device(const bsx_device_runs& runs, int64_t t_end_ns, hipStream_t stream);
+  // The input each pixel would hold after a call on these runs (runs null or empty: the held inputs),
+  // pixels x groups into out (host), after the work queued on stream; the model is not changed.
+  void held_frame(const bsx_device_runs* runs, hipStream_t stream, float* out);
   // advance_device for callers that cannot allocate: the call's scratch (run index, checks,
   // counters, descriptors) lives in the library's static device scratch, the raw events go to the
   // caller's device buffer of cap events, and nothing is allocated. The walk runs without phase A and

@@ -185,12 +185,22 @@ def event_error(test, ref, width):
 WINDOW_NS = 1 << 23
 
 
+def train_schedule(batch):
+    """The (learning rate, epochs) schedule of train_proposer: three epochs at 3e-3, 1e-3 and 3e-4 for
+    4096-row steps, the rates scaled with the square root of the batch for larger steps."""
+    k = (batch / 4096) ** 0.5
+    return ((3e-3 * k, 1), (1e-3 * k, 1), (3e-4 * k, 1))
+
+
 def train_proposer(dark, n_examples=20_000_000, scene_seeds=tuple(range(1000, 1020)), theta_band=0.2, tol_log=0.01,
-                   batch=4096, schedule=((3e-3, 1), (1e-3, 1), (3e-4, 1))):
+                   batch=65536, schedule=None):
     """The proposer trained on dense-trace examples of seeded config-1-style scenes (one equal share of
     n_examples per seed, from 30 ms into each trace), the last bias set to the mean target, then Adam
-    over the (learning rate, epochs) schedule at `batch` rows per step. Returns (proposer, stats, X, T),
-    the examples left on the device for the caller."""
+    over the (learning rate, epochs) schedule (default train_schedule(batch)) at `batch` rows per step
+    (1,024 groups of 64 rows at the default: every CU busy). Returns (proposer, stats, X, T), the
+    examples left on the device for the caller."""
+    if schedule is None:
+        schedule = train_schedule(batch)
     import time
 
     import torch

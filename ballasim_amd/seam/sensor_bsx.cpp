@@ -170,8 +170,14 @@ void CudaSensorKernel::set_glare(const GlareRamp<float>& glare) {
   if (glare.on && !zero) throw std::invalid_argument("bsx: veiling glare is not supported by the bsx backend");
 }
 
-std::vector<float> CudaSensorKernel::end_frame(const DeviceRuns&, void*) {
-  throw std::invalid_argument("bsx: veiling glare (the held frame's means) is not supported by the bsx backend");
+std::vector<float> CudaSensorKernel::end_frame(const DeviceRuns& runs, void* stream) {
+  bsx_sensor_info_t info{};
+  check(bsx_sensor_info(impl_->s, &info), "end_frame");
+  std::vector<float> out(static_cast<size_t>(info.width) * info.height * static_cast<size_t>(info.groups));
+  if (out.empty()) return out;
+  const bsx_device_runs r = runs_of(runs);
+  check(bsx_sensor_held_frame(impl_->s, runs.n > 0 ? &r : nullptr, stream, out.data()), "end_frame");
+  return out;
 }
 
 DeviceEvents CudaSensorKernel::advance_device(const DeviceRuns& runs, int64_t t_state, int64_t t_end, void* stream,

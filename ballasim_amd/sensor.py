@@ -224,6 +224,19 @@ class SensorModel:
                                                        C.byref(out)))
         return out.data, out.count
 
+    def held_frame(self, runs=None, stream=0):
+        """The input each pixel holds after a call on `runs` (device runs, as advance_device_runs
+        takes, or a tuple of torch CUDA tensors): its run's last sample, raw, or else the input it
+        holds now. With no runs, the held inputs. (height, width, groups) float32; the model is not
+        changed. Reference: SensorModel::held_frame, src/sensor/sensor_model.cpp:748-760, and
+        CudaSensorKernel::end_frame, src/sensor/sensor_cuda.cu:989-1010."""
+        out = np.empty((self._hgt, self._w, self._g), np.float32)
+        r = None if runs is None else _device_runs(runs, self._g, "SensorModel.held_frame")
+        st = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
+        _abi.check(self._lib.bsx_sensor_held_frame(self._h, None if r is None else C.byref(r), C.c_void_p(st),
+                                                   _ptr(out)))
+        return out
+
     def download_events(self, data_ptr, count):
         ev = _abi.DeviceEvents(data_ptr, count)
         _abi.check(self._lib.bsx_sensor_download_events(self._h, C.byref(ev)))

@@ -255,11 +255,8 @@ def bench_learned(args):
     p = rig_params(bsx, "instantaneous")
     dark = float(p.dark_current_fA) / float(p.illuminance_scale)
     band, tol, margin, near = 0.2, 0.01, 0.5, 0.05
-    # learning rates scaled with the square root of the batch against the 4096-row schedule
-    lr_scale = (args.train_batch / 4096) ** 0.5
-    schedule = ((3e-3 * lr_scale, 1), (1e-3 * lr_scale, 1), (3e-4 * lr_scale, 1))
     prop, training, X, T = train_proposer(dark, n_examples=args.train_examples, theta_band=band, tol_log=tol,
-                                          batch=args.train_batch, schedule=schedule)
+                                          batch=args.train_batch)
     rows = training["rows_trained"]
     train_ms = training["train_ms"]
     training["rows_per_s"] = rows / (train_ms * 1e-3)
@@ -336,8 +333,9 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shape (no autotuner trials before the timed windows)")
     ap.add_argument("--train-examples", type=int, default=20_000_000, help="learned: training examples")
-    ap.add_argument("--train-batch", type=int, default=4096,
-                    help="learned: rows per optimizer step (a multiple of 64; 64 rows per group of four waves)")
+    ap.add_argument("--train-batch", type=int, default=65536,
+                    help="learned: rows per optimizer step (a multiple of 64; 64 rows per group of four waves; the "
+                         "learning rates scale with its square root)")
     ap.add_argument("--eval-ms", type=float, default=500.0, help="learned: evaluation span (ms)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline work (0: skip)")
     ap.add_argument("--seed", type=int, default=0)

@@ -273,6 +273,12 @@ int bsx_sensor_advance(bsx_sensor* s, const uint32_t* pixel_ids, size_t n, const
 /* Reference: SensorModel::advance_runs(...), src/sensor/sensor_model.h:157-158. Host arrays. */
 int bsx_sensor_advance_runs(bsx_sensor* s, const uint32_t* pixel_ids, size_t n, const int64_t* t, const float* irr,
                             size_t n_samples, const int64_t* row_start, const int32_t* row_count, int64_t t_end_ns);
+/* Reference: CudaSensorKernel::end_frame(runs, stream), src/sensor/sensor_cuda.h:47-49 (SensorModel's
+ * held frame, sensor_model.cpp:748-760): the input each pixel would hold after a call on these device
+ * runs, its run's last sample (raw) or else its held input; runs null or empty: the held inputs.
+ * out: pixels x groups floats on the host, written after the work queued on `stream`. The model is not
+ * changed. */
+int bsx_sensor_held_frame(bsx_sensor* s, const bsx_device_runs* runs, void* stream, float* out);
 /* Reference: SensorModel::advance_device(runs, t_end, stream), src/sensor/sensor_model.h:166. The
  * runs are in device memory; the events stay there (out). */
 int bsx_sensor_advance_device(bsx_sensor* s, const bsx_device_runs* runs, int64_t t_end_ns, void* stream,

@@ -2,7 +2,8 @@
 // reference's sources with BALLASIM_SENSOR_CUDA=1 and ballasim_amd/seam/sensor_bsx.cpp in place of its
 // CUDA backend (oracle/Makefile, target seam), so Device::kCuda runs on libbsx.so. The same seeded calls
 // go to a Precision::kFp32 model on Device::kCpu (the reference's CPU path) and on Device::kCuda; their
-// events (every field, reported times included), saved states and work counters must be identical.
+// events (every field, reported times included), saved states, work counters and the frame means of the
+// held inputs (the backend's end_frame on kCuda) must be identical.
 //
 //   seam_check build   constructs the kCuda model: without a device the reference's own runtime_error
 //                      must come back (prints "no device: ..."), with one it must construct
@@ -128,6 +129,10 @@ int run_case(const Case& c) {
   std::vector<Waveform> groups(1);
   SensorModel cpu(p, W, H, seed, groups, 1, 2, Precision::kFp32, Device::kCpu);
   SensorModel gpu(p, W, H, seed, groups, 1, 0, Precision::kFp32, Device::kCuda);
+  // The frame means of the held inputs at every call's end: on kCuda they come from the backend's
+  // end_frame (the held frame on the device, with the call's device runs or, for host runs, none).
+  cpu.set_track_frame_means(true);
+  gpu.set_track_frame_means(true);
   std::mt19937_64 rng(seed);
   std::normal_distribution<double> N(3.0, 1.0);
   std::vector<float> irr0(P);
@@ -166,6 +171,10 @@ int run_case(const Case& c) {
     }
     if (!same(a, b, why)) {
       std::printf("case %s call %d: events differ: %s\n", c.name, call, why.c_str());
+      return 1;
+    }
+    if (cpu.glare_frame_mean() != gpu.glare_frame_mean() || gpu.glare_frame_mean().size() != groups.size()) {
+      std::printf("case %s call %d: frame means of the held inputs differ\n", c.name, call);
       return 1;
     }
     events += a.size();
