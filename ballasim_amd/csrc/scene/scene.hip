@@ -46,7 +46,8 @@ __device__ uint32_t sample_pixel(const SceneDev& sc, const ViewDev& view, const 
     }
     ++count;
   };
-  const auto at = [&](uint32_t dt) { return eval_scene(sc, mask, x, y, tw_s + static_cast<double>(dt) * 1e-9); };
+  // A sample's time in seconds as the engine's render hook forms it: (t_w + dt) 1e-9.
+  const auto at = [&](uint32_t dt) { return eval_scene(sc, mask, x, y, static_cast<double>(t_w + dt) * 1e-9); };
   if (r.fixed_period > 0) {
     for (int64_t d = r.fixed_period; d <= kWindowNs; d += r.fixed_period) emit(static_cast<uint32_t>(d), at(d).e);
     return count;
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(256) void counts_write_kernel(SceneDev sc, ViewDev 
   for (int w = 0; w < kMaskWords; ++w) mask[w] = 0;
   for (int i = 0; i < sc.n; ++i) mask[i >> 6] |= 1ull << (i & 63);
   t[s] = t_w + dt;
-  irr[s] = eval_scene(sc, mask, x, y, static_cast<double>(t_w) * 1e-9 + static_cast<double>(dt) * 1e-9).e;
+  irr[s] = eval_scene(sc, mask, x, y, static_cast<double>(t_w + dt) * 1e-9).e;
 }
 
 // The requests of the engine's RenderHook: request i (pixel, time t_base + t_off_ns) writes the scene's

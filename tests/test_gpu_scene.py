@@ -112,8 +112,8 @@ def _levels(shapes, view, t_w, d_px=0.25, l_min=1, l_max=13):
 
 
 def _signatures(sc, view, t_w, ids, t, offs):
-    """The scene's signature (1 + shape index, 0 the background) at the carried sample and at every sample
-    of every run, rendered on the device through the engine's request interface."""
+    """The scene's signature (1 + shape index, 0 the background) and value at the carried sample and at
+    every sample of every run, rendered on the device through the engine's request interface."""
     import torch
 
     from ballasim_amd import scene as S
@@ -131,7 +131,7 @@ def _signatures(sc, view, t_w, ids, t, offs):
     sig = torch.zeros(len(rows), dtype=torch.int64, device=dev)
     sc.render_requests(view, t_w, rq, len(rows), None, irr, sig)
     torch.cuda.synchronize()
-    return sig.cpu().numpy(), rows
+    return sig.cpu().numpy(), irr.cpu().numpy(), rows
 
 
 @pytest.mark.parametrize("signature", [False, True])
@@ -140,16 +140,15 @@ def test_runs_well_formed_and_rules_hold(signature):
     (the motion-bound rule, restated in numpy; a pixel no shape reaches takes no samples), and is exactly
     the sample set the controller's rules give on its own values and, with signature flags on, on the
     scene's signatures at those samples (a flip between an interval's ends refines it). The signatures
-    come from the engine's request interface, which forms a sample's time as (t_w + dt) 1e-9 s where the
-    controller forms t_w 1e-9 + dt 1e-9; the two agree bit for bit only at t_w = 0, so the signature
-    case checks the first window."""
+    come from the engine's request interface, which forms a sample's time as (t_w + dt) 1e-9 s, as the
+    controller does, so the values of the two agree bit for bit in every window."""
     from ballasim_amd import scene as S
     sc, view, shapes, _ = _level_scene()
     dark, tol = 0.0021, 0.01
     rules = S.rules(dark, tol=tol, signature=signature)
     checked = refined = flips = static = 0
     levels_seen = set()
-    for w in ([0] if signature else range(3)):
+    for w in range(3):
         runs, total = sc.sample_window(view, rules, w * WINDOW, slot=0)
         ids, t, irr, offs = S.runs_to_host(runs)
         carried = sc.render(view, w * WINDOW)
@@ -158,10 +157,13 @@ def test_runs_well_formed_and_rules_hold(signature):
         assert total == len(t) == offs[-1]
         sig_at = None
         if signature:
-            sig, rows = _signatures(sc, view, w * WINDOW, ids, t, offs)
+            sig, rendered, rows = _signatures(sc, view, w * WINDOW, ids, t, offs)
             sig_at = {}
             for (p, off), v in zip(rows, sig):
                 sig_at.setdefault(p, {})[off] = int(v)
+            # The request interface renders every sample's value bit for bit (the carried ones: the frame's).
+            want = np.concatenate([np.concatenate([[carried[p]], irr[offs[p]:offs[p + 1], 0]]) for p in ids])
+            assert np.array_equal(rendered, want.astype(np.float32))
         for p in range(len(ids)):
             tt = t[offs[p]:offs[p + 1]] - w * WINDOW
             if level_of[p] < 0:

@@ -26,3 +26,20 @@ def test_reference_sensor_model_builds_against_bsx():
         assert "no device: SensorModel: no CUDA device available" in out.stdout, out.stdout
     else:
         assert "constructed on the device" in out.stdout, out.stdout
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/src/engine"), reason="the reference's sources are absent")
+def test_reference_window_engine_builds_against_bsx():
+    """The reference's host WindowEngine (src/engine/window.cpp, par_host.cpp) with the engine hook of
+    ballasim_amd/seam/scene_hook_bsx.h, linked to libbsx.so (oracle/Makefile, target window): it builds,
+    loads next to libbsx.so and exports its entry layer. tests/test_gpu_window_engine.py runs it."""
+    import ctypes
+
+    from oracle import window_ref
+    r = subprocess.run(["make", "-s", "-f", os.path.join(os.path.dirname(O.__file__), "Makefile"), "window"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert window_ref.available()
+    lib = window_ref._load()
+    for name in ("wr_run", "wr_samples", "wr_fetch", "wr_free"):
+        assert isinstance(getattr(lib, name), ctypes._CFuncPtr)
