@@ -200,6 +200,8 @@ SIGNATURES = {
     "bsx_scene_render": (C.c_int, [P, C.POINTER(View), C.c_int64, P]),
     "bsx_scene_sample_counts": (C.c_int, [P, C.POINTER(View), P, C.c_int64, C.c_int32, P, C.POINTER(DeviceRuns),
                                           C.POINTER(C.c_uint64)]),
+    "bsx_scene_set_capture_budget": (C.c_int, [P, C.c_uint64]),
+    "bsx_scene_capture_budget": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "bsx_scene_sample_window": (C.c_int, [P, C.POINTER(View), C.POINTER(SamplerRules), C.c_int64, C.c_int32, P,
                                           C.POINTER(DeviceRuns), C.POINTER(C.c_uint64)]),
 }

@@ -29,10 +29,14 @@ __global__ void render_kernel(SceneDev sc, ViewDev view, double t, float* out) {
   out[p] = eval_scene(sc, mask, x, y, t).e;
 }
 
-// One pixel's samples over (t_w, t_w + W]: counted (kWrite false) or written from out_base.
-template <bool kWrite>
+// What one pass of sample_pixel does with each sample: count it, write it into the pixel's run (t_out,
+// irr_out), or capture the first `cap` of them as (dt, value) pairs into the pixel's scratch slots.
+enum class Emit { kCount, kWrite, kCapture };
+
+// One pixel's samples over (t_w, t_w + W] under mode M; returns the pixel's sample count.
+template <Emit M>
 __device__ uint32_t sample_pixel(const SceneDev& sc, const ViewDev& view, const RulesDev& r, int64_t t_w, uint32_t p,
-                                 int64_t* t_out, float* irr_out) {
+                                 int64_t* t_out, float* irr_out, CapturedSample* cap_out = nullptr, uint32_t cap = 0) {
   const double x = view.ox + (static_cast<double>(p % view.width) + 0.5) * view.scale;
   const double y = view.oy + (static_cast<double>(p / view.width) + 0.5) * view.scale;
   const double tw_s = static_cast<double>(t_w) * 1e-9;
@@ -40,9 +44,11 @@ __device__ uint32_t sample_pixel(const SceneDev& sc, const ViewDev& view, const 
   const float v_hat = plan_pixel(sc, x, y, tw_s, tw_s + kWindowNs * 1e-9, mask);
   uint32_t count = 0;
   const auto emit = [&](uint32_t dt, float e) {
-    if constexpr (kWrite) {
+    if constexpr (M == Emit::kWrite) {
       t_out[count] = t_w + dt;
       irr_out[count] = e;
+    } else if constexpr (M == Emit::kCapture) {
+      if (count < cap) cap_out[count] = CapturedSample{dt, e};
     }
     ++count;
   };
@@ -111,7 +117,41 @@ __global__ __launch_bounds__(256) void count_kernel(SceneDev sc, ViewDev view, R
                                                     int32_t* counts) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= view.width * view.height) return;
-  counts[p] = static_cast<int32_t>(sample_pixel<false>(sc, view, r, t_w, p, nullptr, nullptr));
+  counts[p] = static_cast<int32_t>(sample_pixel<Emit::kCount>(sc, view, r, t_w, p, nullptr, nullptr));
+}
+
+// The one-pass form: every pixel's count, its first `cap` samples captured into scratch slots
+// [p cap, (p + 1) cap), and the window's largest count (atomicMax of the waves' maxima) in *max_count.
+__global__ __launch_bounds__(256) void capture_kernel(SceneDev sc, ViewDev view, RulesDev r, int64_t t_w, uint32_t cap,
+                                                      int32_t* counts, CapturedSample* scratch,
+                                                      unsigned int* max_count) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t c = 0;
+  if (p < view.width * view.height) {
+    c = sample_pixel<Emit::kCapture>(sc, view, r, t_w, p, nullptr, nullptr,
+                                     scratch + static_cast<size_t>(p) * cap, cap);
+    counts[p] = static_cast<int32_t>(c);
+  }
+  for (int off = 32; off > 0; off >>= 1) c = max(c, static_cast<uint32_t>(__shfl_xor(static_cast<int>(c), off)));
+  if ((threadIdx.x & 63) == 0 && c > 0) atomicMax(max_count, c);
+}
+
+// The captured runs moved to their place in the window's runs: one wave per pixel, its samples copied
+// 64 at a time. A pixel with more samples than slots is left to overflow_write_kernel.
+__global__ __launch_bounds__(256) void compact_kernel(uint32_t n_pixels, int64_t t_w, uint32_t cap,
+                                                      const int32_t* counts, const int64_t* row_start,
+                                                      const CapturedSample* scratch, int64_t* t, float* irr) {
+  const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= n_pixels) return;
+  const uint32_t c = static_cast<uint32_t>(counts[p]);
+  if (c > cap) return;
+  const CapturedSample* src = scratch + static_cast<size_t>(p) * cap;
+  const int64_t base = row_start[p];
+  for (uint32_t k = threadIdx.x & 63; k < c; k += 64) {
+    const CapturedSample v = src[k];
+    t[base + k] = t_w + v.dt;
+    irr[base + k] = v.e;
+  }
 }
 
 __global__ __launch_bounds__(256) void write_kernel(SceneDev sc, ViewDev view, RulesDev r, int64_t t_w,
@@ -119,7 +159,17 @@ __global__ __launch_bounds__(256) void write_kernel(SceneDev sc, ViewDev view, R
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= view.width * view.height) return;
   const int64_t s = row_start[p];
-  sample_pixel<true>(sc, view, r, t_w, p, t + s, irr + s);
+  sample_pixel<Emit::kWrite>(sc, view, r, t_w, p, t + s, irr + s);
+}
+
+// write_kernel for the pixels whose count exceeds the capture slots (the others return at once).
+__global__ __launch_bounds__(256) void overflow_write_kernel(SceneDev sc, ViewDev view, RulesDev r, int64_t t_w,
+                                                             uint32_t cap, const int32_t* counts,
+                                                             const int64_t* row_start, int64_t* t, float* irr) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= view.width * view.height || static_cast<uint32_t>(counts[p]) <= cap) return;
+  const int64_t s = row_start[p];
+  sample_pixel<Emit::kWrite>(sc, view, r, t_w, p, t + s, irr + s);
 }
 
 // Samples at imposed per-pixel counts: pixel p's n = counts[p] samples at t_w + floor((k + 1) W / n),
@@ -235,6 +285,11 @@ __global__ void scan_add_kernel(int64_t* out, size_t n, const int64_t* block_sum
   if (i < n) out[i] += block_sums[blockIdx.x];
 }
 
+__global__ void fill_u32_kernel(unsigned int* dst, size_t n, unsigned int v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+
 RulesDev rules_dev(const bsx_sampler_rules& r) {
   RulesDev d{};
   d.d_px = r.d_px;
@@ -270,6 +325,34 @@ hipError_t launch_write(const SceneDev& sc, const ViewDev& view, const bsx_sampl
   const size_t P = static_cast<size_t>(view.width) * view.height;
   write_kernel<<<static_cast<unsigned>((P + 255) / 256), 256, 0, stream>>>(sc, view, rules_dev(r), t_w, row_start, t,
                                                                           irr);
+  return hipGetLastError();
+}
+
+hipError_t launch_capture(const SceneDev& sc, const ViewDev& view, const bsx_sampler_rules& r, int64_t t_w,
+                          uint32_t cap, int32_t* counts, CapturedSample* scratch, unsigned int* max_count,
+                          hipStream_t stream) {
+  const size_t P = static_cast<size_t>(view.width) * view.height;
+  fill_u32_kernel<<<1, 64, 0, stream>>>(max_count, 1, 0u);
+  capture_kernel<<<static_cast<unsigned>((P + 255) / 256), 256, 0, stream>>>(sc, view, rules_dev(r), t_w, cap, counts,
+                                                                            scratch, max_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact(const ViewDev& view, int64_t t_w, uint32_t cap, const int32_t* counts,
+                          const int64_t* row_start, const CapturedSample* scratch, int64_t* t, float* irr,
+                          hipStream_t stream) {
+  const size_t P = static_cast<size_t>(view.width) * view.height;
+  compact_kernel<<<static_cast<unsigned>((P + 3) / 4), 256, 0, stream>>>(static_cast<uint32_t>(P), t_w, cap, counts,
+                                                                        row_start, scratch, t, irr);
+  return hipGetLastError();
+}
+
+hipError_t launch_overflow_write(const SceneDev& sc, const ViewDev& view, const bsx_sampler_rules& r, int64_t t_w,
+                                 uint32_t cap, const int32_t* counts, const int64_t* row_start, int64_t* t, float* irr,
+                                 hipStream_t stream) {
+  const size_t P = static_cast<size_t>(view.width) * view.height;
+  overflow_write_kernel<<<static_cast<unsigned>((P + 255) / 256), 256, 0, stream>>>(sc, view, rules_dev(r), t_w, cap,
+                                                                                   counts, row_start, t, irr);
   return hipGetLastError();
 }
 

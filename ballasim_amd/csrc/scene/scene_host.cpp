@@ -1,6 +1,7 @@
 // The scene object and its C ABI (include/bsx/bsx_scene.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -162,6 +163,8 @@ class Scene {
     w.row_start.ensure(P * 8);
     scan_.ensure(scan_scratch_count(P) * 8);
     ensure_ids(P);
+    const uint32_t cap = capture_slots(P);
+    if (cap > 0 && reserve_capture(P, cap)) return sample_once(v, r, t_w, w, P, cap, stream, total_out);
     hip_check(launch_count(dev(), v, r, t_w, w.counts.as<int32_t>(), stream), "count samples");
     hip_check(launch_exclusive_scan(w.counts.as<int32_t>(), P, w.row_start.as<int64_t>(), scan_.as<int64_t>(), stream),
               "scan");
@@ -178,9 +181,78 @@ class Scene {
     return runs_of(w, P, total);
   }
 
+  // Device memory the one-pass sampler may hold for its capture slots (0: the two-pass sampler).
+  void set_capture_budget(uint64_t bytes) {
+    capture_budget_ = bytes;
+    if (capture_.bytes() > bytes) {
+      DeviceBuffer none;
+      capture_.swap(none);
+    }
+  }
+  uint64_t capture_budget() const { return capture_budget_; }
+  // Capture slots per pixel for a view of P pixels: as many as the budget holds, at most kMaxCaptureSlots.
+  uint32_t capture_slots(size_t P) const {
+    const uint64_t k = capture_budget_ / (static_cast<uint64_t>(P) * sizeof(CapturedSample));
+    return static_cast<uint32_t>(std::min<uint64_t>(k, kMaxCaptureSlots));
+  }
+
+ private:
+  static constexpr uint32_t kMaxCaptureSlots = 1024;
+
+  // One pass over the scene: every pixel's samples counted and its first `cap` captured; the counts
+  // scanned; the captured runs copied into place and the pixels with more samples sampled again there.
+  // The capture slots and the largest-count word; when the device cannot hold them, the budget drops to
+  // 0 and this scene samples in two passes from then on.
+  bool reserve_capture(size_t P, uint32_t cap) {
+    try {
+      capture_.ensure(P * cap * sizeof(CapturedSample) + 256);
+      return true;
+    } catch (const std::runtime_error&) {
+      (void)hipGetLastError();
+      set_capture_budget(0);
+      return false;
+    }
+  }
+
+  bsx_device_runs sample_once(const ViewDev& v, const bsx_sampler_rules& r, int64_t t_w, WindowSlot& w, size_t P,
+                              uint32_t cap, hipStream_t stream, uint64_t* total_out) {
+    auto* scratch = capture_.as<CapturedSample>();
+    auto* max_count = reinterpret_cast<unsigned int*>(capture_.as<uint8_t>() + P * cap * sizeof(CapturedSample));
+    hip_check(launch_capture(dev(), v, r, t_w, cap, w.counts.as<int32_t>(), scratch, max_count, stream),
+              "capture samples");
+    hip_check(launch_exclusive_scan(w.counts.as<int32_t>(), P, w.row_start.as<int64_t>(), scan_.as<int64_t>(), stream),
+              "scan");
+    int64_t total = 0;
+    uint32_t most = 0;
+    const size_t blocks = scan_scratch_count(P) - 1;
+    hip_check(hipMemcpyAsync(&total, scan_.as<int64_t>() + blocks, 8, hipMemcpyDeviceToHost, stream), "total");
+    hip_check(hipMemcpyAsync(&most, max_count, 4, hipMemcpyDeviceToHost, stream), "largest count");
+    hip_check(hipStreamSynchronize(stream), "sample window");
+    w.t.ensure(std::max<int64_t>(total, 1) * 8);
+    w.irr.ensure(std::max<int64_t>(total, 1) * 4);
+    hip_check(launch_compact(v, t_w, cap, w.counts.as<const int32_t>(), w.row_start.as<const int64_t>(), scratch,
+                             w.t.as<int64_t>(), w.irr.as<float>(), stream),
+              "place samples");
+    if (most > cap) {
+      hip_check(launch_overflow_write(dev(), v, r, t_w, cap, w.counts.as<const int32_t>(),
+                                      w.row_start.as<const int64_t>(), w.t.as<int64_t>(), w.irr.as<float>(), stream),
+                "write samples");
+      ++overflow_windows_;
+    }
+    hip_check(hipStreamSynchronize(stream), "sample window");
+    if (total_out != nullptr) *total_out = static_cast<uint64_t>(total);
+    return runs_of(w, P, total);
+  }
+
+ public:
+  uint64_t overflow_windows() const { return overflow_windows_; }
+
  private:
   int32_t n_;
   float background_;
+  uint64_t capture_budget_ = uint64_t{16} << 30;
+  uint64_t overflow_windows_ = 0;
+  DeviceBuffer capture_;
   DeviceBuffer shapes_, frame_, scan_;
   std::vector<std::unique_ptr<DeviceBuffer>> ids_;
   std::vector<std::unique_ptr<WindowSlot>> slots_;
@@ -382,6 +454,21 @@ int bsx_scene_sample_window(bsx_scene* s, const bsx_view* view, const bsx_sample
       throw std::invalid_argument("bsx_scene_sample_window: null");
     }
     *out = s->s->sample(*view, *rules, t_w, slot, static_cast<hipStream_t>(stream), total_samples);
+  });
+}
+
+int bsx_scene_set_capture_budget(bsx_scene* s, uint64_t bytes) {
+  return scene_guarded([&] {
+    if (s == nullptr) throw std::invalid_argument("bsx_scene_set_capture_budget: null");
+    s->s->set_capture_budget(bytes);
+  });
+}
+
+int bsx_scene_capture_budget(bsx_scene* s, uint64_t* bytes, uint64_t* overflow_windows) {
+  return scene_guarded([&] {
+    if (s == nullptr) throw std::invalid_argument("bsx_scene_capture_budget: null");
+    if (bytes != nullptr) *bytes = s->s->capture_budget();
+    if (overflow_windows != nullptr) *overflow_windows = s->s->overflow_windows();
   });
 }
 

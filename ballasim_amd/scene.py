@@ -48,6 +48,17 @@ class Scene:
                                                      C.c_void_p(st), C.byref(runs), C.byref(total)))
         return runs, total.value
 
+    def set_capture_budget(self, nbytes):
+        """Device memory the one-pass sampler may hold (0: count and write in two passes); the runs are the
+        same either way."""
+        _abi.check(self._lib.bsx_scene_set_capture_budget(self._h, int(nbytes)))
+
+    def capture_budget(self):
+        """(budget in bytes, windows whose largest pixel count exceeded the capture slots)."""
+        b, o = C.c_uint64(), C.c_uint64()
+        _abi.check(self._lib.bsx_scene_capture_budget(self._h, C.byref(b), C.byref(o)))
+        return b.value, o.value
+
     def sample_counts(self, view, counts, t_w, slot=0, stream=0):
         """Runs over (t_w, t_w + 2^23] at imposed per-pixel counts (int32, width*height), samples
         evenly spread in the window: (DeviceRuns, samples)."""

@@ -79,6 +79,16 @@ int bsx_scene_render(bsx_scene* s, const bsx_view* view, int64_t t_ns, float* ou
 int bsx_scene_sample_window(bsx_scene* s, const bsx_view* view, const bsx_sampler_rules* rules, int64_t t_w,
                             int32_t slot, void* stream, bsx_device_runs* out, uint64_t* total_samples);
 
+/* The device memory bsx_scene_sample_window may hold for its one-pass form (default 16 GiB). With
+ * k = min(1024, bytes / (8 width height)) >= 1 slots per pixel it renders each sample once: every pixel's
+ * first k samples are captured as 8-byte (offset, value) slots while it is counted, then copied into place
+ * once the counts are scanned; a pixel with more samples is sampled again in place. With bytes too small
+ * for one slot per pixel it counts in one pass and writes in a second. The runs are the same either way.
+ * No reference counterpart. When the device cannot hold the slots, the budget drops to 0. */
+int bsx_scene_set_capture_budget(bsx_scene* s, uint64_t bytes);
+/* The capture budget now in force, and the windows so far whose largest pixel count exceeded the slots. */
+int bsx_scene_capture_budget(bsx_scene* s, uint64_t* bytes, uint64_t* overflow_windows);
+
 /* Samples of window (t_w, t_w + 2^23 ns] at imposed per-pixel counts (a host array of width*height
  * values in [0, 2^23]): pixel p's n samples at t_w + floor((k + 1) 2^23 / n), k = 0 .. n - 1, with the
  * scene's irradiance at each; runs in pixel order in buffer set `slot`, as bsx_scene_sample_window.

@@ -319,3 +319,36 @@ def test_render_requests_equal_frames():
         assert irr[row, 1] == 0.0
         assert (kl[row] != 0) == (sig[row] != 0)
     assert (sig != 0).any() and (sig == 0).any()
+
+
+@pytest.mark.parametrize("scene", ["levels", "rectangles"])
+def test_one_pass_sampler_equals_two_passes(scene):
+    """The one-pass sampler (each pixel's first k samples captured while it is counted, then copied into
+    place; a pixel with more samples sampled again in place) gives the runs of the two-pass sampler (count,
+    then write), with slots for every pixel's samples, with 4 slots (nearly every pixel sampled again) and
+    with a budget below one slot per pixel (two passes)."""
+    from ballasim_amd import scene as S
+    if scene == "levels":
+        sc, view, _, _ = _level_scene()
+    else:
+        shapes, bg = S.config2_shapes(5, n=16, width=320, height=180)
+        sc, view = S.Scene(shapes, bg), S.view(320, 180)
+    P = view.width * view.height
+    rules = S.rules(0.0021, tol=0.01, signature=True)
+    budgets = {"two passes": 0, "4 slots": 4 * 8 * P, "default": None}
+    for w in range(3):
+        got = {}
+        for name, b in budgets.items():
+            if b is not None:
+                sc.set_capture_budget(b)
+            else:
+                sc.set_capture_budget(16 << 30)
+            runs, total = sc.sample_window(view, rules, w * WINDOW, slot=0)
+            got[name] = S.runs_to_host(runs) + (total,)
+        ref = got["two passes"]
+        for name, g in got.items():
+            for a, b_ in zip(ref, g):
+                assert np.array_equal(np.asarray(a), np.asarray(b_)), f"window {w}: {name} differs from two passes"
+    budget, overflow = sc.capture_budget()
+    assert budget == 16 << 30
+    assert overflow >= 3  # the 4-slot windows
