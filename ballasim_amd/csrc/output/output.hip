@@ -1,18 +1,22 @@
-// Kernels of the output stage: event build with crop and readout, key passes, rocPRIM's stable radix
-// sort, gather with the window cut, and the C4 columns.
+// Kernels of the output stage: event build with crop and readout, key passes, a stable LSD radix sort,
+// gather with the window cut, and the C4 columns.
 #include <hip/hip_runtime.h>
-
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "output_kernels.h"
 
 namespace bsx {
 
+// Exclusive scan of int32 counts into int64 offsets (defined with the scene kernels).
+size_t scan_scratch_count(size_t n);
+hipError_t launch_exclusive_scan(const int32_t* counts, size_t n, int64_t* out, int64_t* scratch, hipStream_t stream);
+
 namespace {
 
-// S1 class of a sensor label (signal 0, noise 2, leak 3, hot 4).
+// S1 class of a sensor label's two class bits (signal 0, noise 2, leak 3, hot 4; reference: label_s1,
+// src/sensor/output_order.h:23-26).
 __device__ __forceinline__ uint8_t label_s1(uint8_t c) {
-  return c == 0 ? 0 : static_cast<uint8_t>((c & 3) + 1);
+  const uint8_t c3 = c & 3;
+  return c3 == 0 ? 0 : static_cast<uint8_t>(c3 + 1);
 }
 
 constexpr uint64_t kBias = uint64_t{1} << 63;
@@ -120,6 +124,77 @@ __global__ void pack_kernel(const OutEventDev* ev, size_t n, StreamEventDev* dst
   dst[i] = StreamEventDev{o.t_reported, o.x, o.y, o.p, o.label, o.flags};
 }
 
+// The radix sort: 8-bit digits, least significant first. A block owns a tile of kSortTile consecutive
+// items (kSortItems per thread, item k * 256 + thread of the tile at step k, so that the block walks its
+// tile in order). Per digit: every tile's digit histogram, written digit-major (hist[d * tiles + tile]);
+// the exclusive scan of that array, which gives each (digit, tile) its first output slot with every
+// smaller digit, and every earlier tile's equal digits, before it; then the scatter, in which each tile
+// ranks its items stably (the wave's equal digits by an 8-ballot match, the earlier waves' through LDS
+// counts) and writes them from its slots on.
+constexpr int kSortBits = 8;
+constexpr int kSortBins = 1 << kSortBits;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = 256 * kSortItems;
+
+__global__ __launch_bounds__(256) void digit_hist_kernel(const uint64_t* keys, size_t n, int shift, size_t tiles,
+                                                         int32_t* hist) {
+  __shared__ uint32_t h[kSortBins];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = static_cast<size_t>(blockIdx.x) * kSortTile;
+  for (int k = 0; k < kSortItems; ++k) {
+    const size_t i = base + static_cast<size_t>(k) * 256 + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kSortBins - 1)], 1u);
+  }
+  __syncthreads();
+  hist[static_cast<size_t>(threadIdx.x) * tiles + blockIdx.x] = static_cast<int32_t>(h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void digit_scatter_kernel(const uint64_t* keys, const uint32_t* vals, size_t n,
+                                                            int shift, size_t tiles, const int64_t* offsets,
+                                                            uint64_t* keys_out, uint32_t* vals_out) {
+  __shared__ uint32_t base[kSortBins];          // this tile's next free slot of each digit
+  __shared__ uint32_t wave_count[4][kSortBins];  // the current step's items of each digit, per wave
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  base[threadIdx.x] = static_cast<uint32_t>(offsets[static_cast<size_t>(threadIdx.x) * tiles + blockIdx.x]);
+  for (int w = 0; w < 4; ++w) wave_count[w][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const size_t tile = static_cast<size_t>(blockIdx.x) * kSortTile;
+  for (int k = 0; k < kSortItems; ++k) {
+    const size_t i = tile + static_cast<size_t>(k) * 256 + threadIdx.x;
+    const bool valid = i < n;
+    uint64_t key = 0;
+    uint32_t val = 0, d = 0;
+    if (valid) {
+      key = keys[i];
+      val = vals[i];
+      d = static_cast<uint32_t>((key >> shift) & (kSortBins - 1));
+    }
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < kSortBits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t set = __ballot(bit);
+      peers &= bit ? set : ~set;
+    }
+    const uint32_t rank = static_cast<uint32_t>(__popcll(peers & below));
+    const bool leader = valid && rank == 0;
+    if (leader) wave_count[wave][d] = static_cast<uint32_t>(__popcll(peers));
+    __syncthreads();
+    if (valid) {
+      uint32_t slot = base[d] + rank;
+      for (int w = 0; w < wave; ++w) slot += wave_count[w][d];
+      keys_out[slot] = key;
+      vals_out[slot] = val;
+    }
+    __syncthreads();
+    base[threadIdx.x] += wave_count[0][threadIdx.x] + wave_count[1][threadIdx.x] + wave_count[2][threadIdx.x] +
+                         wave_count[3][threadIdx.x];
+    __syncthreads();
+    if (leader) wave_count[wave][d] = 0;
+  }
+}
+
 inline unsigned grid(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
 }  // namespace
@@ -148,13 +223,33 @@ hipError_t launch_out_keys(const OutEventDev* ev, const uint32_t* perm, uint32_t
 hipError_t launch_out_sort(uint64_t* keys, uint64_t* keys_out, uint32_t* perm, uint32_t* perm_out, size_t n, int bits,
                            DeviceBuffer& temp, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  size_t bytes = 0;
-  hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, keys, keys_out, perm, perm_out, n, 0u,
-                                           static_cast<unsigned>(bits), stream);
-  if (e != hipSuccess) return e;
-  void* t = temp.ensure(bytes > 0 ? bytes : 1);
-  return rocprim::radix_sort_pairs(t, bytes, keys, keys_out, perm, perm_out, n, 0u, static_cast<unsigned>(bits),
-                                   stream);
+  if (n >= (size_t{1} << 32) || bits < 1 || bits > 64) return hipErrorInvalidValue;
+  const size_t tiles = (n + kSortTile - 1) / kSortTile;
+  const size_t cells = tiles * kSortBins;
+  const size_t scan_words = scan_scratch_count(cells);
+  uint8_t* t = static_cast<uint8_t*>(temp.ensure(cells * 4 + cells * 8 + scan_words * 8));
+  int32_t* hist = reinterpret_cast<int32_t*>(t);
+  int64_t* offsets = reinterpret_cast<int64_t*>(t + cells * 4);
+  int64_t* scan = reinterpret_cast<int64_t*>(t + cells * 12);
+  uint64_t* k_src = keys;
+  uint64_t* k_dst = keys_out;
+  uint32_t* v_src = perm;
+  uint32_t* v_dst = perm_out;
+  for (int shift = 0; shift < bits; shift += kSortBits) {
+    digit_hist_kernel<<<static_cast<unsigned>(tiles), 256, 0, stream>>>(k_src, n, shift, tiles, hist);
+    hipError_t e = launch_exclusive_scan(hist, cells, offsets, scan, stream);
+    if (e != hipSuccess) return e;
+    digit_scatter_kernel<<<static_cast<unsigned>(tiles), 256, 0, stream>>>(k_src, v_src, n, shift, tiles, offsets,
+                                                                           k_dst, v_dst);
+    std::swap(k_src, k_dst);
+    std::swap(v_src, v_dst);
+  }
+  if (v_src != perm_out) {  // an even number of digits: the sorted pairs are in the input buffers
+    hipError_t e = hipMemcpyAsync(keys_out, keys, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(perm_out, perm, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_out_gather(const OutEventDev* ev, const uint32_t* perm, size_t n, int64_t cut, OutEventDev* sorted,

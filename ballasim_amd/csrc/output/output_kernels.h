@@ -20,7 +20,8 @@ hipError_t launch_out_minmax(const OutEventDev* ev, size_t n_held, size_t n_max,
 // null: identity, perm_init filled with it).
 hipError_t launch_out_keys(const OutEventDev* ev, const uint32_t* perm, uint32_t* perm_init, size_t n, int pass,
                            int y_bits, int64_t tc_min, int64_t tr_min, uint64_t* keys, hipStream_t stream);
-// Stable radix sort of (keys, perm) by the key's low `bits` bits; the sorted copies land in keys_out / perm_out.
+// Stable radix sort of (keys, perm) by the key's low `bits` bits (n < 2^32); the sorted copies land in keys_out /
+// perm_out, and keys / perm are overwritten.
 hipError_t launch_out_sort(uint64_t* keys, uint64_t* keys_out, uint32_t* perm, uint32_t* perm_out, size_t n, int bits,
                            DeviceBuffer& temp, hipStream_t stream);
 // sorted[i] = ev[perm[i]], counting those reported before cut into scratch[5].

@@ -83,3 +83,55 @@ def test_output_stage_matches_oracle(crop, readout, tmp_path):
     # entry k is the FIRST event with t >= 1000 k
     assert np.all((ms == len(t)) | (t[np.minimum(ms, len(t) - 1)] >= 1000 * k))
     assert np.all((ms == 0) | (t[np.maximum(ms - 1, 0)] < 1000 * k))
+
+
+def test_output_order_of_millions_of_events():
+    """The stage's stable radix passes at scale: about 2.5 M synthetic raw events per window over a
+    1280x720 camera (several hundred sort tiles per pass), their crossing times on a coarse grid and a
+    1-us readout quantum, so that the reported time, x and y each tie among many events and the order rests
+    on every pass being stable. Two windows and the flush, against the oracle's lexsort."""
+    import torch
+
+    import ballasim_amd as bsx
+    from ballasim_amd.output import OutputStage
+    p = bsx.preset("prophesee_gen3")
+    p.photoreceptor = "instantaneous"
+    p.noise = False
+    p.readout_delay_s = 3e-4
+    p.readout_quantum_s = 1e-6
+    Wd, Hd = 1280, 720
+    m = bsx.SensorModel(p, Wd, Hd, 3, ["constant"], 0, device="cuda")
+    m.init(0, np.full((Wd * Hd, 1), 50.0, np.float32))
+    info = m._info()
+    rd = dict(quantum=round(p.readout_quantum_s * 1e9), delay=round(p.readout_delay_s * 1e9),
+              phase=int(info.readout_phase_ns), offset=int(p.clock_offset_ns), drift_ppm=float(p.clock_drift_ppm))
+    stage = OutputStage(m, t_offset_us=0)
+    oracle = Q.OutputOracle(Wd, Hd, rd, t_offset_us=0)
+    rng = np.random.default_rng(11)
+    for w in range(2):
+        n = 2_600_000
+        pix = rng.integers(0, Wd * Hd, n).astype(np.uint32)
+        tc = (w * W + rng.integers(0, W // 4096, n) * 4096 + rng.integers(0, 3, n)).astype(np.int64)
+        key = pix.astype(np.int64) * (1 << 24) + (tc - w * W)
+        _, first = np.unique(key, return_index=True)  # one event per (pixel, t_cross): a total order
+        first = rng.permutation(first)
+        raw = np.zeros(len(first), O.EVENT_DTYPE)
+        raw["pixel"] = pix[first]
+        raw["t_cross"] = tc[first]
+        raw["polarity"] = rng.integers(0, 2, len(first))
+        raw["label"] = rng.integers(0, 8, len(first))  # the class is the label's two low bits
+        raw["flags"] = rng.integers(0, 4, len(first))
+        dev = torch.from_numpy(raw.view(np.uint8).copy()).cuda()
+        torch.cuda.synchronize()
+        got = stage.window((dev.data_ptr(), len(raw)), w * W)
+        want = oracle.window(raw, w * W)
+        assert len(got["t"]) == len(want["t"]) > 10 ** 6, w
+        for k_got, k_want in (("x", "x"), ("y", "y"), ("p", "p"), ("t_reported_ns", "t_reported"),
+                              ("t_cross_ns", "t_cross"), ("label", "label"), ("flags", "flags")):
+            assert np.array_equal(got[k_got], want[k_want]), (w, k_got)
+        assert stage.held == len(oracle.held["t_cross"]) > 0
+        del dev
+    got = stage.flush()
+    want = oracle.flush()
+    for k_got, k_want in (("x", "x"), ("y", "y"), ("t_reported_ns", "t_reported"), ("t_cross_ns", "t_cross")):
+        assert np.array_equal(got[k_got], want[k_want]), k_got
