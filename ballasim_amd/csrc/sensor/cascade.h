@@ -510,8 +510,98 @@ struct CascadeLane : InstantLane<kMaxG> {
     }
   }
 
-  // The sub-steps of one linear piece, each followed by the comparator (cmp(sub): the plain one,
-  // or the noise lane's tree comparator).
+  // The sub-steps of one linear piece as a resumable machine, so that a wave's lanes can share each
+  // sub-step attempt whatever piece or segment they are in: piece_begin(pc); then piece_prepare(cmp)
+  // runs the settled sub-steps (each followed by cmp) and opens the next sub-step's first try, and
+  // returns false once the piece is done; piece_attempt(cmp) makes one try_substep and, on success,
+  // commits the sub-step and runs the comparator on it (cmp(sub): the plain one, or the noise lane's
+  // tree comparator). Each lane makes the same steps in the same order as a loop over the piece.
+  cascade::Piece pc_;
+  int64_t sub_ta_, sub_tb_, h_hint_;
+  float sub_Ia_;
+  bool attempting_;
+
+  BSX_HD BSX_INLINE void piece_begin(const cascade::Piece& pc) {
+    pc_ = pc;
+    sub_ta_ = pc.t_a;
+    h_hint_ = 0;
+    attempting_ = false;
+  }
+
+  template <class Cmp>
+  BSX_HD BSX_INLINE bool piece_prepare(Cmp&& cmp) {
+    while (!attempting_) {
+      if (!(sub_ta_ < pc_.t_b)) return false;
+      const cascade::State st{z, s2, s3};
+      sub_Ia_ = pc_.at(sub_ta_);
+      ++n_substeps;
+      if (pc_.I_a == pc_.I_b && cascade::settled(cc, st, x1, sub_Ia_)) {
+        // settled under constant input: the rest of the piece in one step, no transcendental
+        cascade::SubStep sub;
+        sub.t_a = sub_ta_;
+        const float h = static_cast<float>(pc_.t_b - sub_ta_) * 1e-9f;
+        sub.s3 = cascade::ExpCubic{{s3, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, {cc.a2, cc.a3}, 0};
+        sub.x1_a = sub.x1_b = x1;
+        sub.int_ku = cc.k * h / z;
+        sub.fallback = false;
+        ++n_settled;
+        if (tv != 0.0f) tv = tv * pm::exp(-sub.int_ku);
+        sub.t_b = pc_.t_b;
+        sub_ta_ = pc_.t_b;
+        cmp(sub);
+      } else {
+        sub_tb_ = h_hint_ > 0 ? vmin(pc_.t_b, sub_ta_ + 2 * h_hint_) : pc_.t_b;
+        attempting_ = true;
+      }
+    }
+    return true;
+  }
+
+  template <class Cmp>
+  BSX_HD BSX_INLINE void piece_attempt(Cmp&& cmp) {
+    const int64_t t_a = sub_ta_, t_b = sub_tb_, h_ns = t_b - t_a;
+    const cascade::State st{z, s2, s3};
+    cascade::State next{0.0f, 0.0f, 0.0f};
+    cascade::SubStep sub;
+    ++n_attempts;
+    if (!cascade::try_substep(cc, st, i_scale, sub_Ia_, pc_.at(t_b), static_cast<float>(h_ns) * 1e-9f, h_ns <= 1, sub,
+                              next)) {
+      sub_tb_ = t_a + h_ns / 2;
+      return;
+    }
+    attempting_ = false;
+    if (t_b < pc_.t_b || h_hint_ > 0) h_hint_ = t_b - t_a;
+    if (sub.fallback) {
+      flags |= kFlagSolverFallback;
+      ++n_fallbacks;
+    }
+    if (!(next.z - next.z == 0.0f && next.z > 0.0f && next.s2 - next.s2 == 0.0f && next.s3 - next.s3 == 0.0f)) {
+      // quarantine: a non-finite cascade restarts at the steady state of the current input
+      const float I = pc_.at(t_b);
+      z = 1.0f / I;
+      x1 = -pm::log(z * i_scale);
+      s2 = s3 = x1;
+      tv = 0.0f;
+      sub.s3 = cascade::ExpCubic{{s3, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f}, {cc.a2, cc.a3}, 0};
+      sub.x1_a = sub.x1_b = x1;
+      flags |= kFlagQuarantined;
+      ++n_quarantines;
+    } else {
+      z = next.z;
+      s2 = next.s2;
+      s3 = next.s3;
+      x1 = sub.x1_b;
+      tv = tv * pm::exp(-sub.int_ku) + fabs_(sub.x1_b - sub.x1_a);
+    }
+    sub.t_a = t_a;
+    sub.t_b = t_b;
+    sub_ta_ = t_b;
+    cmp(sub);
+  }
+
+  // The sub-steps of one linear piece in nested loops, each followed by the comparator (cmp(sub): the
+  // noise lane's tree comparator). The same steps in the same order as piece_begin, piece_prepare and
+  // piece_attempt; the noise lane keeps its loops, which hold its state in fewer registers.
   template <class Cmp>
   BSX_HD BSX_INLINE void run_piece(const cascade::Piece& pc, Cmp&& cmp) {
     int64_t t_a = pc.t_a, h_hint = 0;
@@ -572,27 +662,55 @@ struct CascadeLane : InstantLane<kMaxG> {
     }
   }
 
+
+  // The call as a resumable machine over the segments and their pieces: begin(t_end); then
+  // prepare(sink) returns kAttempt when a sub-step try is due (attempt(sink) makes it) and kFinished
+  // once the call is done; then end().
+  static constexpr int kAttempt = 1, kFinished = 2;
+
+  template <class Sink>
+  BSX_HD BSX_INLINE int prepare(Sink& sink) {
+    const auto cmp = [&](const cascade::SubStep& sub) { compare(sub, sink); };
+    while (true) {
+      if (this->in_piece) {
+        if (piece_prepare(cmp)) return kAttempt;
+        this->in_piece = false;
+        this->t_cur = pc_.t_b;
+      }
+      if (!this->seg_open) {
+        if (!(this->t_cur < this->t_end_)) return kFinished;
+        this->open_next();
+        this->stop = vmin(this->t_end_, tb);
+        this->seg_open = true;
+      }
+      if (this->t_cur < this->stop) {
+        const int64_t t = this->t_cur;
+        const int64_t e = vmin(this->stop, ((t >> kPlainSegmentLog2) + 1) * kPlainSegmentNs);
+        ++n_pieces;
+        piece_begin(cascade::Piece{t, e, this->photocurrent(t), this->photocurrent(e)});
+        this->in_piece = true;
+        continue;
+      }
+      // the segment is done: the next one starts at its end sample
+      if (tb == kNoBreakpoint) return kFinished;
+      this->seg_open = false;
+      ta = tb;
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < G) va[g] = vb[g];
+    }
+  }
+
+  template <class Sink>
+  BSX_HD BSX_INLINE void attempt(Sink& sink) {
+    piece_attempt([&](const cascade::SubStep& sub) { compare(sub, sink); });
+  }
+
   // The call: the segments of the augmented samples in order, cut at multiples of 2^23 ns, each a
   // linear piece of photocurrent (constant waveforms); the segment ends are fetched kAhead samples ahead.
   template <class Sink>
   BSX_HD BSX_INLINE void advance(int64_t t_end, Sink& sink) {
     this->begin(t_end);
-    int64_t t = ta;
-    while (t < t_end) {
-      this->open_next();
-      const int64_t stop = vmin(t_end, tb);
-      while (t < stop) {
-        const int64_t e = vmin(stop, ((t >> kPlainSegmentLog2) + 1) * kPlainSegmentNs);
-        ++n_pieces;
-        run_piece(cascade::Piece{t, e, this->photocurrent(t), this->photocurrent(e)},
-                  [&](const cascade::SubStep& sub) { compare(sub, sink); });
-        t = e;
-      }
-      if (tb == kNoBreakpoint) break;
-      ta = tb;
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < G) va[g] = vb[g];
-    }
+    while (prepare(sink) == kAttempt) attempt(sink);
     this->end();
   }
 };
