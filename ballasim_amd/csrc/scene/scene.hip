@@ -120,17 +120,182 @@ __global__ __launch_bounds__(256) void count_kernel(SceneDev sc, ViewDev view, R
   counts[p] = static_cast<int32_t>(sample_pixel<Emit::kCount>(sc, view, r, t_w, p, nullptr, nullptr));
 }
 
+// sample_pixel's adaptive controller as a resumable machine, so that a wave's lanes share every
+// scene evaluation whatever lattice slot or refinement node each is at: a lane states the offset it
+// needs evaluated next (want), the wave evaluates all of them at once, and feed(value) runs the lane's
+// controller on until it needs the next one. A lane makes the same evaluations and emits the same
+// samples in the same order as sample_pixel.
+struct PixelSampler {
+  struct Node {
+    uint32_t lo, hi;
+    float e_lo, e_hi;
+    uint32_t s_lo, s_hi;
+    int32_t depth;  // > 0: refine at this depth; -1: emit (lo, e_lo)
+  };
+  enum : int { kCarried, kFirst, kSecond, kNode, kSlide, kDone };
+  RulesDev r;
+  uint32_t n, step, j;
+  int stage;
+  bool mid;
+  Sample sm2, sm1, s0, sp1;
+  Node* stack;  // the lane's refinement stack (16 nodes), apart from the rest so that the rest stays in registers
+  int sp;
+  Node cur;          // the node whose midpoint is being evaluated
+  uint32_t want;     // the offset to evaluate next (stage != kDone)
+  CapturedSample* out;
+  uint32_t cap, count;
+
+  __device__ __forceinline__ void emit(uint32_t dt, float e) {
+    if (count < cap) out[count] = CapturedSample{dt, e};
+    ++count;
+  }
+
+  // The flags of lattice interval j, then its refinement (or, with none, its end sample).
+  __device__ __forceinline__ void interval() {
+    const uint32_t lo = (j - 1) * step, hi = j * step;
+    const bool s = r.signature && sm1.sig != s0.sig;
+    bool m = false;
+    if (mid) {
+      m = (j + 1 <= n && midpoint_eta(r.dark, sm1.e, s0.e, sp1.e) > r.tol) ||
+          (j >= 2 && midpoint_eta(r.dark, sm2.e, sm1.e, s0.e) > r.tol);
+    }
+    sp = 0;
+    if ((s || m) && step >= r.min_split && r.max_levels > 0) stack[sp++] = Node{lo, hi, sm1.e, s0.e, sm1.sig, s0.sig, 1};
+    pop();
+  }
+
+  // Pops the refinement stack until a midpoint needs evaluating; with the stack empty, emits the
+  // interval's end sample and slides the lattice window.
+  __device__ __forceinline__ void pop() {
+    while (sp > 0) {
+      const Node nd = stack[--sp];
+      if (nd.depth < 0) {
+        emit(nd.lo, nd.e_lo);
+        continue;
+      }
+      cur = nd;
+      want = (nd.lo + nd.hi) >> 1;
+      stage = kNode;
+      return;
+    }
+    emit(j * step, s0.e);
+    sm2 = sm1;
+    sm1 = s0;
+    s0 = sp1;
+    if (j + 2 <= n) {
+      want = (j + 2) * step;
+      stage = kSlide;
+      return;
+    }
+    next();
+  }
+
+  __device__ __forceinline__ void next() {
+    if (++j > n) {
+      stage = kDone;
+      return;
+    }
+    interval();
+  }
+
+  __device__ __forceinline__ void feed(const Sample& v) {
+    switch (stage) {
+      case kCarried:
+        sm1 = v;
+        want = step;
+        stage = kFirst;
+        return;
+      case kFirst:
+        s0 = v;
+        if (n >= 2) {
+          want = 2 * step;
+          stage = kSecond;
+          return;
+        }
+        j = 1;
+        interval();
+        return;
+      case kSecond:
+        sp1 = v;
+        j = 1;
+        interval();
+        return;
+      case kNode: {
+        const Node nd = cur;
+        const uint32_t g = want;
+        const bool s1 = r.signature && nd.s_lo != v.sig, s2 = r.signature && v.sig != nd.s_hi;
+        const bool bad = mid && midpoint_eta(r.dark, nd.e_lo, v.e, nd.e_hi) > r.tol;
+        const bool p1 = (s1 || bad) && g - nd.lo >= r.min_split;
+        const bool p2 = (s2 || bad) && nd.hi - g >= r.min_split;
+        const bool deeper = nd.depth < r.max_levels;
+        if (p2 && deeper) stack[sp++] = Node{g, nd.hi, v.e, nd.e_hi, v.sig, nd.s_hi, nd.depth + 1};
+        stack[sp++] = Node{g, 0u, v.e, 0.0f, 0u, 0u, -1};
+        if (p1 && deeper) stack[sp++] = Node{nd.lo, g, nd.e_lo, v.e, nd.s_lo, v.sig, nd.depth + 1};
+        pop();
+        return;
+      }
+      case kSlide:
+        sp1 = v;
+        next();
+        return;
+      default:
+        return;
+    }
+  }
+};
+
 // The one-pass form: every pixel's count, its first `cap` samples captured into scratch slots
 // [p cap, (p + 1) cap), and the window's largest count (atomicMax of the waves' maxima) in *max_count.
+// The adaptive controller runs as PixelSampler machines, one scene evaluation per lane per turn.
 __global__ __launch_bounds__(256) void capture_kernel(SceneDev sc, ViewDev view, RulesDev r, int64_t t_w, uint32_t cap,
                                                       int32_t* counts, CapturedSample* scratch,
                                                       unsigned int* max_count) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t c = 0;
-  if (p < view.width * view.height) {
-    c = sample_pixel<Emit::kCapture>(sc, view, r, t_w, p, nullptr, nullptr,
-                                     scratch + static_cast<size_t>(p) * cap, cap);
-    counts[p] = static_cast<int32_t>(c);
+  if (r.fixed_period > 0) {
+    if (p < view.width * view.height) {
+      c = sample_pixel<Emit::kCapture>(sc, view, r, t_w, p, nullptr, nullptr,
+                                       scratch + static_cast<size_t>(p) * cap, cap);
+      counts[p] = static_cast<int32_t>(c);
+    }
+  } else {
+    const bool live = p < view.width * view.height;
+    const double x = live ? view.ox + (static_cast<double>(p % view.width) + 0.5) * view.scale : 0.0;
+    const double y = live ? view.oy + (static_cast<double>(p / view.width) + 0.5) * view.scale : 0.0;
+    const double tw_s = static_cast<double>(t_w) * 1e-9;
+    uint64_t mask[kMaskWords];
+    for (int w = 0; w < kMaskWords; ++w) mask[w] = 0;
+    PixelSampler::Node stack[16];
+    PixelSampler m;
+    m.stack = stack;
+    m.r = r;
+    m.mid = r.tol > 0.0f;
+    m.out = scratch + static_cast<size_t>(live ? p : 0) * cap;
+    m.cap = cap;
+    m.count = 0;
+    m.stage = PixelSampler::kDone;
+    if (live) {
+      const float v_hat = plan_pixel(sc, x, y, tw_s, tw_s + kWindowNs * 1e-9, mask);
+      if (v_hat > 0.0f) {  // a pixel no shape reaches is static: no samples
+        const double lv = ceil(log2(static_cast<double>(v_hat) / view.scale * (kWindowNs * 1e-9) / r.d_px));
+        const int level = lv != lv ? r.l_min : static_cast<int>(fmin(fmax(lv, static_cast<double>(r.l_min)),
+                                                                     static_cast<double>(r.l_max)));
+        m.n = 1u << level;
+        m.step = 1u << (kWindowLog2 - level);
+        m.sm2 = Sample{0.0f, 0u};
+        m.sp1 = Sample{0.0f, 0u};
+        m.want = 0;
+        m.stage = PixelSampler::kCarried;
+      }
+    }
+    while (__ballot(m.stage != PixelSampler::kDone) != 0) {
+      if (m.stage != PixelSampler::kDone)
+        m.feed(eval_scene(sc, mask, x, y, static_cast<double>(t_w + m.want) * 1e-9));
+    }
+    if (live) {
+      c = m.count;
+      counts[p] = static_cast<int32_t>(c);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) c = max(c, static_cast<uint32_t>(__shfl_xor(static_cast<int>(c), off)));
   if ((threadIdx.x & 63) == 0 && c > 0) atomicMax(max_count, c);
