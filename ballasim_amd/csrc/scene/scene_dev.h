@@ -79,10 +79,14 @@ struct Sample {
   uint32_t sig;
 };
 
-// The scene at scene point (x, y) and time t (s), over the candidate shapes of the mask.
+// The scene at scene point (x, y) and time t (s), over the candidate shapes of the mask: the topmost
+// shape that holds the point, found first, and then its texture, so that the lanes of a wave that find
+// their shapes at different candidates evaluate the texture together.
 __device__ __forceinline__ Sample eval_scene(const SceneDev& sc, const uint64_t* mask, double x, double y,
                                              double t) {
-  for (int w = kMaskWords - 1; w >= 0; --w) {
+  int hit = -1;
+  float hu = 0.0f, hv = 0.0f;
+  for (int w = kMaskWords - 1; w >= 0 && hit < 0; --w) {
     uint64_t m = mask[w];
     while (m != 0) {
       const int b = 63 - __clzll(static_cast<long long>(m));
@@ -100,11 +104,16 @@ __device__ __forceinline__ Sample eval_scene(const SceneDev& sc, const uint64_t*
       const bool inside = s.kind == BSX_SHAPE_DISK ? (u * u + v * v <= s.hx * s.hx)
                                                    : (fabsf(u) <= s.hx && fabsf(v) <= s.hy);
       if (!inside) continue;
-      const float n = value_noise(u + s.hx, v + s.hy, s.seed, s.octaves, s.inv_cell);
-      return Sample{s.lux * (1.0f + s.contrast * (2.0f * n - 1.0f)), static_cast<uint32_t>(i + 1)};
+      hit = i;
+      hu = u;
+      hv = v;
+      break;
     }
   }
-  return Sample{sc.background, 0u};
+  if (hit < 0) return Sample{sc.background, 0u};
+  const DevShape& s = sc.shapes[hit];
+  const float n = value_noise(hu + s.hx, hv + s.hy, s.seed, s.octaves, s.inv_cell);
+  return Sample{s.lux * (1.0f + s.contrast * (2.0f * n - 1.0f)), static_cast<uint32_t>(hit + 1)};
 }
 
 // The shapes whose bounding circle, swept over [t0, t1] (s), passes within a pixel of (x, y), and
