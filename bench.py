@@ -330,6 +330,8 @@ def main():
                          "proposer's training and the learned controller against the plain one; stress: the "
                          "2048x1536 sensor for 1 s with the event stream")
     ap.add_argument("--chunk-mib", type=int, default=256, help="stress: event stream chunk size (MiB)")
+    ap.add_argument("--serial-sampling", dest="overlap_sampling", action="store_false",
+                    help="stress: sample on the advance's stream (no overlap of sampling with the advance)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shape (no autotuner trials before the timed windows)")
     ap.add_argument("--train-examples", type=int, default=20_000_000, help="learned: training examples")
@@ -426,19 +428,42 @@ def bench_stress(args):
                 seen["last_t"] = int(t[-1])
 
     def run(n_windows, stream_chunks):
+        import queue
+
+        import torch
         m = SensorModel(p, W, H, 1, ["constant"], 0, device="cuda")
         m.init(0, frame)
         rig = Rig([m])
         stage = OutputStage(m)
         es = EventStream(sink, chunk_bytes=args.chunk_mib << 20) if stream_chunks else None
         samples = events = released = 0
-        t_sample = 0.0
+        busy = [0.0]
         k0 = rig.stats()["kernel_ms"]
+        # The sampler runs one window ahead on a stream of its own, in a thread of its own (the library's
+        # calls release the interpreter), so that a window's sampling overlaps the previous window's advance
+        # and output stage; three slots hold the window being advanced, the next one and the one after.
+        sampler_stream = torch.cuda.Stream()
+        ready = queue.Queue(maxsize=1)
+
+        def produce():
+            try:
+                for w in range(n_windows):
+                    ts = time.perf_counter()
+                    item = scene.sample_window(view, rules, w * WINDOW, slot=w % 3,
+                                               stream=sampler_stream if args.overlap_sampling else 0)
+                    busy[0] += time.perf_counter() - ts
+                    ready.put(item)
+            except BaseException as e:  # handed to the consumer, which raises it
+                ready.put(e)
+
+        producer = threading.Thread(target=produce, daemon=True)
         t0 = time.perf_counter()
+        producer.start()
         for w in range(n_windows):
-            ts = time.perf_counter()
-            runs, n_s = scene.sample_window(view, rules, w * WINDOW, slot=0)
-            t_sample += time.perf_counter() - ts
+            item = ready.get()
+            if isinstance(item, BaseException):
+                raise item
+            runs, n_s = item
             (ptr, n_ev), = rig.advance_device([runs], (w + 1) * WINDOW)
             samples += n_s
             events += n_ev
@@ -447,10 +472,12 @@ def bench_stress(args):
             if (w + 1) % 10 == 0:
                 print(f"# stress window {w + 1}/{n_windows}: {samples} samples, {events} events", file=sys.stderr,
                       flush=True)
+        producer.join()
         if es is not None:
             released += es.flush(stage)
             es.finish()
         elapsed = time.perf_counter() - t0
+        t_sample = busy[0]
         kernel_ms = rig.stats()["kernel_ms"] - k0
         st = es.stats() if es is not None else None
         if es is not None:
@@ -476,7 +503,9 @@ def bench_stress(args):
                                "down to 0.25-0.5 px), %d windows of 2^23 ns (%.1f ms), plain controller tol 0.01, "
                                "instantaneous response, shot 0.5 Hz/px, leak 0.1 Hz/px, events streamed in %d MiB "
                                "double-buffered pinned chunks; a step = sampling + advance + output + stream of one "
-                               "window" % (n, n * WINDOW / 1e6, args.chunk_mib),
+                               "window%s" % (n, n * WINDOW / 1e6, args.chunk_mib,
+                                             ", the next window sampled on its own stream meanwhile"
+                                             if args.overlap_sampling else ""),
                    "pixels": P, "samples": samples, "events": events, "windows": n},
         "events_per_s": events / elapsed,
         "samples_per_event": samples / max(events, 1),
