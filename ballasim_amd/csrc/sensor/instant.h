@@ -663,31 +663,57 @@ struct InstantLane {
   // the block's last sample as the next segment's start. The bounds, when formed here, are formed from
   // max(t_search, t_cur), as the block's first piece would form them; bounds formed earlier are valid
   // for later times too (dl does not decrease).
-  BSX_HD BSX_INLINE bool try_block_skip() {
-    if (qb == nullptr || !screen || blind_until != kNotBlind || n3_ref != 0.0f || !(leak >= 0.0f)) return false;
+  // Blocks are tried kSkipAhead at a time: their summaries, last sample times and last values are
+  // loaded together, and then passed in order while each passes, so that a lane crossing quiet blocks
+  // waits on one round of loads per kSkipAhead blocks rather than on two dependent loads per block.
+  // Each block is decided as alone: the checks of block j + 1 see the state block j left.
+  static constexpr int kSkipAhead = 4;
+
+  // Returns the number of blocks passed (0: none).
+  BSX_HD BSX_INLINE int try_block_skip() {
+    if (qb == nullptr || !screen || blind_until != kNotBlind || n3_ref != 0.0f || !(leak >= 0.0f)) return 0;
     const int64_t q = q_base + next_i;
-    if ((q & (kScreenBlock - 1)) != 0 || next_i + kScreenBlock > n) return false;
-    const int last = next_i + kScreenBlock - 1;
-    const int64_t t_last = ts[last];
-    if (!(shot_next > t_last) || ((t_cur >> kPlainSegmentLog2) + 1) * kPlainSegmentNs < t_last) return false;
-    const ScreenBlock b = qb[q >> kScreenBlockLog2];
-    const int64_t from = vmax(t_search, t_cur);
-    if (!scr_valid) screen_bounds(from, t_end_ - 1);
-    // a steady block (the light b.lo at every time of it, the current start value's) is decided
-    // exactly by its two ends; a block wholly in dead time fires nothing
-    if (!in_screen(b.lo, b.hi) && !(b.steady != 0u && (from >= t_last || steady_quiet(b.lo, from, t_last - 1))))
-      return false;
-    n_substeps += kScreenBlock;
-    n_pieces += kScreenBlock;
-    ++n_block_skips;
-    t_search = vmax(t_search, t_last);
-    ta = t_last;
-    for (int g = 0; g < kMaxG; ++g)
-      if (g < G) va[g] = clampv(irr[static_cast<int64_t>(last) * G + g]);
-    t_cur = t_last;
-    next_i += kScreenBlock;
-    for (int k = 0; k < kAhead; ++k) fetch(k, next_i + k);
-    return true;
+    if ((q & (kScreenBlock - 1)) != 0 || next_i + kScreenBlock > n) return 0;
+    int64_t t_lasts[kSkipAhead];
+    ScreenBlock blocks[kSkipAhead];
+    float v_lasts[kSkipAhead][kMaxG];
+    int m = 0;  // blocks inside the run
+    for (int j = 0; j < kSkipAhead; ++j) {
+      if (next_i + (j + 1) * kScreenBlock <= n) {
+        const int last = next_i + (j + 1) * kScreenBlock - 1;
+        t_lasts[j] = ts[last];
+        blocks[j] = qb[(q >> kScreenBlockLog2) + j];
+        for (int g = 0; g < kMaxG; ++g)
+          if (g < G) v_lasts[j][g] = irr[static_cast<int64_t>(last) * G + g];
+        m = j + 1;
+      }
+    }
+    int passed = 0;
+    for (int j = 0; j < kSkipAhead; ++j) {
+      if (j >= m || (j > 0 && !(t_cur < t_end_))) break;
+      const int64_t t_last = t_lasts[j];
+      if (!(shot_next > t_last) || ((t_cur >> kPlainSegmentLog2) + 1) * kPlainSegmentNs < t_last) break;
+      const ScreenBlock b = blocks[j];
+      const int64_t from = vmax(t_search, t_cur);
+      if (!scr_valid) screen_bounds(from, t_end_ - 1);
+      // a steady block (the light b.lo at every time of it, the current start value's) is decided
+      // exactly by its two ends; a block wholly in dead time fires nothing
+      if (!in_screen(b.lo, b.hi) && !(b.steady != 0u && (from >= t_last || steady_quiet(b.lo, from, t_last - 1))))
+        break;
+      n_substeps += kScreenBlock;
+      n_pieces += kScreenBlock;
+      ++n_block_skips;
+      t_search = vmax(t_search, t_last);
+      ta = t_last;
+      for (int g = 0; g < kMaxG; ++g)
+        if (g < G) va[g] = clampv(v_lasts[j][g]);
+      t_cur = t_last;
+      next_i += kScreenBlock;
+      ++passed;
+    }
+    if (passed > 0)
+      for (int k = 0; k < kAhead; ++k) fetch(k, next_i + k);
+    return passed;
   }
 
   // first_loud: the index of the first input sample ending a segment that phase A did not find quiet
@@ -721,8 +747,9 @@ struct InstantLane {
     while (true) {
       if (!seg_open) {
         if (!(t_cur < t_end_)) return kDone;
-        if (try_block_skip()) {
-          if (--budget <= 0) return kWalk;
+        if (const int passed = try_block_skip()) {
+          budget -= passed;
+          if (budget <= 0) return kWalk;
           continue;
         }
         open_next();
