@@ -20,6 +20,7 @@ __device__ __forceinline__ uint8_t label_s1(uint8_t c) {
 }
 
 constexpr uint64_t kBias = uint64_t{1} << 63;
+constexpr int kOutBlock = 256;  // threads per block of the per-event kernels
 
 __global__ void build_kernel(const RawEventDev* raw, size_t n, uint32_t width, Readout r, int crop, CropBox box,
                              OutEventDev* out, unsigned long long* scratch) {
@@ -38,25 +39,40 @@ __global__ void build_kernel(const RawEventDev* raw, size_t n, uint32_t width, R
     o.p = e.polarity;
     o.label = label_s1(e.label);
   }
-  // wave-aggregated slots (the order of the kept events does not matter: the output key is unique)
+  // block-aggregated slots, one atomic per block (the order of the kept events does not matter: the
+  // output key is unique)
+  __shared__ unsigned int wave_kept[kOutBlock / 64];
+  __shared__ unsigned long long block_base;
   const unsigned long long mask = __ballot(keep);
-  const int lane = __lane_id();
-  unsigned long long base = 0;
-  if (mask != 0) {
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    if (lane == leader) base = atomicAdd(scratch, static_cast<unsigned long long>(__popcll(mask)));
-    base = __shfl(base, leader);
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  if (lane == 0) wave_kept[wave] = static_cast<unsigned int>(__popcll(mask));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long total = 0;
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) total += wave_kept[w];
+    block_base = total != 0 ? atomicAdd(scratch, total) : 0;
   }
+  __syncthreads();
+  unsigned long long base = block_base;
+  for (int w = 0; w < wave; ++w) base += wave_kept[w];
   if (keep) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = o;
 }
 
+// The extremes of t_cross and t_reported (biased to unsigned order) over ev[0 .. n_held + scratch[0]):
+// a grid-stride loop over a bounded grid, reduced in the wave and then in the block, so that each block
+// makes one set of four atomics.
 __global__ void minmax_kernel(const OutEventDev* ev, size_t n_held, size_t n_max, unsigned long long* scratch) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t n = n_held + static_cast<size_t>(scratch[0]);
+  __shared__ unsigned long long part[4][kOutBlock / 64];
+  const size_t n = vmin(n_held + static_cast<size_t>(scratch[0]), n_max);
   unsigned long long a = ~0ull, b = 0ull, c = ~0ull, d = 0ull;
-  if (i < n && i < n_max) {
-    a = b = static_cast<uint64_t>(ev[i].t_cross) ^ kBias;
-    c = d = static_cast<uint64_t>(ev[i].t_reported) ^ kBias;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const unsigned long long tc = static_cast<uint64_t>(ev[i].t_cross) ^ kBias;
+    const unsigned long long tr = static_cast<uint64_t>(ev[i].t_reported) ^ kBias;
+    a = min(a, tc);
+    b = max(b, tc);
+    c = min(c, tr);
+    d = max(d, tr);
   }
   for (int off = 32; off > 0; off >>= 1) {
     a = min(a, static_cast<unsigned long long>(__shfl_xor(a, off)));
@@ -64,11 +80,27 @@ __global__ void minmax_kernel(const OutEventDev* ev, size_t n_held, size_t n_max
     c = min(c, static_cast<unsigned long long>(__shfl_xor(c, off)));
     d = max(d, static_cast<unsigned long long>(__shfl_xor(d, off)));
   }
-  if (__lane_id() == 0 && a != ~0ull) {
-    atomicMin(scratch + 1, a);
-    atomicMax(scratch + 2, b);
-    atomicMin(scratch + 3, c);
-    atomicMax(scratch + 4, d);
+  const int wave = threadIdx.x >> 6;
+  if (__lane_id() == 0) {
+    part[0][wave] = a;
+    part[1][wave] = b;
+    part[2][wave] = c;
+    part[3][wave] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+      a = min(a, part[0][w]);
+      b = max(b, part[1][w]);
+      c = min(c, part[2][w]);
+      d = max(d, part[3][w]);
+    }
+    if (a != ~0ull) {
+      atomicMin(scratch + 1, a);
+      atomicMax(scratch + 2, b);
+      atomicMin(scratch + 3, c);
+      atomicMax(scratch + 4, d);
+    }
   }
 }
 
@@ -94,8 +126,15 @@ __global__ void gather_kernel(const OutEventDev* ev, const uint32_t* perm, size_
     sorted[i] = o;
     ready = o.t_reported < cut;
   }
+  __shared__ unsigned int wave_ready[kOutBlock / 64];
   const unsigned long long m = __ballot(ready);
-  if (__lane_id() == 0 && m != 0) atomicAdd(scratch + 5, static_cast<unsigned long long>(__popcll(m)));
+  if (__lane_id() == 0) wave_ready[threadIdx.x >> 6] = static_cast<unsigned int>(__popcll(m));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long total = 0;
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) total += wave_ready[w];
+    if (total != 0) atomicAdd(scratch + 5, total);
+  }
 }
 
 __global__ void columns_kernel(const OutEventDev* ev, size_t n, int64_t t_offset_us, uint16_t* x, uint16_t* y,
@@ -195,21 +234,21 @@ __global__ __launch_bounds__(256) void digit_scatter_kernel(const uint64_t* keys
   }
 }
 
-inline unsigned grid(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
+inline unsigned grid(size_t n) { return static_cast<unsigned>((n + kOutBlock - 1) / kOutBlock); }
 
 }  // namespace
 
 hipError_t launch_out_build(const RawEventDev* raw, size_t n, uint32_t width, const Readout& r, bool crop,
                             const CropBox& box, OutEventDev* out, unsigned long long* scratch, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  build_kernel<<<grid(n), 256, 0, stream>>>(raw, n, width, r, crop ? 1 : 0, box, out, scratch);
+  build_kernel<<<grid(n), kOutBlock, 0, stream>>>(raw, n, width, r, crop ? 1 : 0, box, out, scratch);
   return hipGetLastError();
 }
 
 hipError_t launch_out_minmax(const OutEventDev* ev, size_t n_held, size_t n_max, unsigned long long* scratch,
                              hipStream_t stream) {
   if (n_max == 0) return hipSuccess;
-  minmax_kernel<<<grid(n_max), 256, 0, stream>>>(ev, n_held, n_max, scratch);
+  minmax_kernel<<<vmin(grid(n_max), 1024u), kOutBlock, 0, stream>>>(ev, n_held, n_max, scratch);
   return hipGetLastError();
 }
 
@@ -255,7 +294,7 @@ hipError_t launch_out_sort(uint64_t* keys, uint64_t* keys_out, uint32_t* perm, u
 hipError_t launch_out_gather(const OutEventDev* ev, const uint32_t* perm, size_t n, int64_t cut, OutEventDev* sorted,
                              unsigned long long* scratch, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  gather_kernel<<<grid(n), 256, 0, stream>>>(ev, perm, n, cut, sorted, scratch);
+  gather_kernel<<<grid(n), kOutBlock, 0, stream>>>(ev, perm, n, cut, sorted, scratch);
   return hipGetLastError();
 }
 
