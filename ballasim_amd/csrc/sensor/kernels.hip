@@ -351,14 +351,16 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
   bool blk_ok = true, blk_steady = true;
   int blk_run = -1;
   float blk_lo = INFINITY, blk_hi = -INFINITY;
-  for (int k = 0; k < kPrepPerThread; ++k) {
-    const int64_t q = qt + k;
-    if (!(q < total)) {
-      blk_ok = false;
-      break;
-    }
+  // The thread's positions, a run at a time: the run's lookup, bookkeeping and constants once per run,
+  // then its positions in a plain loop.
+  const int64_t q_end = vmin(qt + kPrepPerThread, total);
+  if (qt + kPrepPerThread > total) blk_ok = false;
+  for (int64_t q = qt; q < q_end;) {
     advance_run(rr, br, pos, runs, pixel, n_runs, q);
     const int run = static_cast<int>(rr.i);
+    const int64_t k1 = rr.i + 1 - br.first;
+    const int64_t next_pos = rr.i + 1 < n_runs ? (k1 < br.count ? br.pos[k1] : pos[rr.i + 1]) : total;
+    const int64_t q_stop = vmin(q_end, next_pos);
     blk_ok &= blk_run < 0 || run == blk_run;
     blk_run = run;
     if (run != acc_run) {
@@ -390,57 +392,59 @@ __global__ __launch_bounds__(256) void chain_screen_kernel(const CameraDev* cams
       usable = L.blind_until == kNotBlind && L.leak >= 0.0f && L.n3_ref == 0.0f;
       if (usable) L.screen_bounds(t_state, t_end - 1);
     }
-    const int64_t j = q - rr.pos;
     const int64_t* t = runs.t + rr.row_start;
     const float* v = runs.irr + rr.row_start * kc.groups;
-    if (j == 0) {
-      if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
+    for (; q < q_stop; ++q) {
+      const int64_t j = q - rr.pos;
+      if (j == 0) {
+        if (t[0] == t_state) {  // the first sample is the model time's: no segment ends at it
+          for (int g = 0; g < kMaxG; ++g)
+            if (g < kc.groups) tally.add(v[g], kc.irr_max);
+          prev_end = -1;
+          blk_ok = false;
+          continue;
+        }
+        L.ta = t_state;
         for (int g = 0; g < kMaxG; ++g)
-          if (g < kc.groups) tally.add(v[g], kc.irr_max);
-        prev_end = -1;
-        blk_ok = false;
-        continue;
+          if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
+      } else if (prev_end == q - 1) {  // the previous position's segment ends where this one starts
+        L.ta = L.tb;
+        for (int g = 0; g < kMaxG; ++g)
+          if (g < kc.groups) L.va[g] = L.vb[g];
+      } else {
+        L.ta = t[j - 1];
+        for (int g = 0; g < kMaxG; ++g)
+          if (g < kc.groups) L.va[g] = L.clampv(v[(j - 1) * kc.groups + g]);
       }
-      L.ta = t_state;
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < kc.groups) L.va[g] = in.held[static_cast<int64_t>(g) * kc.n_pixels + p];
-    } else if (prev_end == q - 1) {  // the previous position's segment ends where this one starts
-      L.ta = L.tb;
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < kc.groups) L.va[g] = L.vb[g];
-    } else {
-      L.ta = t[j - 1];
-      for (int g = 0; g < kMaxG; ++g)
-        if (g < kc.groups) L.va[g] = L.clampv(v[(j - 1) * kc.groups + g]);
-    }
-    L.tb = t[j];
-    for (int g = 0; g < kMaxG; ++g) {
-      if (g < kc.groups) {
-        const float x = v[j * kc.groups + g];
-        tally.add(x, kc.irr_max);
-        L.vb[g] = L.clampv(x);
-      }
-    }
-    prev_end = q;
-    float seg_lo, seg_hi;
-    L.segment_range(seg_lo, seg_hi);
-    blk_lo = fminf(blk_lo, seg_lo);
-    blk_hi = fmaxf(blk_hi, seg_hi);
-    blk_steady &= L.steady_segment();
-    const bool quiet = usable & (L.shot_next >= L.tb) &
-                       (((L.ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= L.tb) & L.in_screen(seg_lo, seg_hi);
-    if (!quiet) {
-      loud_acc = vmin(loud_acc, static_cast<int>(j));
-      float Ia = L.dark, Ib = L.dark;
+      L.tb = t[j];
       for (int g = 0; g < kMaxG; ++g) {
         if (g < kc.groups) {
-          Ia += L.scale * L.va[g];
-          Ib += L.scale * L.vb[g];
+          const float x = v[j * kc.groups + g];
+          tally.add(x, kc.irr_max);
+          L.vb[g] = L.clampv(x);
         }
       }
-      // (a schedule only: hardware logarithm and reciprocal)
-      cost_acc += 1.0f + kSearchWeight * fabs_(__logf(Ib) - __logf(Ia)) *
-                             __builtin_amdgcn_rcpf(vmin(L.theta_on, L.theta_off));
+      prev_end = q;
+      float seg_lo, seg_hi;
+      L.segment_range(seg_lo, seg_hi);
+      blk_lo = fminf(blk_lo, seg_lo);
+      blk_hi = fmaxf(blk_hi, seg_hi);
+      blk_steady &= L.steady_segment();
+      const bool quiet = usable & (L.shot_next >= L.tb) &
+                         (((L.ta >> kPlainSegmentLog2) + 1) * kPlainSegmentNs >= L.tb) & L.in_screen(seg_lo, seg_hi);
+      if (!quiet) {
+        loud_acc = vmin(loud_acc, static_cast<int>(j));
+        float Ia = L.dark, Ib = L.dark;
+        for (int g = 0; g < kMaxG; ++g) {
+          if (g < kc.groups) {
+            Ia += L.scale * L.va[g];
+            Ib += L.scale * L.vb[g];
+          }
+        }
+        // (a schedule only: hardware logarithm and reciprocal)
+        cost_acc += 1.0f + kSearchWeight * fabs_(__logf(Ib) - __logf(Ia)) *
+                               __builtin_amdgcn_rcpf(vmin(L.theta_on, L.theta_off));
+      }
     }
   }
   wave_segmented_cost_loud(work, first_loud, acc_run, cost_acc, loud_acc);
