@@ -8,7 +8,8 @@ the work counters, window by window.
 - config 1: the 346x260 textured disk through the cascade at the 200-Hz mapping, all twelve windows
   of the 100-ms run;
 - config 3: one window of the four rig cameras at their real shapes (3.39 M pixels, log-normal(ln 20,
-  1.5) sample counts) in one rig launch, each camera against its own oracle model.
+  1.5) sample counts) in one rig launch, each camera against its own oracle model;
+- config 5: the 2048x1536 stress sensor's first two windows (about 720 M samples each).
 The instantaneous path's leaf_intervals counter counts this backend's own search evaluations, so it
 is the one counter not compared there."""
 import os
@@ -138,3 +139,21 @@ def test_config3_rig_window_each_camera_equals_oracle():
         _compare_events(ev, want, rep, v.width, f"camera {c}")
         _compare_state(cams[c], refs[c], f"camera {c}", True)
     assert sum(totals) > 150_000_000
+
+
+def test_config5_stress_windows_equal_oracle():
+    """Config 5 at its own shape: the 2048x1536 sensor over 256 textured rectangles at up to 5000 px/s
+    (7 texture octaves), shot 0.5 Hz/px and leak 0.1 Hz/px, the plain controller's runs of the first two
+    windows (the heaviest of the run: about 720 M samples each), bit for bit against the oracle on the
+    host's cores: events, reported times, every pixel's state, the counters."""
+    from ballasim_amd import configs
+    from ballasim_amd import scene as S
+    shapes, bg = S.config2_shapes(0, n=256, width=2048, height=1536, v_max=5000.0, cell_range=(32.0, 64.0),
+                                  octaves=7)
+    scene, view = S.Scene(shapes, bg), S.view(2048, 1536)
+    p = configs.config2_params("instantaneous")
+    p.shot_rate_hz = 0.5
+    p.leak_rate_hz = 0.1
+    rules = S.rules(float(p.dark_current_fA) / float(p.illuminance_scale), tol=0.01)
+    samples, events, _ = _run_windows(scene, view, p, rules, 2)
+    assert samples > 10 ** 9 and events > 10 ** 6, (samples, events)
