@@ -321,12 +321,23 @@ def test_render_requests_equal_frames():
     assert (sig != 0).any() and (sig == 0).any()
 
 
+RULE_VARIANTS = {
+    "default": dict(tol=0.01, signature=True),
+    "no_signature": dict(tol=0.01, signature=False),
+    "no_midpoint": dict(tol=0.0, signature=True),
+    "no_refinement": dict(tol=0.01, signature=True, max_refine_levels=0),
+    "deep": dict(tol=0.003, signature=True, max_refine_levels=7, min_split_ns=1024),
+}
+
+
+@pytest.mark.parametrize("rules_name", sorted(RULE_VARIANTS))
 @pytest.mark.parametrize("scene", ["levels", "rectangles"])
-def test_one_pass_sampler_equals_two_passes(scene):
+def test_one_pass_sampler_equals_two_passes(scene, rules_name):
     """The one-pass sampler (each pixel's first k samples captured while it is counted, then copied into
     place; a pixel with more samples sampled again in place) gives the runs of the two-pass sampler (count,
     then write), with slots for every pixel's samples, with 4 slots (nearly every pixel sampled again) and
-    with a budget below one slot per pixel (two passes)."""
+    with a budget below one slot per pixel (two passes); under the default rules and with signature flags, the
+    midpoint test or refinement off, and with seven levels of refinement down to 1024-ns intervals."""
     from ballasim_amd import scene as S
     if scene == "levels":
         sc, view, _, _ = _level_scene()
@@ -334,7 +345,7 @@ def test_one_pass_sampler_equals_two_passes(scene):
         shapes, bg = S.config2_shapes(5, n=16, width=320, height=180)
         sc, view = S.Scene(shapes, bg), S.view(320, 180)
     P = view.width * view.height
-    rules = S.rules(0.0021, tol=0.01, signature=True)
+    rules = S.rules(0.0021, **RULE_VARIANTS[rules_name])
     budgets = {"two passes": 0, "4 slots": 4 * 8 * P, "default": None}
     for w in range(3):
         got = {}

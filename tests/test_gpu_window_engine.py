@@ -20,9 +20,9 @@ needs_ref = pytest.mark.skipif(not window_ref.available(),
                                reason="oracle/_ref/libwindow_ref.so was not built (the reference is absent)")
 
 
-def _compare(sc, view, shapes, windows, tol, dark, signature, min_refined):
+def _compare(sc, view, shapes, windows, tol, dark, signature, min_refined, max_levels=4, min_split=2048):
     from ballasim_amd import scene as S
-    rules = S.rules(dark, tol=tol, signature=signature)
+    rules = S.rules(dark, tol=tol, signature=signature, max_refine_levels=max_levels, min_split_ns=min_split)
     total_refined = total_pixels = 0
     for w in windows:
         t_w = w * WINDOW
@@ -30,6 +30,7 @@ def _compare(sc, view, shapes, windows, tol, dark, signature, min_refined):
         level = _levels(shapes, view, t_w)
         active = np.nonzero(level >= 0)[0].astype(np.uint32)
         offsets, t, irr, stats = window_ref.sample_window(sc, view, t_w, active, level[active], tol, dark,
+                                                          max_refine_levels=max_levels, min_split=min_split,
                                                           signature=signature)
         runs, total = sc.sample_window(view, rules, t_w, slot=0)
         ids, tt, ii, offs = S.runs_to_host(runs)
@@ -66,3 +67,11 @@ def test_runs_equal_reference_window_engine_textured_rectangles():
     shapes, bg = S.config2_shapes(5, n=16, width=320, height=180)
     sc, view = S.Scene(shapes, bg), S.view(320, 180)
     _compare(sc, view, shapes, range(2), 0.01, 0.0021, True, min_refined=200)
+
+
+@needs_ref
+def test_runs_equal_reference_window_engine_deep_refinement():
+    """Seven refinement levels down to 1024-ns intervals at a tighter tolerance, and the midpoint test off."""
+    sc, view, shapes, _ = _level_scene()
+    _compare(sc, view, shapes, range(2), 0.003, 0.0021, True, min_refined=50, max_levels=7, min_split=1024)
+    _compare(sc, view, shapes, range(1), 0.0, 0.0021, True, min_refined=0)
