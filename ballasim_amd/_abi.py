@@ -90,7 +90,8 @@ class Stats(C.Structure):
 
 class LaunchPolicy(C.Structure):
     _fields_ = [("event_capacity", C.c_uint64), ("block_size", C.c_int32), ("walk_budget", C.c_int32),
-                ("search_batch", C.c_int32), ("queue_chunk", C.c_int32)]
+                ("search_batch", C.c_int32), ("queue_chunk", C.c_int32), ("head_lanes", C.c_int32),
+                ("head_per_mille", C.c_int32)]
 
 
 class RigStats(C.Structure):

@@ -72,6 +72,8 @@ bsx_launch_policy policy_of(const bsx::WalkTuning& t) {
   p.walk_budget = t.walk_budget;
   p.search_batch = t.search_batch;
   p.queue_chunk = t.chunk;
+  p.head_lanes = t.head_lanes;
+  p.head_per_mille = t.head_per_mille;
   return p;
 }
 
@@ -449,6 +451,8 @@ int bsx_rig_set_tuning(bsx_rig* rig, const bsx_launch_policy* tuning) {
     if (tuning->walk_budget != 0) t.walk_budget = tuning->walk_budget;
     if (tuning->search_batch != 0) t.search_batch = tuning->search_batch;
     if (tuning->queue_chunk != 0) t.chunk = tuning->queue_chunk < 0 ? 0 : tuning->queue_chunk;
+    if (tuning->head_lanes != 0) t.head_lanes = tuning->head_lanes;
+    if (tuning->head_per_mille != 0) t.head_per_mille = tuning->head_per_mille < 0 ? 0 : tuning->head_per_mille;
     ch.set_tuning(t);
   });
 }

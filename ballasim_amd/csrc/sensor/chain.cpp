@@ -40,6 +40,10 @@ void LaunchChain::set_tuning(const WalkTuning& tu) {
   if (tu.walk_budget < 1 || tu.walk_budget > 1024) throw std::invalid_argument("launch tuning: walk_budget in [1, 1024]");
   if (tu.search_batch < 1 || tu.search_batch > 64) throw std::invalid_argument("launch tuning: search_batch in [1, 64]");
   if (tu.chunk < 0 || tu.chunk > 64) throw std::invalid_argument("launch tuning: chunk in [0, 64]");
+  if (tu.head_lanes < 1 || tu.head_lanes > 64) throw std::invalid_argument("launch tuning: head_lanes in [1, 64]");
+  if (tu.head_per_mille < 0 || tu.head_per_mille > 1000) {
+    throw std::invalid_argument("launch tuning: head_per_mille in [0, 1000]");
+  }
   if (std::memcmp(&tu, &tu_, sizeof tu) != 0) graph_valid_ = false;
   tu_ = tu;
 }
@@ -65,6 +69,10 @@ ChainArgs LaunchChain::args(const WalkTuning& tu) const {
   a.perm = d_perm_.as<uint32_t>();
   a.queue = d_queue_.as<unsigned int>();
   a.tu = tu;
+  // The head's groups: its share of the order in groups of head_lanes, never past the pixel space.
+  const uint64_t lanes = static_cast<uint64_t>(tu.head_lanes);
+  const uint64_t head = static_cast<uint64_t>(n_total_) * static_cast<uint64_t>(tu.head_per_mille) / 1000;
+  a.tu.head_groups = static_cast<int32_t>(std::min<uint64_t>((head + lanes - 1) / lanes, n_total_ / lanes));
   return a;
 }
 

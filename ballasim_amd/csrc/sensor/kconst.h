@@ -156,13 +156,18 @@ struct CallDev {
 };
 
 // The walk's launch shape, chosen by the autotuner: threads per block, segments a walking lane does
-// before the wave looks again, waiting lanes that start a joint search, 64-pixel groups a wave takes
-// per grab of the persistent queue (0: one group per wave, no queue).
+// before the wave looks again, waiting lanes that start a joint search, groups a wave takes per grab
+// of the persistent queue (0: one group per wave, no queue), and the head of the heavy-first order
+// walked in groups of fewer than 64 pixels.
 struct WalkTuning {
   int32_t block;
   int32_t walk_budget;
   int32_t search_batch;
   int32_t chunk;
+  int32_t head_lanes = 64;     // pixels a group takes at the head of the heavy-first order, [1, 64]
+  int32_t head_per_mille = 0;  // the head's share of the order's pixels, in thousandths
+  int32_t head_groups = 0;     // the head's groups, set per launch from the above (head_groups * head_lanes
+                               // <= the walk's pixels); 0 where the walk has no heavy-first order
 };
 
 struct RawEventDev {

@@ -755,18 +755,32 @@ __device__ __forceinline__ void instant_group(const CameraDev* cams, int n_cams,
   camera_add(cams, ci, 1 + kStLeaves, L.n_leaves);
 }
 
+// The walk's groups over n pixels of the heavy-first order: tu.head_groups groups of tu.head_lanes
+// pixels at the head (the heaviest pixels, spread over more waves so that fewer lanes share each
+// wave's turns), then groups of 64.
+BSX_HD inline uint32_t walk_group_count(uint32_t n, const WalkTuning& tu) {
+  const uint32_t head = static_cast<uint32_t>(tu.head_groups) * static_cast<uint32_t>(tu.head_lanes);
+  return static_cast<uint32_t>(tu.head_groups) + (n - head + 63) / 64;
+}
+
+// The order position lane `lane` of group `group` takes (past the pixel space for an idle head lane).
+__device__ __forceinline__ uint32_t walk_index(uint32_t group, uint32_t lane, const WalkTuning& tu) {
+  const uint32_t ng = static_cast<uint32_t>(tu.head_groups), s = static_cast<uint32_t>(tu.head_lanes);
+  if (group < ng) return lane < s ? group * s + lane : 0xffffffffu;
+  return ng * s + (group - ng) * 64 + lane;
+}
+
 // The walk over the pixels of n_cams cameras, in a launch's pixel order. Without a queue (tu.chunk =
-// 0) each wave takes one 64-pixel group; with one, waves grab tu.chunk groups at a time from the
-// queue counter until the pixel space is exhausted (a persistent grid: every wave exits once a grab
-// is empty).
+// 0) each wave takes one group; with one, waves grab tu.chunk groups at a time from the queue counter
+// until the pixel space is exhausted (a persistent grid: every wave exits once a grab is empty).
 template <class Group>
 __device__ __forceinline__ void walk_groups(uint32_t n_total, unsigned int* queue, const WalkTuning& tu,
                                             const Group& group_at) {
   const uint32_t lane = __lane_id();
-  const uint32_t n_groups = (n_total + 63) / 64;
+  const uint32_t n_groups = walk_group_count(n_total, tu);
   if (tu.chunk <= 0 || queue == nullptr) {
     const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    if (group < n_groups) group_at(group * 64 + lane);
+    if (group < n_groups) group_at(walk_index(group, lane, tu));
     return;
   }
   while (true) {
@@ -775,7 +789,7 @@ __device__ __forceinline__ void walk_groups(uint32_t n_total, unsigned int* queu
     g0 = __shfl(g0, 0);
     if (g0 >= n_groups) break;
     const uint32_t g1 = vmin(g0 + static_cast<uint32_t>(tu.chunk), n_groups);
-    for (uint32_t group = g0; group < g1; ++group) group_at(group * 64 + lane);
+    for (uint32_t group = g0; group < g1; ++group) group_at(walk_index(group, lane, tu));
   }
 }
 
@@ -1284,7 +1298,7 @@ hipError_t launch_chain_prepare(const ChainArgs& a, hipStream_t stream) {
 
 int chain_walk_blocks(const ChainArgs& a) {
   const int block = a.tu.block;
-  const unsigned full = static_cast<unsigned>((static_cast<uint64_t>(a.n_total) + block - 1) / block);
+  const unsigned full = static_cast<unsigned>((uint64_t{walk_group_count(a.n_total, a.tu)} * 64 + block - 1) / block);
   if (a.tu.chunk <= 0 || a.queue == nullptr) return static_cast<int>(full);
   const int kind = a.instant ? 1 : a.noise ? 2 : 0;
   return static_cast<int>(vmin(full, static_cast<unsigned>(resident_blocks(kind, group_tier(a.groups), block))));

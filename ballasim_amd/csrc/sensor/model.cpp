@@ -986,10 +986,16 @@ void SensorModel::set_policy(const bsx_launch_policy& p) {
   if (p.walk_budget < 0 || p.walk_budget > 1024) throw std::invalid_argument("set_launch_policy: walk_budget in [0, 1024]");
   if (p.search_batch < 0 || p.search_batch > 64) throw std::invalid_argument("set_launch_policy: search_batch in [0, 64]");
   if (p.queue_chunk < -1 || p.queue_chunk > 64) throw std::invalid_argument("set_launch_policy: queue_chunk in [-1, 64]");
+  if (p.head_lanes < 0 || p.head_lanes > 64) throw std::invalid_argument("set_launch_policy: head_lanes in [0, 64]");
+  if (p.head_per_mille < -1 || p.head_per_mille > 1000) {
+    throw std::invalid_argument("set_launch_policy: head_per_mille in [-1, 1000]");
+  }
   if (p.block_size != 0) tu.block = p.block_size;
   if (p.walk_budget != 0) tu.walk_budget = p.walk_budget;
   if (p.search_batch != 0) tu.search_batch = p.search_batch;
   if (p.queue_chunk != 0) tu.chunk = p.queue_chunk < 0 ? 0 : p.queue_chunk;
+  if (p.head_lanes != 0) tu.head_lanes = p.head_lanes;
+  if (p.head_per_mille != 0) tu.head_per_mille = p.head_per_mille < 0 ? 0 : p.head_per_mille;
   chain_->set_tuning(tu);
   if (p.event_capacity != 0) cap_ = p.event_capacity;
 }

@@ -13,7 +13,8 @@ namespace {
 constexpr int kK = 1 + kStCount;  // counter words per camera
 
 bool same_tuning(const WalkTuning& a, const WalkTuning& b) {
-  return a.block == b.block && a.walk_budget == b.walk_budget && a.search_batch == b.search_batch && a.chunk == b.chunk;
+  return a.block == b.block && a.walk_budget == b.walk_budget && a.search_batch == b.search_batch &&
+         a.chunk == b.chunk && a.head_lanes == b.head_lanes && a.head_per_mille == b.head_per_mille;
 }
 
 }  // namespace
@@ -163,16 +164,23 @@ std::vector<RigTrial> Rig::autotune(const bsx_device_runs* runs, int64_t t_end_n
   static const int kBudgets[] = {4, 8, 16, 32, 64};
   static const int kBatches[] = {1, 4, 8, 16, 32, 64};
   static const int kChunks[] = {0, 1, 2, 4, 8, 16};
+  // The head (lanes, thousandths of the order): none, or the heaviest pixels in groups of 16 or 32.
+  static const int kHeads[][2] = {{64, 0}, {16, 8}, {32, 8}, {32, 32}};
   WalkTuning best = chain_.tuning();
   double best_ms = time_walk(best);
   for (int pass = 0; pass < 2; ++pass) {
-    for (int axis = 0; axis < 4; ++axis) {
+    for (int axis = 0; axis < 5; ++axis) {
       const int* vals = axis == 0 ? kBlocks : axis == 1 ? kBudgets : axis == 2 ? kBatches : kChunks;
-      const int nv = axis == 0 ? 3 : axis == 1 ? 5 : 6;
+      const int nv = axis == 0 ? 3 : axis == 1 ? 5 : axis == 4 ? 4 : 6;
       const WalkTuning base = best;
       for (int k = 0; k < nv; ++k) {
         WalkTuning t = base;
-        (axis == 0 ? t.block : axis == 1 ? t.walk_budget : axis == 2 ? t.search_batch : t.chunk) = vals[k];
+        if (axis == 4) {
+          t.head_lanes = kHeads[k][0];
+          t.head_per_mille = kHeads[k][1];
+        } else {
+          (axis == 0 ? t.block : axis == 1 ? t.walk_budget : axis == 2 ? t.search_batch : t.chunk) = vals[k];
+        }
         const double ms = time_walk(t);
         if (ms < best_ms) {
           best_ms = ms;

@@ -10,6 +10,9 @@ import numpy as np
 
 from . import _abi
 
+# A walk launch shape's fields (bsx_launch_policy without the event capacity).
+TUNING_KEYS = ("block_size", "walk_budget", "search_batch", "queue_chunk", "head_lanes", "head_per_mille")
+
 _ENUMS = {
     "photoreceptor": _abi.PHOTORECEPTOR,
     "reset_mode": _abi.RESET,
@@ -281,8 +284,10 @@ class SensorModel:
     def reset_stats(self):
         _abi.check(self._lib.bsx_sensor_reset_stats(self._h))
 
-    def set_launch_policy(self, event_capacity=0, block_size=0, walk_budget=0, search_batch=0, queue_chunk=0):
-        pol = _abi.LaunchPolicy(event_capacity, block_size, walk_budget, search_batch, queue_chunk)
+    def set_launch_policy(self, event_capacity=0, block_size=0, walk_budget=0, search_batch=0, queue_chunk=0,
+                          head_lanes=0, head_per_mille=0):
+        pol = _abi.LaunchPolicy(event_capacity, block_size, walk_budget, search_batch, queue_chunk, head_lanes,
+                                head_per_mille)
         _abi.check(self._lib.bsx_sensor_set_launch_policy(self._h, C.byref(pol)))
 
     def _info(self):
@@ -398,14 +403,14 @@ class Rig:
         _abi.check(self._lib.bsx_rig_advance_device(self._h, arr, int(t_end_ns), C.c_void_p(st), out))
         return [(o.data, o.count) for o in out]
 
-    def set_tuning(self, block_size=0, walk_budget=0, search_batch=0, queue_chunk=0):
-        pol = _abi.LaunchPolicy(0, block_size, walk_budget, search_batch, queue_chunk)
+    def set_tuning(self, block_size=0, walk_budget=0, search_batch=0, queue_chunk=0, head_lanes=0, head_per_mille=0):
+        pol = _abi.LaunchPolicy(0, block_size, walk_budget, search_batch, queue_chunk, head_lanes, head_per_mille)
         _abi.check(self._lib.bsx_rig_set_tuning(self._h, C.byref(pol)))
 
     def tuning(self):
         pol = _abi.LaunchPolicy()
         _abi.check(self._lib.bsx_rig_get_tuning(self._h, C.byref(pol)))
-        return {k: getattr(pol, k) for k in ("block_size", "walk_budget", "search_batch", "queue_chunk")}
+        return {k: getattr(pol, k) for k in TUNING_KEYS}
 
     def set_graph(self, on):
         _abi.check(self._lib.bsx_rig_set_graph(self._h, 1 if on else 0))
@@ -426,5 +431,4 @@ class Rig:
         st = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
         _abi.check(self._lib.bsx_rig_autotune(self._h, arr, int(t_end_ns), C.c_void_p(st), int(reps), tun, ms, cap,
                                               C.byref(n)))
-        keys = ("block_size", "walk_budget", "search_batch", "queue_chunk")
-        return [({k: getattr(tun[i], k) for k in keys}, ms[i]) for i in range(min(n.value, cap))]
+        return [({k: getattr(tun[i], k) for k in TUNING_KEYS}, ms[i]) for i in range(min(n.value, cap))]

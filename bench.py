@@ -151,7 +151,8 @@ def bench_rig(args):
     # The same windows camera by camera (a fresh set of cameras with the same tuning).
     solo = cameras()
     for m in solo:
-        m.set_launch_policy(**{k: (v if v != 0 or k != "queue_chunk" else -1) for k, v in tuning.items()})
+        m.set_launch_policy(**{k: (v if v != 0 or k not in ("queue_chunk", "head_per_mille") else -1)
+                               for k, v in tuning.items()})
     for i in range(args.warmup):
         for c, m in enumerate(solo):
             m.advance_device_runs(runs[i][c], (i + 1) * WINDOW)

@@ -181,6 +181,10 @@ typedef struct bsx_launch_policy {
   int32_t search_batch; /* waiting lanes that start a joint exact-search pass, [1, 64] (default 8) */
   int32_t queue_chunk;  /* 64-pixel groups per grab of a persistent work queue, [0, 64]; 0 = no queue;
                            -1 sets 0 (default 0) */
+  int32_t head_lanes;     /* pixels a walk group takes at the head of the heavy-first pixel order, [1, 64]
+                             (default 64) */
+  int32_t head_per_mille; /* the head's share of the order, in thousandths of the pixels, [0, 1000]; -1 sets
+                             0 (default 0: every group takes 64 pixels) */
 } bsx_launch_policy;
 
 typedef struct bsx_sensor_info_t {

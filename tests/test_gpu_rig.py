@@ -131,6 +131,24 @@ def test_rig_instant_without_graph_and_persistent_queue():
     assert rig.stats()["graph_launches"] == 0
 
 
+@pytest.mark.parametrize("graph,tuning", [
+    (True, dict(head_lanes=16, head_per_mille=250)),
+    (False, dict(head_lanes=8, head_per_mille=500, queue_chunk=2, block_size=128)),
+])
+def test_rig_instant_head_groups(graph, tuning):
+    """The heaviest pixels of the order walked in groups of fewer than 64 (idle lanes in those waves)
+    give every camera the events, state and counters of advancing it alone in 64-pixel groups."""
+    rig, _ = run_pair("instant", SHAPES, graph=graph, tuning=tuning)
+    got = rig.tuning()
+    assert got["head_lanes"] == tuning["head_lanes"] and got["head_per_mille"] == tuning["head_per_mille"]
+    rig.set_tuning(head_per_mille=-1)
+    assert rig.tuning()["head_per_mille"] == 0
+
+
+def test_rig_cascade_head_groups():
+    run_pair("cascade", [(24, 16), (17, 9)], calls=2, tuning=dict(head_lanes=32, head_per_mille=300))
+
+
 def test_rig_cameras_at_different_times_and_overflow():
     rig, _ = run_pair("instant", SHAPES[:3], t0s=[0, 2 ** 23 + 777, 3 * 2 ** 22 - 5], caps=[32, 0, 16],
                       tuning=dict(queue_chunk=1))
